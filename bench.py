@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the Cholesky J/K Fock build on one MI355X.
+
+Workload (one "step" = one J+K build): synthetic single FP64 J/K build,
+nbas=512, nchol=2048, nocc=64, D = C C^T -- Cholesky vectors resident in HBM
+before the timed region, as in an SCF iteration.  A step runs
+    J = two streaming passes over L      (HBM-bound)
+    K = half-transform + SYRK on MFMA     (MFMA-bound)
+    F = h + J - 1/2 K                     (FP64 assembly)
+through the C ABI.  Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--nbas", type=int, default=512)
+    p.add_argument("--nchol", type=int, default=2048)
+    p.add_argument("--nocc", type=int, default=64)
+    p.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample-vectors", type=int, default=256)
+    return p.parse_args()
+
+
+def cpu_baseline(n, M, o, seed, sample):
+    """The CPU oracle (C, OpenMP) on the first `sample` vectors of the same
+    workload, scaled to the full nchol: builds/s of CHO_FSCF-shaped work."""
+    import numpy as np
+
+    import oracle
+    from openmolcas_amd.synthetic import chol_vectors_np, fold, npair, orbitals_np
+
+    ms = min(sample, M)
+    L = chol_vectors_np(n, ms, seed)
+    C = orbitals_np(n, o, seed + 1)
+    dlt = fold(C @ C.T)
+    st = oracle.CholStore.full(L)
+    flt = np.zeros((1, npair(n)))
+    oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=64)  # warm
+    t0 = time.perf_counter()
+    oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=64)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(1.0 / (dt * M / ms), 6),
+        "unit": "builds/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle CHO_FSCF restatement (C, OpenMP) on {ms} of {M} vectors, "
+                  f"nbas={n} nocc={o}, {dt:.2f} s, scaled x{M / ms:.1f}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the J/K build "
+              "shards over Cholesky vectors with one all-reduce of the LT Fock matrix, which is "
+              "not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from openmolcas_amd.fock import JKContext
+    from openmolcas_amd.synthetic import chol_vectors_torch, npair, orbitals_torch
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n, M, o = args.nbas, args.nchol, args.nocc
+    npr = npair(n)
+
+    L = chol_vectors_torch(n, M, args.seed, device=dev)
+    C = orbitals_torch(n, o, args.seed + 1, device=dev)
+    P = C @ C.T
+    ii, jj = torch.tril_indices(n, n, device=dev)
+    dlt = (P[ii, jj] * torch.where(ii == jj, 1.0, 2.0).to(torch.float64)).unsqueeze(0).contiguous()
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed + 2)
+    h = torch.randn((n, n), generator=g, dtype=torch.float64, device=dev)
+    h = ((h + h.T) / (2.0 * n))[ii, jj].unsqueeze(0).contiguous()
+
+    ctx = JKContext(n, M, device=dev)
+    ctx.upload_packed(L)
+    del L
+    torch.cuda.synchronize()
+    F = torch.empty((1, npr), dtype=torch.float64, device=dev)
+    jsrc, fj, fk = [0], [1.0], [-0.5]
+
+    def step():
+        J = ctx.coulomb(dlt)
+        K = ctx.exchange([C], precision=args.precision)
+        F.copy_(h)
+        ctx.fock_update(F, J, jsrc, fj, K, fk)
+        return J, K
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases = ctx.phase_times()
+    ctx.set_profiling(False)
+    elapsed = t1 - t0
+    ms_step = 1000.0 * elapsed / args.steps
+
+    # algorithmic work (per build)
+    bytes_j = 2.0 * M * npr * 8 + 2.0 * npr * 8 + 2.0 * M * 8
+    flops_ht = 2.0 * M * o * n * n
+    flops_syrk = 1.0 * M * o * n * (n + 1)
+    peak = PEAK_TFLOPS[args.precision]
+
+    def avg(name):
+        ms, calls = phases[name]
+        return ms / calls if calls else float("nan"), calls
+
+    ht_ms, ht_calls = avg("half_transform")
+    sy_ms, sy_calls = avg("exchange")
+    jv_ms, _ = avg("coulomb_v")
+    jj_ms, _ = avg("coulomb_j")
+    ht_per_launch = flops_ht / (ht_calls / args.steps)
+    sy_per_launch = flops_syrk / (sy_calls / args.steps)
+    ht_tf = ht_per_launch / (ht_ms * 1e-3) / 1e12
+    sy_tf = sy_per_launch / (sy_ms * 1e-3) / 1e12
+    j_gbs = bytes_j / ((jv_ms + jj_ms) * 1e-3) / 1e9
+    k_ms = phases["half_transform"][0] / args.steps + phases["exchange"][0] / args.steps
+    k_tf = (flops_ht + flops_syrk) / (k_ms * 1e-3) / 1e12
+    # dominant kernel: the larger of the two exchange kernels by time
+    if ht_ms * ht_calls >= sy_ms * sy_calls:
+        dom = ("k_ht_f64 (half-transform X = C^T L)", ht_tf)
+    else:
+        dom = ("k_syrk_f64 (K = sum X^T X)", sy_tf)
+
+    out = {
+        "metric": "J+K builds/sec (HBM GB/s on J, MFMA utilisation on K)",
+        "value": round(args.steps / elapsed, 4),
+        "unit": "builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else args.precision,
+        "data": "synthetic (seeded N(0,1) symmetric vectors, norms 1..1e-6; QR orbitals)",
+        "config": {
+            "workload": f"synthetic single J/K build nbas={n} nchol={M} nocc={o}",
+            "nbas": n, "nchol": M, "nocc": o, "precision": args.precision,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": dom[0],
+            "achieved": round(dom[1], 3),
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": round(dom[1] / peak, 4),
+            "traffic": None,
+        },
+        "kernels": {
+            "coulomb_J_GBs": round(j_gbs, 1),
+            "coulomb_J_frac_hbm": round(j_gbs / HBM_PEAK_GBS, 4),
+            "coulomb_ms": round(jv_ms + jj_ms, 4),
+            "half_transform_TFs": round(ht_tf, 2),
+            "half_transform_ms": round(ht_ms, 4),
+            "syrk_TFs": round(sy_tf, 2),
+            "syrk_ms": round(sy_ms, 4),
+            "exchange_TFs": round(k_tf, 2),
+            "exchange_frac_peak": round(k_tf / peak, 4),
+            "phase_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in phases.items()},
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n, M, o, args.seed, args.cpu_sample_vectors)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

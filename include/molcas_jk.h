@@ -1,0 +1,178 @@
+/*
+ * molcas_jk.h -- C ABI of the MI355X (gfx950) Cholesky J/K Fock build.
+ *
+ * Drop-in for OpenMolcas's Cholesky-vector Coulomb/exchange build
+ * (CHO_FSCF / CHO_FMCSCF in src/fock_util, fed by the reduced-set vectors of
+ * src/cholesky_util).  All arguments are plain pointers and sizes; every entry
+ * point returns an int rc (0 = success) and never aborts.  On a nonzero rc,
+ * mjk_last_error() describes the failure.
+ *
+ * Storage conventions (nSym = 1, JSYM = 1, the C1 case):
+ *   - LT packed: element (a,b), a >= b, 0-based, at a*(a+1)/2 + b
+ *     (src/misc_util/index_functions.F90:138-146).
+ *   - Coulomb densities are folded (off-diagonal doubled,
+ *     src/misc_util/fold.F90:14); Fock matrices are plain LT.
+ *   - Orbital matrices follow Porb(d)%SB(1)%A2(nOcc,nBas)
+ *     (src/system_util/data_structures.F90:62-70): element (k,a) at k + a*nOcc.
+ *
+ * Threading: one context per thread; calls on one context are serialised by
+ * the caller, as the reference's single-threaded callers do.
+ */
+#ifndef MOLCAS_JK_H
+#define MOLCAS_JK_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mjk_context mjk_context;
+
+/* Return codes.  33 and 77 keep the reference's meanings
+ * (src/fock_util/cho_fscf.F90:147-155 and :193-196). */
+enum {
+  MJK_OK = 0,
+  MJK_ERR_ARG = 1,        /* invalid argument / shape */
+  MJK_ERR_HIP = 2,        /* HIP runtime failure */
+  MJK_ERR_STATE = 3,      /* call out of order (e.g. no reduced set) */
+  MJK_ERR_NOMEM = 33,     /* device memory exhausted */
+  MJK_ERR_SHORT_READ = 77 /* vectors requested that were never uploaded */
+};
+
+/* Arithmetic of the exchange (K) contraction.  Coulomb, reductions over J and
+ * the Fock/energy assembly are always FP64. */
+enum {
+  MJK_PREC_FP64 = 0,   /* FP64 MFMA */
+  MJK_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulate, FP64 reduction over J */
+  MJK_PREC_BF16X3 = 2  /* split-BF16, 3 products, FP32 accumulate */
+};
+
+/* Phases reported by mjk_phase_times(). */
+enum {
+  MJK_PHASE_COULOMB_V = 0, /* V_J = sum_pq L^J_pq D_pq           */
+  MJK_PHASE_COULOMB_J = 1, /* J_pq = sum_J L^J_pq V_J            */
+  MJK_PHASE_HALF_TRANSFORM = 2, /* X^J = C^T L^J (MFMA)          */
+  MJK_PHASE_EXCHANGE = 3,  /* K = sum_J X^J^T X^J (MFMA)         */
+  MJK_PHASE_REDUCE = 4,    /* partial-sum reductions            */
+  MJK_PHASE_ASSEMBLY = 5,  /* Fock update / energy               */
+  MJK_PHASE_PACK = 6,      /* density / orbital staging          */
+  MJK_NUM_PHASES = 7
+};
+
+/* ---- lifecycle --------------------------------------------------------- */
+
+/* Creates a context on the current HIP device for nbas basis functions and
+ * room for nchol Cholesky vectors kept resident in HBM.  Replaces the vector
+ * buffer set up by Cho_X_Init (src/cholesky_util/cho_x_init.F90:55-469). */
+int mjk_create(mjk_context **ctx, int32_t nbas, int64_t nchol);
+
+/* Frees every device allocation of the context.  Cho_X_Final counterpart
+ * (called at src/scf/wfctl_scf.F90:1202). */
+int mjk_destroy(mjk_context *ctx);
+
+/* Text of the last error on this context ("" if none). */
+const char *mjk_last_error(const mjk_context *ctx);
+
+/* Queues all later device work of the context on `hip_stream` (a
+ * hipStream_t; NULL = the default stream).  A new context uses a stream of
+ * its own.  Library-internal design: no reference counterpart. */
+int mjk_set_stream(mjk_context *ctx, void *hip_stream);
+
+/* Bytes of device memory one chunk of half-transformed vectors may use. */
+int mjk_set_x_budget(mjk_context *ctx, int64_t bytes);
+
+/* ---- vector store (src/cholesky_util vector API) ----------------------- */
+
+/* Declares reduced set `jred` (0-based) with nrs pairs; rs_to_lt[r] is the
+ * 0-based LT index of reduced-set position r (host array).  Replaces the
+ * index state that Cho_X_SetRed / CHO_GETRED load
+ * (src/cholesky_util/cho_x_setred.F90:43-70, cho_getred.F90:12-78) and that
+ * swap_full2rs/swap_rs2full consume (src/fock_util/swap_full2rs.F90:37-46). */
+int mjk_set_reduced_set(mjk_context *ctx, int32_t jred, int64_t nrs,
+                        const int64_t *rs_to_lt);
+
+/* Uploads vectors jvec..jvec+jnum-1 (0-based) stored on reduced set jred as
+ * lrs(nrs, jnum), column-major -- the Lrs buffer CHO_VECRD fills
+ * (src/cholesky_util/cho_vecrd.F90:12-64).  lrs is a host pointer unless
+ * on_device != 0. */
+int mjk_upload_vectors(mjk_context *ctx, int32_t jred, int64_t jvec, int64_t jnum,
+                       const double *lrs, int32_t on_device);
+
+/* ---- reference-shaped drop-ins (host arrays, synchronous) -------------- */
+
+/* CHO_FSCF (src/fock_util/cho_fscf.F90:14), nSym = 1:
+ *   FLT(d) += J[DLT(1)] - ExFac * sum_{J,k} X(k,J,a) X(k,J,b),  d < nden,
+ * nden in {1,2}; porb holds the nden orbital matrices back to back, density d
+ * with nforb[d]+niorb[d] rows.  Called from src/scf/choscf_drv.F90:197,420. */
+int mjk_cho_fscf(mjk_context *ctx, int32_t nden, double *flt, const int32_t *nforb,
+                 const int32_t *niorb, const double *porb, const double *dlt,
+                 double exfac, int32_t precision);
+
+/* CHO_FMCSCF (src/fock_util/cho_fmcscf.F90:14), nSym = 1:
+ *   FI += J[DI] + FactXI * sum X_I X_I^T      (porb1: nforb+niorb rows)
+ *   FA += J[DA] - 1/2 * sum X_A X_A^T         (porb2: nchm rows, if do_active)
+ *   W_PWXY(p,w,xy) = sum_J (pw|J)(xy|J)        (porb3: naorb rows)
+ * flt = [FI, FA], dlt = [DI, DA] (folded), cmo is CMO(n,n) column-major and
+ * p runs over norb MOs after nfro frozen ones; w_pwxy may be NULL when
+ * naorb == 0.  Called from src/fock_util/cho_cas_drv.F90:277. */
+int mjk_cho_fmcscf(mjk_context *ctx, double *flt, int32_t nforb, int32_t niorb,
+                   int32_t naorb, double factxi, const double *dlt, int32_t do_active,
+                   const double *porb1, int32_t nchm, const double *porb2,
+                   const double *porb3, double *w_pwxy, const double *cmo,
+                   int32_t nfro, int32_t norb, double exfac, int32_t precision);
+
+/* ---- device engine (device pointers, asynchronous on the context stream) */
+
+/* Coulomb for nd folded LT densities dlt(d*ldd + p): jlt(d*ldj + p) =
+ * sum_J L^J_p V_J(d), V_J(d) = sum_q L^J_q dlt(d,q); two streaming passes over
+ * the resident vectors.  v_out (optional, M x nd) receives V.
+ * Replaces the DGEMV pair of src/fock_util/cho_fscf.F90:208,215. */
+int mjk_coulomb(mjk_context *ctx, int32_t nd, const double *dlt, int64_t ldd,
+                double *jlt, int64_t ldj, double *v_out);
+
+/* Exchange for nk orbital sets: klt(d*ldk + LT(a,b)) = sum_{J,k}
+ * X_d(k,J,a) X_d(k,J,b), X_d(k,J,a) = sum_b C_d(k,b) L^J_ab.  corb is a host
+ * array of nk device pointers, corb[d] holding nocc[d] x nbas (orbital index
+ * fastest).  Replaces Cho_X_getVtra + DGEMM_Tri
+ * (src/fock_util/cho_fscf.F90:250,276). */
+int mjk_exchange(mjk_context *ctx, int32_t nk, const int32_t *nocc,
+                 const double *const *corb, double *klt, int64_t ldk,
+                 int32_t precision);
+
+/* flt(d) += fj[d] * jlt(jsrc[d]) + fk[d] * klt(d), d < nd (jsrc[d] < 0: no
+ * J term).  The LT update of src/fock_util/cho_fscf.F90:276,301. */
+int mjk_fock_update(mjk_context *ctx, int32_t nd, double *flt, int64_t ldf,
+                    const double *jlt, int64_t ldj, const int32_t *jsrc,
+                    const double *fj, const double *klt, int64_t ldk,
+                    const double *fk);
+
+/* e_dev[0] = sum_LT h*dlt + 1/2 sum_LT g*dlt (FP64), the energy expression
+ * of src/scf/eneclc.F90:58,77,104 without E_nuc. */
+int mjk_energy(mjk_context *ctx, const double *h, const double *dlt,
+               const double *g, double *e_dev);
+
+/* (pw|xy) integrals from the resident vectors (device pointers):
+ *   w(p + norb*(w + naorb*xy)) = sum_J (pw|J)(xy|J),
+ * (pw|J) = sum_ab CMO(a,nfro+p) C3(w,b) L^J_ab, (xy|J) = sum_ab C3(x,a) C3(y,b)
+ * L^J_ab with xy = y(y+1)/2 + x, x <= y.  porb3 is C3 (naorb x nbas, orbital
+ * index fastest), cmo is CMO(nbas,nbas) column-major.  Replaces the Lxy /
+ * CHO_eval_waxy steps of src/fock_util/cho_fmcscf.F90:403-487. */
+int mjk_puvx(mjk_context *ctx, int32_t naorb, const double *porb3, const double *cmo,
+             int32_t nfro, int32_t norb, double *w);
+
+/* Waits for all work queued on the context stream. */
+int mjk_sync(mjk_context *ctx);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Turns per-phase HIP-event timing on (1) or off (0); resets the counters. */
+int mjk_set_profiling(mjk_context *ctx, int32_t on);
+
+/* Copies accumulated milliseconds and launch counts per phase (arrays of
+ * MJK_NUM_PHASES entries; either may be NULL). */
+int mjk_phase_times(mjk_context *ctx, double *ms, int64_t *calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOLCAS_JK_H */

@@ -1,0 +1,456 @@
+// C ABI: context lifecycle, resident vector store, reference-shaped drop-ins
+// and the device engine entry points.
+#include <cstdio>
+#include <cstring>
+
+#include "mjk_internal.h"
+
+namespace mjk {
+
+int set_error(Context *c, int rc, const std::string &msg) {
+  if (c) c->err = msg;
+  return rc;
+}
+
+int hip_check(Context *c, hipError_t e, const char *what) {
+  if (e == hipSuccess) return MJK_OK;
+  const int rc = (e == hipErrorOutOfMemory) ? MJK_ERR_NOMEM : MJK_ERR_HIP;
+  return set_error(c, rc, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensure(Context *c, DevBuf &b, size_t bytes) {
+  if (bytes <= b.bytes && b.ptr) return MJK_OK;
+  if (b.ptr) {
+    // queued work may still read the old buffer
+    int rc = hip_check(c, hipStreamSynchronize(c->stream), "sync before realloc");
+    if (rc) return rc;
+    hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  const size_t want = std::max<size_t>(bytes, 256);
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(c, MJK_ERR_NOMEM,
+                     "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  b.ptr = p;
+  b.bytes = want;
+  return MJK_OK;
+}
+
+static hipEvent_t take_event(Context *c, int *idx) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev_pool.push_back(e);
+  }
+  *idx = int(c->ev_used);
+  return c->ev_pool[c->ev_used++];
+}
+
+void phase_begin(Context *c) {
+  if (!c->profile) return;
+  int idx;
+  hipEvent_t e = take_event(c, &idx);
+  if (!e) return;
+  hipEventRecord(e, c->stream);
+  c->ev_open = idx;
+}
+
+void phase_end(Context *c, int phase) {
+  if (!c->profile || c->ev_open < 0) return;
+  int idx;
+  hipEvent_t e = take_event(c, &idx);
+  if (!e) return;
+  hipEventRecord(e, c->stream);
+  c->pending.push_back({phase, c->ev_open, idx});
+  c->ev_open = -1;
+  if (c->ev_used > 8192) resolve_phases(c);  // bound the pool
+}
+
+int resolve_phases(Context *c) {
+  if (c->pending.empty()) {
+    c->ev_used = 0;
+    return MJK_OK;
+  }
+  int rc = hip_check(c, hipEventSynchronize(c->ev_pool[c->pending.back().ev_end]), "event sync");
+  if (rc) return rc;
+  for (const auto &p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_pool[p.ev_begin], c->ev_pool[p.ev_end]) == hipSuccess) {
+      c->phase_ms[p.phase] += ms;
+      c->phase_calls[p.phase] += 1;
+    }
+  }
+  c->pending.clear();
+  c->ev_used = 0;
+  c->ev_open = -1;
+  return MJK_OK;
+}
+
+namespace {
+
+// L[(jvec + j)*ldl + map[r]] = src[r + j*nrs]
+__global__ void k_scatter_rs(const double *__restrict__ src, int64_t nrs, int64_t jnum,
+                             const int32_t *__restrict__ map, double *__restrict__ L, int64_t ldl,
+                             int64_t jvec) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nrs * jnum) return;
+  const int64_t j = i / nrs, r = i % nrs;
+  L[(jvec + j) * ldl + map[r]] = src[i];
+}
+
+void free_buf(DevBuf &b) {
+  if (b.ptr) hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+}  // namespace
+}  // namespace mjk
+
+using namespace mjk;
+
+extern "C" {
+
+int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
+  if (!out) return MJK_ERR_ARG;
+  *out = nullptr;
+  if (nbas < 1 || nbas > 46340 || nchol < 1) return MJK_ERR_ARG;
+  auto *c = new (std::nothrow) mjk_context();
+  if (!c) return MJK_ERR_NOMEM;
+  c->n = nbas;
+  c->np = int64_t(nbas) * (nbas + 1) / 2;
+  c->ldl = round_up(c->np, kRowAlign);
+  c->n_pad = int32_t(round_up(nbas, 128));
+  c->nchol = nchol;
+  c->loaded.assign(size_t(nchol), 0);
+  int rc;
+  if ((rc = hip_check(c, hipGetDevice(&c->device), "hipGetDevice")) ||
+      (rc = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking),
+                      "stream create"))) {
+    delete c;
+    return rc;
+  }
+  c->own_stream = true;
+  const size_t lbytes = sizeof(double) * size_t(nchol) * size_t(c->ldl);
+  void *p = nullptr;
+  if (hipMalloc(&p, lbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    hipStreamDestroy(c->stream);
+    delete c;
+    return MJK_ERR_NOMEM;
+  }
+  c->L = static_cast<double *>(p);
+  if ((rc = hip_check(c, hipMemsetAsync(c->L, 0, lbytes, c->stream), "zero L")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) {
+    hipFree(c->L);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return MJK_OK;
+}
+
+int mjk_destroy(mjk_context *c) {
+  if (!c) return MJK_OK;
+  hipStreamSynchronize(c->stream);
+  for (DevBuf *b : {&c->ws_d, &c->ws_vpart, &c->ws_v, &c->ws_jpart, &c->ws_cbank, &c->ws_x,
+                    &c->ws_kpart, &c->ws_stage, &c->ws_j, &c->ws_k, &c->ws_red, &c->ws_misc})
+    free_buf(*b);
+  for (auto &r : c->rsets)
+    if (r.d_map) hipFree(r.d_map);
+  if (c->L) hipFree(c->L);
+  for (auto e : c->ev_pool) hipEventDestroy(e);
+  if (c->own_stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MJK_OK;
+}
+
+const char *mjk_last_error(const mjk_context *c) { return c ? c->err.c_str() : "null context"; }
+
+int mjk_set_stream(mjk_context *c, void *s) {
+  if (!c) return MJK_ERR_ARG;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  if (c->own_stream) {
+    (void)hipStreamDestroy(c->stream);
+    c->own_stream = false;
+  }
+  c->stream = static_cast<hipStream_t>(s);  // NULL: the default stream
+  return MJK_OK;
+}
+
+int mjk_set_x_budget(mjk_context *c, int64_t bytes) {
+  if (!c || bytes < 1) return MJK_ERR_ARG;
+  c->x_budget = bytes;
+  return MJK_OK;
+}
+
+int mjk_set_reduced_set(mjk_context *c, int32_t jred, int64_t nrs, const int64_t *map) {
+  if (!c) return MJK_ERR_ARG;
+  if (jred < 0 || jred > 1 << 20 || nrs < 0 || nrs > c->np || (nrs > 0 && !map))
+    return set_error(c, MJK_ERR_ARG, "set_reduced_set: bad jred/nrs/map");
+  std::vector<int32_t> m32(static_cast<size_t>(nrs));
+  bool ident = (nrs == c->np);
+  for (int64_t r = 0; r < nrs; ++r) {
+    if (map[r] < 0 || map[r] >= c->np)
+      return set_error(c, MJK_ERR_ARG, "set_reduced_set: LT index out of range");
+    m32[size_t(r)] = int32_t(map[r]);
+    ident = ident && (map[r] == r);
+  }
+  if (size_t(jred) >= c->rsets.size()) c->rsets.resize(size_t(jred) + 1);
+  ReducedSet &rs = c->rsets[size_t(jred)];
+  if (rs.d_map) {
+    hipStreamSynchronize(c->stream);
+    hipFree(rs.d_map);
+    rs.d_map = nullptr;
+  }
+  rs.nrs = nrs;
+  rs.identity = ident;
+  if (nrs > 0) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(int32_t) * size_t(nrs)) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(c, MJK_ERR_NOMEM, "set_reduced_set: allocation failed");
+    }
+    rs.d_map = static_cast<int32_t *>(p);
+    int rc = hip_check(c, hipMemcpy(rs.d_map, m32.data(), sizeof(int32_t) * size_t(nrs),
+                                    hipMemcpyHostToDevice),
+                       "copy map");
+    if (rc) return rc;
+  }
+  return MJK_OK;
+}
+
+int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum, const double *lrs,
+                       int32_t on_device) {
+  if (!c) return MJK_ERR_ARG;
+  if (jred < 0 || size_t(jred) >= c->rsets.size())
+    return set_error(c, MJK_ERR_STATE, "upload_vectors: reduced set not declared");
+  if (jvec < 0 || jnum < 0 || jvec + jnum > c->nchol)
+    return set_error(c, MJK_ERR_ARG, "upload_vectors: vector range outside the context capacity");
+  if (jnum == 0) return MJK_OK;
+  if (!lrs) return set_error(c, MJK_ERR_ARG, "upload_vectors: null lrs");
+  const ReducedSet &rs = c->rsets[size_t(jred)];
+  const int64_t nrs = rs.nrs;
+  int rc;
+  double *dst = c->L + jvec * c->ldl;
+  if (rs.identity) {
+    rc = hip_check(c, hipMemcpy2DAsync(dst, sizeof(double) * c->ldl, lrs, sizeof(double) * nrs,
+                                       sizeof(double) * nrs, size_t(jnum),
+                                       on_device ? hipMemcpyDeviceToDevice
+                                                 : hipMemcpyHostToDevice,
+                                       c->stream),
+                   "copy vectors");
+    if (rc) return rc;
+  } else {
+    if ((rc = hip_check(c, hipMemsetAsync(dst, 0, sizeof(double) * c->ldl * jnum, c->stream),
+                        "zero vectors")))
+      return rc;
+    if (nrs > 0) {
+      const double *src = lrs;
+      // stage host data in bounded pieces
+      const int64_t per = on_device ? jnum : std::max<int64_t>(1, (int64_t(256) << 20) / (8 * nrs));
+      for (int64_t j0 = 0; j0 < jnum; j0 += per) {
+        const int64_t jn = std::min(per, jnum - j0);
+        if (!on_device) {
+          if ((rc = ensure(c, c->ws_stage, sizeof(double) * nrs * jn))) return rc;
+          if ((rc = hip_check(c, hipMemcpyAsync(c->ws_stage.ptr, lrs + j0 * nrs,
+                                                sizeof(double) * nrs * jn, hipMemcpyHostToDevice,
+                                                c->stream),
+                              "stage vectors")))
+            return rc;
+          src = static_cast<const double *>(c->ws_stage.ptr);
+        } else {
+          src = lrs + j0 * nrs;
+        }
+        hipLaunchKernelGGL(k_scatter_rs, dim3(unsigned(ceil_div(nrs * jn, 256))), dim3(256), 0,
+                           c->stream, src, nrs, jn, rs.d_map, c->L, c->ldl, jvec + j0);
+        if (!on_device && (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))
+          return rc;
+      }
+    }
+  }
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "upload sync"))) return rc;
+  for (int64_t j = jvec; j < jvec + jnum; ++j) {
+    if (!c->loaded[size_t(j)]) {
+      c->loaded[size_t(j)] = 1;
+      ++c->nloaded;
+    }
+  }
+  return MJK_OK;
+}
+
+int mjk_coulomb(mjk_context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
+                int64_t ldj, double *v_out) {
+  if (!c) return MJK_ERR_ARG;
+  if (!dlt || !jlt) return set_error(c, MJK_ERR_ARG, "coulomb: null pointer");
+  return coulomb(c, nd, dlt, ldd, jlt, ldj, v_out);
+}
+
+int mjk_exchange(mjk_context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                 double *klt, int64_t ldk, int32_t precision) {
+  if (!c) return MJK_ERR_ARG;
+  if (!nocc || !corb || !klt) return set_error(c, MJK_ERR_ARG, "exchange: null pointer");
+  for (int d = 0; d < nk; ++d)
+    if (nocc[d] > 0 && !corb[d]) return set_error(c, MJK_ERR_ARG, "exchange: null orbital matrix");
+  return exchange(c, nk, nocc, corb, klt, ldk, precision);
+}
+
+int mjk_fock_update(mjk_context *c, int32_t nd, double *flt, int64_t ldf, const double *jlt,
+                    int64_t ldj, const int32_t *jsrc, const double *fj, const double *klt,
+                    int64_t ldk, const double *fk) {
+  if (!c) return MJK_ERR_ARG;
+  if (!flt) return set_error(c, MJK_ERR_ARG, "fock_update: null flt");
+  return fock_update(c, nd, flt, ldf, jlt, ldj, jsrc, fj, klt, ldk, fk);
+}
+
+int mjk_energy(mjk_context *c, const double *h, const double *dlt, const double *g, double *e) {
+  if (!c) return MJK_ERR_ARG;
+  return energy(c, h, dlt, g, e);
+}
+
+int mjk_sync(mjk_context *c) {
+  if (!c) return MJK_ERR_ARG;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int mjk_set_profiling(mjk_context *c, int32_t on) {
+  if (!c) return MJK_ERR_ARG;
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  c->pending.clear();
+  c->ev_used = 0;
+  c->ev_open = -1;
+  c->profile = on != 0;
+  for (int i = 0; i < kNumPhases; ++i) {
+    c->phase_ms[i] = 0.0;
+    c->phase_calls[i] = 0;
+  }
+  return MJK_OK;
+}
+
+int mjk_phase_times(mjk_context *c, double *ms, int64_t *calls) {
+  if (!c) return MJK_ERR_ARG;
+  int rc = resolve_phases(c);
+  if (rc) return rc;
+  for (int i = 0; i < kNumPhases; ++i) {
+    if (ms) ms[i] = c->phase_ms[i];
+    if (calls) calls[i] = c->phase_calls[i];
+  }
+  return MJK_OK;
+}
+
+// ---- reference-shaped drop-ins -------------------------------------------
+
+int mjk_cho_fscf(mjk_context *c, int32_t nden, double *flt, const int32_t *nforb,
+                 const int32_t *niorb, const double *porb, const double *dlt, double exfac,
+                 int32_t precision) {
+  if (!c) return MJK_ERR_ARG;
+  if (nden != 1 && nden != 2)
+    return set_error(c, MJK_ERR_ARG, "cho_fscf: invalid parameter nDen");  // cho_fscf.F90:75-78
+  if (!flt || !nforb || !niorb || !dlt) return set_error(c, MJK_ERR_ARG, "cho_fscf: null pointer");
+  const int64_t np = c->np;
+  int32_t nocc[2] = {0, 0};
+  int64_t tot = 0;
+  for (int d = 0; d < nden; ++d) {
+    if (nforb[d] < 0 || niorb[d] < 0) return set_error(c, MJK_ERR_ARG, "cho_fscf: negative nOcc");
+    nocc[d] = nforb[d] + niorb[d];
+    tot += int64_t(nocc[d]) * c->n;
+  }
+  if (tot > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_fscf: null porb");
+  // device scratch: D | J | K(nden) | F(nden) | orbitals
+  const size_t need = sizeof(double) * size_t(np * (2 + 2 * nden) + tot);
+  int rc;
+  if ((rc = ensure(c, c->ws_misc, need))) return rc;
+  double *dD = static_cast<double *>(c->ws_misc.ptr);
+  double *dJ = dD + np, *dK = dJ + np, *dF = dK + nden * np, *dC = dF + nden * np;
+  if ((rc = hip_check(c, hipMemcpyAsync(dD, dlt, sizeof(double) * np, hipMemcpyHostToDevice,
+                                        c->stream), "copy DLT")) ||
+      (rc = hip_check(c, hipMemcpyAsync(dF, flt, sizeof(double) * np * nden,
+                                        hipMemcpyHostToDevice, c->stream), "copy FLT")))
+    return rc;
+  if (tot > 0 && (rc = hip_check(c, hipMemcpyAsync(dC, porb, sizeof(double) * tot,
+                                                   hipMemcpyHostToDevice, c->stream),
+                                 "copy Porb")))
+    return rc;
+  if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
+  const double *cptr[2] = {dC, dC + int64_t(nocc[0]) * c->n};
+  if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
+  const int32_t jsrc[2] = {0, 0};
+  const double fj[2] = {1.0, 1.0};                 // FactCI, cho_fscf.F90:64
+  const double fk[2] = {-exfac, -exfac};           // FactXI, cho_fscf.F90:65
+  if ((rc = fock_update(c, nden, dF, np, dJ, np, jsrc, fj, dK, np, fk))) return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(flt, dF, sizeof(double) * np * nden,
+                                        hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "cho_fscf sync");
+}
+
+int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,
+                   double factxi, const double *dlt, int32_t do_active, const double *porb1,
+                   int32_t nchm, const double *porb2, const double *porb3, double *w_pwxy,
+                   const double *cmo, int32_t nfro, int32_t norb, double exfac,
+                   int32_t precision) {
+  (void)exfac;  // the reference only prints a warning for ExFac /= 1 (cho_fmcscf.F90:88-94)
+  if (!c) return MJK_ERR_ARG;
+  if (!flt || !dlt) return set_error(c, MJK_ERR_ARG, "cho_fmcscf: null pointer");
+  if (nforb < 0 || niorb < 0 || naorb < 0 || nchm < 0)
+    return set_error(c, MJK_ERR_ARG, "cho_fmcscf: negative orbital count");
+  const int nden = do_active ? 2 : 1;
+  const int64_t np = c->np, n = c->n;
+  const int32_t nk1 = nforb + niorb;
+  const int32_t nk2 = do_active ? nchm : 0;
+  if ((nk1 > 0 && !porb1) || (nk2 > 0 && !porb2))
+    return set_error(c, MJK_ERR_ARG, "cho_fmcscf: null orbital matrix");
+  const bool do_puvx = naorb > 0 && w_pwxy;
+  if (do_puvx && (!porb3 || !cmo || nfro < 0 || norb < 0 || nfro + norb > n))
+    return set_error(c, MJK_ERR_ARG, "cho_fmcscf: bad (pw|xy) arguments");
+  const int64_t nxy = int64_t(naorb) * (naorb + 1) / 2;
+  const int64_t nw = do_puvx ? int64_t(norb) * naorb * nxy : 0;
+  const size_t need = sizeof(double) * size_t(np * 4 * nden + (nk1 + nk2) * n +
+                                              (do_puvx ? naorb * n + n * n + nw : 0));
+  int rc;
+  if ((rc = ensure(c, c->ws_misc, need))) return rc;
+  double *dD = static_cast<double *>(c->ws_misc.ptr);
+  double *dJ = dD + nden * np, *dK = dJ + nden * np, *dF = dK + nden * np, *dC1 = dF + nden * np;
+  double *dC2 = dC1 + int64_t(nk1) * n, *dC3 = dC2 + int64_t(nk2) * n;
+  double *dCMO = dC3 + (do_puvx ? int64_t(naorb) * n : 0), *dW = dCMO + (do_puvx ? n * n : 0);
+  auto h2d = [&](void *dst, const void *src, size_t cnt, const char *what) {
+    return hip_check(c, hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice,
+                                       c->stream), what);
+  };
+  if ((rc = h2d(dD, dlt, size_t(np * nden), "copy DLT")) ||
+      (rc = h2d(dF, flt, size_t(np * nden), "copy FLT")))
+    return rc;
+  if (nk1 > 0 && (rc = h2d(dC1, porb1, size_t(nk1 * n), "copy POrb(1)"))) return rc;
+  if (nk2 > 0 && (rc = h2d(dC2, porb2, size_t(nk2 * n), "copy POrb(2)"))) return rc;
+  if ((rc = coulomb(c, nden, dD, np, dJ, np, nullptr))) return rc;
+  const int32_t nocc[2] = {nk1, nk2};
+  const double *cptr[2] = {dC1, dC2};
+  if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
+  const int32_t jsrc[2] = {0, 1};
+  const double fj[2] = {1.0, 1.0};        // FactCI, FactCA (cho_fmcscf.F90:71)
+  const double fk[2] = {factxi, -0.5};    // FactXI, FactXA
+  if ((rc = fock_update(c, nden, dF, np, dJ, np, jsrc, fj, dK, np, fk))) return rc;
+  if (do_puvx) {
+    if ((rc = h2d(dC3, porb3, size_t(naorb * n), "copy POrb(3)")) ||
+        (rc = h2d(dCMO, cmo, size_t(n * n), "copy CMO")))
+      return rc;
+    if ((rc = mjk_puvx(c, naorb, dC3, dCMO, nfro, norb, dW))) return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync(w_pwxy, dW, sizeof(double) * nw,
+                                          hipMemcpyDeviceToHost, c->stream), "copy W_PWXY")))
+      return rc;
+  }
+  if ((rc = hip_check(c, hipMemcpyAsync(flt, dF, sizeof(double) * np * nden,
+                                        hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "cho_fmcscf sync");
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+// Coulomb (J) build from HBM-resident packed Cholesky vectors.
+//
+// Reference: the DGEMV pair of src/fock_util/cho_fscf.F90:208,215 (also
+// cho_lk_scf.F90:442-449, cho_fmcscf.F90:261-285):
+//   V_J  = sum_p L^J_p D_p          (D folded: off-diagonal doubled)
+//   J_p  = sum_J L^J_p V_J
+// Both passes stream L once; they are HBM-bound.  L is stored vector-major
+// (row J holds the packed LT of L^J, stride ldl, a multiple of 512 doubles),
+// so each pass reads whole 4 KiB row segments per workgroup iteration.
+#include "mjk_internal.h"
+
+namespace mjk {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSeg = 2 * kThreads;  // doubles per row segment per iteration
+
+template <int N>
+__device__ inline void block_sum(double (&v)[N], double *lds /* [N][4] */) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double x = v[i];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_down(x, off, kWave);
+    if (lane == 0) lds[i * 4 + wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const double *p = lds + threadIdx.x * 4;
+    v[0] = (p[0] + p[1]) + (p[2] + p[3]);
+  }
+}
+
+// Pass 1.  grid = (ceil(M/JB), nsplit).  Workgroup (jb, s) accumulates
+// V(J, d) for JB vectors over the pair range of split s into
+// vpart[(s*M + J)*nd + d].
+template <int JB, int ND>
+__global__ __launch_bounds__(kThreads) void k_coulomb_v(
+    const double *__restrict__ L, int64_t ldl, int64_t M, const double *__restrict__ D,
+    int nd, int64_t pchunk, double *__restrict__ vpart) {
+  __shared__ double lds[JB * ND * 4];
+  const int64_t j0 = int64_t(blockIdx.x) * JB;
+  const int64_t p0 = int64_t(blockIdx.y) * pchunk;
+  const int64_t p1 = min(p0 + pchunk, ldl);
+  double acc[JB * ND];
+#pragma unroll
+  for (int i = 0; i < JB * ND; ++i) acc[i] = 0.0;
+  for (int64_t p = p0 + 2 * threadIdx.x; p < p1; p += kSeg) {
+    v2d dv[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+      dv[d] = (d < nd) ? *reinterpret_cast<const v2d *>(D + d * ldl + p)
+                       : v2d{0.0, 0.0};
+#pragma unroll
+    for (int jb = 0; jb < JB; ++jb) {
+      if (j0 + jb < M) {
+        const v2d l = __builtin_nontemporal_load(
+            reinterpret_cast<const v2d *>(L + (j0 + jb) * ldl + p));
+#pragma unroll
+        for (int d = 0; d < ND; ++d) acc[jb * ND + d] += l.x * dv[d].x + l.y * dv[d].y;
+      }
+    }
+  }
+  block_sum<JB * ND>(acc, lds);
+  if (threadIdx.x < JB * ND) {
+    const int jb = threadIdx.x / ND, d = threadIdx.x % ND;
+    if (j0 + jb < M && d < nd) vpart[(int64_t(blockIdx.y) * M + j0 + jb) * nd + d] = acc[0];
+  }
+}
+
+// V(J, d) = sum_s vpart[(s*M + J)*nd + d]  (fixed order: deterministic).
+__global__ void k_reduce_v(const double *__restrict__ vpart, int64_t M, int nd, int nsplit,
+                           double *__restrict__ v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= M * nd) return;
+  double s = 0.0;
+  for (int k = 0; k < nsplit; ++k) s += vpart[int64_t(k) * M * nd + i];
+  v[i] = s;
+}
+
+// Pass 2.  grid = (ldl/kSeg, nsplit).  Workgroup (c, s) owns pairs
+// [c*kSeg, (c+1)*kSeg) and sums over the vectors of split s:
+// jpart[(s*nd + d)*ldl + p] = sum_J L(J,p) V(J,d).
+template <int ND>
+__global__ __launch_bounds__(kThreads) void k_coulomb_j(
+    const double *__restrict__ L, int64_t ldl, int64_t M, const double *__restrict__ V,
+    int nd, int64_t jchunk, double *__restrict__ jpart) {
+  const int64_t p = int64_t(blockIdx.x) * kSeg + 2 * threadIdx.x;
+  const int64_t ja = int64_t(blockIdx.y) * jchunk;
+  const int64_t jz = min(ja + jchunk, M);
+  v2d acc[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) acc[d] = v2d{0.0, 0.0};
+  int64_t j = ja;
+  for (; j + 4 <= jz; j += 4) {
+    v2d l[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      l[u] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(L + (j + u) * ldl + p));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        if (d < nd) {
+          const double v = V[(j + u) * nd + d];
+          acc[d].x += l[u].x * v;
+          acc[d].y += l[u].y * v;
+        }
+      }
+    }
+  }
+  for (; j < jz; ++j) {
+    const v2d l = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(L + j * ldl + p));
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      if (d < nd) {
+        const double v = V[j * nd + d];
+        acc[d].x += l.x * v;
+        acc[d].y += l.y * v;
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < ND; ++d)
+    if (d < nd)
+      *reinterpret_cast<v2d *>(jpart + (int64_t(blockIdx.y) * nd + d) * ldl + p) = acc[d];
+}
+
+// jlt(d*ldj + p) = sum_s jpart[(s*nd + d)*ldl + p], p < np.
+__global__ void k_reduce_j(const double *__restrict__ jpart, int64_t ldl, int64_t np, int nd,
+                           int nsplit, double *__restrict__ jlt, int64_t ldj) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (p >= np) return;
+  double s = 0.0;
+  for (int k = 0; k < nsplit; ++k) s += jpart[(int64_t(k) * nd + d) * ldl + p];
+  jlt[d * ldj + p] = s;
+}
+
+// Copies nd densities of stride ldd into the zero-padded staging buffer of
+// stride ldl.
+__global__ void k_pack_dens(const double *__restrict__ dlt, int64_t ldd, int64_t np,
+                            int64_t ldl, double *__restrict__ out) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (p >= ldl) return;
+  out[d * ldl + p] = (p < np) ? dlt[d * ldd + p] : 0.0;
+}
+
+template <int JB, int ND>
+void launch_v(Context *c, const double *D, int nd, double *vpart, int nsplit, int64_t pchunk) {
+  dim3 grid(unsigned(ceil_div(c->nchol, JB)), unsigned(nsplit));
+  hipLaunchKernelGGL((k_coulomb_v<JB, ND>), grid, dim3(kThreads), 0, c->stream, c->L, c->ldl,
+                     c->nchol, D, nd, pchunk, vpart);
+}
+
+template <int ND>
+void launch_j(Context *c, const double *V, int nd, double *jpart, int nsplit, int64_t jchunk) {
+  dim3 grid(unsigned(c->ldl / kSeg), unsigned(nsplit));
+  hipLaunchKernelGGL((k_coulomb_j<ND>), grid, dim3(kThreads), 0, c->stream, c->L, c->ldl,
+                     c->nchol, V, nd, jchunk, jpart);
+}
+
+}  // namespace
+
+int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+            double *v_out) {
+  if (nd < 1 || nd > 16) return set_error(c, MJK_ERR_ARG, "coulomb: nd must be in [1,16]");
+  if (ldd < c->np || ldj < c->np) return set_error(c, MJK_ERR_ARG, "coulomb: ld < nbas(nbas+1)/2");
+  if (c->nloaded != c->nchol)
+    return set_error(c, MJK_ERR_SHORT_READ, "coulomb: not all Cholesky vectors were uploaded");
+  const int64_t M = c->nchol, ldl = c->ldl;
+  int rc;
+
+  // staging copy of the densities (zero padded to ldl)
+  if ((rc = ensure(c, c->ws_d, sizeof(double) * nd * ldl))) return rc;
+  double *Dp = static_cast<double *>(c->ws_d.ptr);
+  phase_begin(c);
+  hipLaunchKernelGGL(k_pack_dens, dim3(unsigned(ceil_div(ldl, 256)), unsigned(nd)), dim3(256), 0,
+                     c->stream, dlt, ldd, c->np, ldl, Dp);
+  phase_end(c, MJK_PHASE_PACK);
+
+  // ---- pass 1: V = L^T D
+  const int ndt = nd <= 1 ? 1 : nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : 16;
+  const int jb = ndt <= 2 ? 8 : ndt == 4 ? 4 : ndt == 8 ? 2 : 1;
+  const int64_t nblk = ceil_div(M, jb);
+  const int64_t nseg = ldl / kSeg;
+  int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nblk), 1), nseg));
+  const int64_t pchunk = ceil_div(nseg, nsplit1) * kSeg;
+  nsplit1 = int(ceil_div(ldl, pchunk));
+  if ((rc = ensure(c, c->ws_vpart, sizeof(double) * nsplit1 * M * nd))) return rc;
+  if ((rc = ensure(c, c->ws_v, sizeof(double) * M * nd))) return rc;
+  double *vpart = static_cast<double *>(c->ws_vpart.ptr);
+  double *V = v_out ? v_out : static_cast<double *>(c->ws_v.ptr);
+  phase_begin(c);
+  switch (ndt) {
+    case 1: launch_v<8, 1>(c, Dp, nd, vpart, nsplit1, pchunk); break;
+    case 2: launch_v<8, 2>(c, Dp, nd, vpart, nsplit1, pchunk); break;
+    case 4: launch_v<4, 4>(c, Dp, nd, vpart, nsplit1, pchunk); break;
+    case 8: launch_v<2, 8>(c, Dp, nd, vpart, nsplit1, pchunk); break;
+    default: launch_v<1, 16>(c, Dp, nd, vpart, nsplit1, pchunk); break;
+  }
+  phase_end(c, MJK_PHASE_COULOMB_V);
+  phase_begin(c);
+  hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M * nd, 256))), dim3(256), 0, c->stream,
+                     vpart, M, nd, nsplit1, V);
+  phase_end(c, MJK_PHASE_REDUCE);
+
+  // ---- pass 2: J = L V
+  int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nseg), 1), M));
+  const int64_t jchunk = ceil_div(M, nsplit2);
+  nsplit2 = int(ceil_div(M, jchunk));
+  if ((rc = ensure(c, c->ws_jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
+  double *jpart = static_cast<double *>(c->ws_jpart.ptr);
+  phase_begin(c);
+  switch (ndt) {
+    case 1: launch_j<1>(c, V, nd, jpart, nsplit2, jchunk); break;
+    case 2: launch_j<2>(c, V, nd, jpart, nsplit2, jchunk); break;
+    case 4: launch_j<4>(c, V, nd, jpart, nsplit2, jchunk); break;
+    case 8: launch_j<8>(c, V, nd, jpart, nsplit2, jchunk); break;
+    default: launch_j<16>(c, V, nd, jpart, nsplit2, jchunk); break;
+  }
+  phase_end(c, MJK_PHASE_COULOMB_J);
+  phase_begin(c);
+  hipLaunchKernelGGL(k_reduce_j, dim3(unsigned(ceil_div(c->np, 256)), unsigned(nd)), dim3(256), 0,
+                     c->stream, jpart, ldl, c->np, nd, nsplit2, jlt, ldj);
+  phase_end(c, MJK_PHASE_REDUCE);
+  return hip_check(c, hipGetLastError(), "coulomb launch");
+}
+
+}  // namespace mjk

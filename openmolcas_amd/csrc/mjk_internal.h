@@ -1,0 +1,116 @@
+// Internal declarations of the MI355X Cholesky J/K library (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "molcas_jk.h"
+
+namespace mjk {
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr int64_t kRowAlign = 512;   // L row stride granularity (doubles, 4 KiB)
+constexpr int kTileA = 64;           // AO tile of the exchange kernels
+constexpr int kNumPhases = MJK_NUM_PHASES;
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+inline int64_t ceil_div(int64_t x, int64_t m) { return (x + m - 1) / m; }
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct ReducedSet {
+  int64_t nrs = 0;
+  bool identity = false;      // rs position r maps to LT index r for all r
+  int32_t *d_map = nullptr;   // device copy of rs -> LT map (0-based)
+};
+
+struct Context {
+  int32_t n = 0;          // nBas
+  int64_t np = 0;         // n(n+1)/2
+  int64_t ldl = 0;        // L row stride (doubles)
+  int32_t n_pad = 0;      // n rounded up to kTileA
+  int64_t nchol = 0;      // vector capacity
+  double *L = nullptr;    // nchol x ldl, packed LT per vector, zero padded
+  std::vector<uint8_t> loaded;  // per vector: uploaded?
+  int64_t nloaded = 0;
+  std::vector<ReducedSet> rsets;
+
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+
+  int64_t x_budget = int64_t(16) << 30;  // bytes for one chunk of X
+  int32_t tile_syrk = 0;                  // 0 = automatic, else 64 or 128
+
+  // workspaces
+  DevBuf ws_d, ws_vpart, ws_v, ws_jpart, ws_cbank, ws_x, ws_kpart, ws_stage, ws_j,
+      ws_k, ws_red, ws_misc;
+
+  // profiling: per-phase accumulated milliseconds and launch counts
+  bool profile = false;
+  double phase_ms[kNumPhases] = {};
+  int64_t phase_calls[kNumPhases] = {};
+  std::vector<hipEvent_t> ev_pool;   // reusable events
+  size_t ev_used = 0;
+  int ev_open = -1;                   // pool index of the pending begin event
+  struct Pending { int phase; int ev_begin; int ev_end; };
+  std::vector<Pending> pending;       // recorded, not yet resolved
+
+  std::string err;
+};
+
+// Returns a device buffer of at least `bytes`, reallocating (grow-only).
+int ensure(Context *c, DevBuf &b, size_t bytes);
+
+// Phase timing helpers (no-ops unless c->profile).  Events are recorded on the
+// context stream and resolved lazily, so timing does not serialise the host.
+void phase_begin(Context *c);
+void phase_end(Context *c, int phase);
+int resolve_phases(Context *c);
+
+int set_error(Context *c, int rc, const std::string &msg);
+int hip_check(Context *c, hipError_t e, const char *what);
+
+// ---- kernels (launchers) -------------------------------------------------
+// Coulomb: jlt(d, p) = sum_J L(J,p) * sum_q L(J,q) dlt(d,q), for nd densities.
+int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
+            int64_t ldj, double *v_out /* optional [M][nd] */);
+
+// Exchange: klt(d) = sum_J sum_k X_d(J,k,a) X_d(J,k,b) (LT), X_d = C_d^T L^J.
+// corb[d] is (nocc[d] x n) with the orbital index fastest, element (k,a) at
+// corb[d][k + a*nocc[d]].
+int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+             double *klt, int64_t ldk, int32_t precision);
+
+// Orbital bank: concatenates the nk sets into ws_cbank as Cb[a][k] with ldc =
+// 16*ot columns; set d starts at column off[d] (multiple of 4), opad[d] =
+// round_up(nocc[d], 4).  tot = sum opad (0: nothing packed).
+int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+              std::vector<int> &off, std::vector<int> &opad, int *ot, int *tot);
+// Half-transform of vectors j0..j0+jn-1 against the bank (16*ot columns):
+// X[(jr*16*ot + k)*n_pad + a] = sum_b Cb[b][k] L^{j0+jr}_{ab}.
+void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, double *X);
+
+// Fock assembly: flt(d) += fj[d]*jlt(jsrc[d]) + fk[d]*klt(d).
+int fock_update(Context *c, int32_t nd, double *flt, int64_t ldf, const double *jlt,
+                int64_t ldj, const int32_t *jsrc, const double *fj, const double *klt,
+                int64_t ldk, const double *fk);
+
+// Energy: e = sum_p h(p) d(p) + 1/2 sum_p g(p) d(p) over LT storage.
+int energy(Context *c, const double *h, const double *dlt, const double *g,
+           double *e_out_dev);
+
+}  // namespace mjk
+
+struct mjk_context : public mjk::Context {};

@@ -1,0 +1,232 @@
+"""Python mirror of OpenMolcas's Cholesky J/K Fock-build interface.
+
+``JKContext`` owns one library context: the Cholesky vectors resident in HBM
+(the role of Cho_X_Init's vector buffer, src/cholesky_util/cho_x_init.F90:55)
+and the device workspaces.  Two layers are exposed:
+
+* ``cho_fscf`` / ``cho_fmcscf`` take host (numpy) arrays with the reference's
+  argument meaning and storage (src/fock_util/cho_fscf.F90:14,
+  src/fock_util/cho_fmcscf.F90:14); FLT is updated in place (``+=``) and a
+  nonzero return code raises ``MjkError``, where the reference would Abend.
+* ``coulomb`` / ``exchange`` / ``fock_update`` / ``energy`` / ``puvx`` work on
+  torch device tensors, queued on torch's current stream.
+
+Every call goes through the C ABI of ``libmolcas_jk.so``; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import MjkError, PREC_FP64
+
+_PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+def precision_code(p) -> int:
+    if isinstance(p, str):
+        return _PREC[p]
+    return int(p)
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    if isinstance(t, torch.Tensor):
+        return ctypes.c_void_p(t.data_ptr())
+    if isinstance(t, np.ndarray):
+        return ctypes.c_void_p(t.ctypes.data)
+    if t is None:
+        return ctypes.c_void_p(0)
+    raise TypeError(type(t))
+
+
+def _dev_f64(t: torch.Tensor, device) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(np.asarray(t))
+    return t.to(device=device, dtype=torch.float64).contiguous()
+
+
+def _host_f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class JKContext:
+    """One device context for nbas basis functions and nchol vectors (nSym=1)."""
+
+    def __init__(self, nbas: int, nchol: int, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("JKContext needs a HIP device (MI355X); there is no CPU fallback")
+        self.lib = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.nbas = int(nbas)
+        self.nchol = int(nchol)
+        self.npair = self.nbas * (self.nbas + 1) // 2
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            rc = self.lib.mjk_create(ctypes.byref(h), self.nbas, self.nchol)
+        if rc != 0:
+            raise MjkError(rc, "mjk_create failed")
+        self._h = h
+        self.bind_stream(torch.cuda.current_stream(self.device))
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self.lib.mjk_last_error(self._h)
+            raise MjkError(rc, msg.decode() if msg else "")
+
+    def bind_stream(self, stream: torch.cuda.Stream):
+        """Queues all device work on ``stream`` (torch's current stream by default)."""
+        self._check(self.lib.mjk_set_stream(self._h, ctypes.c_void_p(stream.cuda_stream)))
+        self.stream = stream
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.mjk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_x_budget(self, nbytes: int):
+        self._check(self.lib.mjk_set_x_budget(self._h, int(nbytes)))
+
+    def sync(self):
+        self._check(self.lib.mjk_sync(self._h))
+
+    # ------------------------------------------------------------ vectors
+    def set_reduced_set(self, jred: int, rs_to_lt) -> None:
+        """Reduced set jred: rs_to_lt[r] = 0-based LT index of pair r."""
+        m = np.ascontiguousarray(rs_to_lt, dtype=np.int64)
+        self._check(self.lib.mjk_set_reduced_set(self._h, int(jred), len(m), _ptr(m)))
+
+    def upload_vectors(self, jred: int, jvec: int, lrs) -> None:
+        """Vectors jvec.. of reduced set jred; lrs is (jnum, nrs), one vector per
+        row (the memory of the reference's Lrs(nRS, JNUM)).  numpy -> host copy,
+        torch CUDA tensor -> device copy."""
+        if isinstance(lrs, torch.Tensor) and lrs.is_cuda:
+            t = lrs.to(torch.float64).contiguous()
+            self._check(self.lib.mjk_upload_vectors(self._h, int(jred), int(jvec), t.shape[0], _ptr(t), 1))
+        else:
+            a = _host_f64(lrs)
+            self._check(self.lib.mjk_upload_vectors(self._h, int(jred), int(jvec), a.shape[0], _ptr(a), 0))
+
+    def upload_packed(self, L) -> None:
+        """All vectors on one reduced set covering every LT pair: L is (nchol, npair)."""
+        if L.shape != (self.nchol, self.npair):
+            raise ValueError(f"expected L of shape {(self.nchol, self.npair)}, got {tuple(L.shape)}")
+        self.set_reduced_set(0, np.arange(self.npair, dtype=np.int64))
+        self.upload_vectors(0, 0, L)
+
+    # ------------------------------------------------------------ drop-ins (host)
+    def cho_fscf(self, nden: int, flt: np.ndarray, nforb, niorb, porb, dlt: np.ndarray,
+                 exfac: float, precision="fp64") -> int:
+        """CHO_FSCF(rc,nDen,FLT,nForb,nIorb,Porb,DLT,ExFac), nSym = 1.
+
+        flt: (nden, npair) float64, updated in place.  porb: list of nden arrays
+        (nbas, nforb[d]+niorb[d]) = Porb(d)%SB(1)%A2(nOcc,nBas) memory.  dlt:
+        folded LT density (only DLT(1) enters Coulomb, as in the reference)."""
+        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
+            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        nf = np.ascontiguousarray(nforb, dtype=np.int32)
+        ni = np.ascontiguousarray(niorb, dtype=np.int32)
+        d = _host_f64(dlt).reshape(-1)[: self.npair].copy()
+        if nden in (1, 2):
+            parts = [_host_f64(porb[k]).reshape(-1) for k in range(nden)]
+            pc = np.concatenate(parts) if parts else np.zeros(1)
+        else:
+            pc = np.zeros(1)
+        rc = self.lib.mjk_cho_fscf(self._h, int(nden), _ptr(flt), _ptr(nf), _ptr(ni), _ptr(pc),
+                                   _ptr(d), float(exfac), precision_code(precision))
+        self._check(rc)
+        return rc
+
+    def cho_fmcscf(self, flt: np.ndarray, nforb: int, niorb: int, naorb: int, factxi: float,
+                   dlt: np.ndarray, do_active: bool, porb1, nchm: int, porb2, porb3,
+                   w_pwxy, cmo, nfro: int, norb: int, exfac: float = 1.0,
+                   precision="fp64") -> int:
+        """CHO_FMCSCF(rc,FLT,nForb,nIorb,nAorb,FactXI,DLT,DoActive,POrb,nChM,
+        W_PWXY,CMO,ExFac), nSym = 1.  flt (2, npair) updated in place; w_pwxy
+        (norb*naorb*naorb(naorb+1)/2,) filled; cmo (nbas, nbas) with cmo[a, p] =
+        CMO(a,p)."""
+        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
+            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        p1 = _host_f64(porb1) if porb1 is not None else np.zeros(1)
+        p2 = _host_f64(porb2) if porb2 is not None else np.zeros(1)
+        p3 = _host_f64(porb3) if porb3 is not None else np.zeros(1)
+        cm = _host_f64(np.asarray(cmo).T) if cmo is not None else np.zeros(1)
+        d = _host_f64(dlt)
+        if w_pwxy is not None and (w_pwxy.dtype != np.float64 or not w_pwxy.flags.c_contiguous):
+            raise TypeError("w_pwxy must be a C-contiguous float64 array")
+        rc = self.lib.mjk_cho_fmcscf(self._h, _ptr(flt), int(nforb), int(niorb), int(naorb),
+                                     float(factxi), _ptr(d), int(bool(do_active)), _ptr(p1),
+                                     int(nchm), _ptr(p2), _ptr(p3), _ptr(w_pwxy), _ptr(cm),
+                                     int(nfro), int(norb), float(exfac), precision_code(precision))
+        self._check(rc)
+        return rc
+
+    # ------------------------------------------------------------ device engine
+    def coulomb(self, dlt: torch.Tensor, want_v: bool = False):
+        """J for folded LT densities dlt (nd, npair) on the device."""
+        d = _dev_f64(dlt, self.device)
+        if d.dim() == 1:
+            d = d.unsqueeze(0)
+        nd = d.shape[0]
+        j = torch.empty((nd, self.npair), dtype=torch.float64, device=self.device)
+        v = torch.empty((self.nchol, nd), dtype=torch.float64, device=self.device) if want_v else None
+        self._check(self.lib.mjk_coulomb(self._h, nd, _ptr(d), d.stride(0), _ptr(j), j.stride(0), _ptr(v)))
+        return (j, v) if want_v else j
+
+    def exchange(self, corbs, precision="fp64") -> torch.Tensor:
+        """K_d = sum_J (L^J C_d)(L^J C_d)^T in LT storage for each orbital
+        matrix C_d (nbas, o_d) on the device (C-order = Porb memory)."""
+        cs = [_dev_f64(c, self.device) for c in corbs]
+        nk = len(cs)
+        nocc = np.array([c.shape[1] if c.dim() == 2 else 0 for c in cs], dtype=np.int32)
+        ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
+        k = torch.empty((nk, self.npair), dtype=torch.float64, device=self.device)
+        self._check(self.lib.mjk_exchange(self._h, nk, _ptr(nocc), ptrs, _ptr(k), k.stride(0),
+                                          precision_code(precision)))
+        return k
+
+    def fock_update(self, flt: torch.Tensor, jlt=None, jsrc=None, fj=None, klt=None, fk=None):
+        """flt[d] += fj[d] * jlt[jsrc[d]] + fk[d] * klt[d] (in place)."""
+        nd = flt.shape[0]
+        js = np.ascontiguousarray(jsrc if jsrc is not None else [-1] * nd, dtype=np.int32)
+        a = np.ascontiguousarray(fj if fj is not None else [0.0] * nd, dtype=np.float64)
+        b = np.ascontiguousarray(fk if fk is not None else [0.0] * nd, dtype=np.float64)
+        self._check(self.lib.mjk_fock_update(
+            self._h, nd, _ptr(flt), flt.stride(0), _ptr(jlt), jlt.stride(0) if jlt is not None else 0,
+            _ptr(js), _ptr(a), _ptr(klt), klt.stride(0) if klt is not None else 0, _ptr(b)))
+        return flt
+
+    def energy(self, h, dlt, g) -> torch.Tensor:
+        """sum_LT h*dlt + 1/2 sum_LT g*dlt (device scalar)."""
+        e = torch.empty(1, dtype=torch.float64, device=self.device)
+        self._check(self.lib.mjk_energy(self._h, _ptr(h), _ptr(dlt), _ptr(g), _ptr(e)))
+        return e
+
+    def puvx(self, porb3: torch.Tensor, cmo: torch.Tensor, nfro: int, norb: int) -> torch.Tensor:
+        """(pw|xy) for active orbitals porb3 (nbas, naorb); cmo (nbas, nbas),
+        cmo[a, p] = CMO(a,p).  Returns W_PWXY as a flat tensor."""
+        c3 = _dev_f64(porb3, self.device)
+        cm = _dev_f64(cmo, self.device).t().contiguous()
+        na = c3.shape[1]
+        w = torch.empty(norb * na * (na * (na + 1) // 2), dtype=torch.float64, device=self.device)
+        self._check(self.lib.mjk_puvx(self._h, na, _ptr(c3), _ptr(cm), int(nfro), int(norb), _ptr(w)))
+        return w
+
+    # ------------------------------------------------------------ measurement
+    def set_profiling(self, on: bool):
+        self._check(self.lib.mjk_set_profiling(self._h, int(bool(on))))
+
+    def phase_times(self):
+        ms = (ctypes.c_double * len(_lib.PHASES))()
+        calls = (ctypes.c_int64 * len(_lib.PHASES))()
+        self._check(self.lib.mjk_phase_times(self._h, ms, calls))
+        return {p: (ms[i], calls[i]) for i, p in enumerate(_lib.PHASES)}

@@ -1,0 +1,269 @@
+"""CPU FP64 oracle for the Cholesky J/K Fock build.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()``
+as the checker, and by ``bench.py``'s ``cpu_baseline`` leg -- never by the
+product package.  The C restatement lives in ``mjk_oracle.c`` (each function
+cites the OpenMolcas routine it follows); this module is its ctypes binding plus
+a few numpy helpers that the checks share.
+
+Parity status of each function (how it is pinned, see tests/test_oracle.py):
+  fold/unfold/itri      -- textbook identities and the reference's formulas
+  swap_full2rs/rs2full  -- round trip on random nested reduced sets
+  cho_vtra2/vtra0       -- dense brute force X = C^T L (numpy einsum)
+  dgemm_tri_tn/nt       -- dense numpy products, lower triangle
+  cho_fscf              -- dense brute force J/K, batching invariance,
+                           reduced-set invariance, 4FC gradient identity
+  cho_fmcscf            -- dense brute force FI/FA/(pw|xy)
+  cd_incore             -- ||D - V V^T||_max <= thr, exact rank, numpy cholesky
+  energy                -- closed form tr(hP) + 1/2 tr(PJ) - 1/4 tr(PK)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libmjk_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class _Chol(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("nred", ctypes.c_int32),
+        ("nrs", _i64p),
+        ("rs_off", _i64p),
+        ("rs2lt", _i64p),
+        ("nvec", ctypes.c_int64),
+        ("vred", _i32p),
+        ("voff", _i64p),
+        ("lvec", _dp),
+    ]
+
+
+def build() -> str:
+    """Compiles the oracle with its Makefile (gcc) and returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.ora_itri.restype = ctypes.c_int64
+        L.ora_itri.argtypes = [ctypes.c_int64, ctypes.c_int64]
+        L.ora_fold.argtypes = [ctypes.c_int32, _dp, _dp]
+        L.ora_unfold.argtypes = [ctypes.c_int32, _dp, _dp]
+        L.ora_swap_full2rs.argtypes = [ctypes.c_int64, _i64p, ctypes.c_int32, ctypes.c_int64, _dp, _dp]
+        L.ora_swap_rs2full.argtypes = [ctypes.c_int64, _i64p, ctypes.c_int32, ctypes.c_int64, _dp, _dp,
+                                       ctypes.c_int32]
+        L.ora_dgemm_tri_tn.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _dp, ctypes.c_int64,
+                                       _dp, ctypes.c_int64, ctypes.c_double, _dp]
+        L.ora_dgemm_tri_nt.argtypes = L.ora_dgemm_tri_tn.argtypes
+        L.ora_cho_vtra2.argtypes = [ctypes.c_int32, ctypes.c_int64, _i64p, ctypes.c_int64, _dp,
+                                    ctypes.c_int32, _dp, _dp]
+        L.ora_cho_vtra0.argtypes = L.ora_cho_vtra2.argtypes
+        L.ora_cho_fscf.restype = ctypes.c_int32
+        L.ora_cho_fscf.argtypes = [ctypes.POINTER(_Chol), ctypes.c_int32, _dp, _i32p, _i32p,
+                                   ctypes.POINTER(_dp), _dp, ctypes.c_double, ctypes.c_int64]
+        L.ora_cho_fmcscf.restype = ctypes.c_int32
+        L.ora_cho_fmcscf.argtypes = [ctypes.POINTER(_Chol), _dp, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_double, _dp, ctypes.c_int32, _dp,
+                                     ctypes.c_int32, _dp, _dp, _dp, _dp, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_double, ctypes.c_int64]
+        L.ora_cd_incore.restype = ctypes.c_int32
+        L.ora_cd_incore.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32, _i32p, ctypes.c_double]
+        L.ora_energy.restype = ctypes.c_double
+        L.ora_energy.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray, ct=_dp):
+    return a.ctypes.data_as(ct)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- vector store
+class CholStore:
+    """Reduced sets + vectors ordered by reduced set (the state Cho_X_Init
+    leaves behind).  ``sets`` is a list of 0-based LT index arrays; ``vectors``
+    a list of (set_id, values) pairs, set ids nondecreasing."""
+
+    def __init__(self, n: int, sets, vectors):
+        self.n = int(n)
+        self.sets = [np.ascontiguousarray(s, dtype=np.int64) for s in sets]
+        self.nrs = np.array([len(s) for s in self.sets], dtype=np.int64)
+        self.rs_off = np.concatenate([[0], np.cumsum(self.nrs)[:-1]]).astype(np.int64)
+        self.rs2lt = np.concatenate(self.sets).astype(np.int64) if self.sets else np.zeros(0, np.int64)
+        self.vred = np.array([r for r, _ in vectors], dtype=np.int32)
+        if len(self.vred) and np.any(np.diff(self.vred) < 0):
+            raise ValueError("vectors must be ordered by reduced set")
+        vals = [np.ascontiguousarray(v, dtype=np.float64) for _, v in vectors]
+        for r, v in zip(self.vred, vals):
+            if v.shape != (self.nrs[r],):
+                raise ValueError("vector length does not match its reduced set")
+        lens = np.array([len(v) for v in vals], dtype=np.int64)
+        self.voff = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+        self.lvec = np.concatenate(vals) if vals else np.zeros(1)
+        self._c = _Chol(self.n, len(self.sets), _p(self.nrs, _i64p), _p(self.rs_off, _i64p),
+                        _p(self.rs2lt, _i64p), len(vals), _p(self.vred, _i32p),
+                        _p(self.voff, _i64p), _p(self.lvec))
+
+    @classmethod
+    def full(cls, L_packed: np.ndarray) -> "CholStore":
+        """One reduced set covering every LT pair; L_packed is (M, n(n+1)/2)."""
+        M, npair = L_packed.shape
+        n = int(round((np.sqrt(8 * npair + 1) - 1) / 2))
+        idx = np.arange(npair, dtype=np.int64)
+        return cls(n, [idx], [(0, L_packed[j]) for j in range(M)])
+
+    def packed(self) -> np.ndarray:
+        """Vectors expanded to full LT storage, (M, n(n+1)/2)."""
+        npair = self.n * (self.n + 1) // 2
+        out = np.zeros((len(self.vred), npair))
+        for j, r in enumerate(self.vred):
+            off = self.voff[j]
+            out[j, self.sets[r]] = self.lvec[off:off + self.nrs[r]]
+        return out
+
+
+# ---------------------------------------------------------------- wrappers
+def itri(i: int, j: int) -> int:
+    return int(lib().ora_itri(i, j))
+
+
+def fold(a_sq: np.ndarray) -> np.ndarray:
+    a = _f64(a_sq)
+    n = a.shape[0]
+    # the C routine reads A(i,j) column-major; a symmetric input makes the
+    # distinction moot, a general one is passed transposed
+    at = _f64(a.T)
+    out = np.zeros(n * (n + 1) // 2)
+    lib().ora_fold(n, _p(at), _p(out))
+    return out
+
+
+def unfold(a_lt: np.ndarray, n: int) -> np.ndarray:
+    a = _f64(a_lt)
+    out = np.zeros((n, n))
+    lib().ora_unfold(n, _p(a), _p(out))
+    return out
+
+
+def swap_full2rs(rs2lt: np.ndarray, xlt: np.ndarray) -> np.ndarray:
+    m = np.ascontiguousarray(rs2lt, dtype=np.int64)
+    x = _f64(xlt)
+    out = np.zeros(len(m))
+    lib().ora_swap_full2rs(len(m), _p(m, _i64p), 1, x.shape[-1], _p(x), _p(out))
+    return out
+
+
+def swap_rs2full(rs2lt: np.ndarray, xrs: np.ndarray, xlt: np.ndarray, add: bool) -> np.ndarray:
+    m = np.ascontiguousarray(rs2lt, dtype=np.int64)
+    x = _f64(xrs)
+    out = _f64(xlt).copy()
+    lib().ora_swap_rs2full(len(m), _p(m, _i64p), 1, out.shape[-1], _p(x), _p(out), int(add))
+    return out
+
+
+def cho_vtra2(n: int, rs2lt: np.ndarray, lrs: np.ndarray, mo: np.ndarray) -> np.ndarray:
+    """lrs (jnum, nrs) row per vector; mo (n, norb) = Porb A2(norb,n) memory.
+    Returns ChoT A3(norb, jnum, n) as a numpy array of shape (n, jnum, norb)."""
+    m = np.ascontiguousarray(rs2lt, dtype=np.int64)
+    L = _f64(lrs)
+    C = _f64(mo)
+    jnum, nrs = L.shape
+    norb = C.shape[1]
+    out = np.zeros((n, jnum, norb))
+    lib().ora_cho_vtra2(n, nrs, _p(m, _i64p), jnum, _p(L), norb, _p(C), _p(out))
+    return out
+
+
+def cho_vtra0(n: int, rs2lt: np.ndarray, lrs: np.ndarray, mo: np.ndarray) -> np.ndarray:
+    """As cho_vtra2 but A3(norb, n, jnum), returned with shape (jnum, n, norb)."""
+    m = np.ascontiguousarray(rs2lt, dtype=np.int64)
+    L = _f64(lrs)
+    C = _f64(mo)
+    jnum, nrs = L.shape
+    norb = C.shape[1]
+    out = np.zeros((jnum, n, norb))
+    lib().ora_cho_vtra0(n, nrs, _p(m, _i64p), jnum, _p(L), norb, _p(C), _p(out))
+    return out
+
+
+def dgemm_tri_tn(A: np.ndarray, alpha: float = 1.0, beta: float = 0.0, c_lt=None) -> np.ndarray:
+    """C_LT = beta C + alpha A^T A for A given as (n, k) memory = Fortran A(k,n)."""
+    a = _f64(A)
+    n, k = a.shape
+    c = np.zeros(n * (n + 1) // 2) if c_lt is None else _f64(c_lt).copy()
+    lib().ora_dgemm_tri_tn(n, k, alpha, _p(a), k, _p(a), k, beta, _p(c))
+    return c
+
+
+def cho_fscf(store: CholStore, flt: np.ndarray, nocc, porbs, dlt: np.ndarray, exfac: float,
+             batch: int = 1 << 30) -> np.ndarray:
+    """CHO_FSCF with nforb = 0, niorb = nocc.  flt (nden, npair) is updated in a
+    copy which is returned; porbs[d] is (n, nocc[d]) (Porb A2(nocc,n) memory)."""
+    nden = len(nocc)
+    f = _f64(flt).reshape(nden, -1).copy()
+    nf = np.zeros(nden, dtype=np.int32)
+    ni = np.array(nocc, dtype=np.int32)
+    cs = [_f64(p) for p in porbs]
+    arr = (_dp * nden)(*[_p(c) for c in cs])
+    d = _f64(dlt)
+    rc = lib().ora_cho_fscf(ctypes.byref(store._c), nden, _p(f), _p(nf, _i32p), _p(ni, _i32p), arr,
+                            _p(d), float(exfac), int(batch))
+    if rc != 0:
+        raise RuntimeError(f"ora_cho_fscf rc={rc}")
+    return f
+
+
+def cho_fmcscf(store: CholStore, flt: np.ndarray, nk: int, naorb: int, factxi: float,
+               dlt: np.ndarray, do_active: bool, porb1, nchm: int, porb2, porb3, cmo,
+               nfro: int, norb: int, batch: int = 1 << 30):
+    """CHO_FMCSCF (nforb = 0, niorb = nk).  Returns (flt (2,npair), w_pwxy)."""
+    f = _f64(flt).reshape(2, -1).copy()
+    d = _f64(dlt).reshape(2, -1)
+    n = store.n
+    p1 = _f64(porb1) if nk else np.zeros(1)
+    p2 = _f64(porb2) if nchm else np.zeros(1)
+    p3 = _f64(porb3) if naorb else np.zeros(1)
+    cm = _f64(np.asarray(cmo).T) if cmo is not None else np.zeros(1)  # column-major CMO(n,n)
+    nxy = naorb * (naorb + 1) // 2
+    w = np.zeros(max(norb * naorb * nxy, 1))
+    rc = lib().ora_cho_fmcscf(ctypes.byref(store._c), _p(f), 0, nk, naorb, float(factxi), _p(d),
+                              int(do_active), _p(p1), nchm, _p(p2), _p(p3), _p(w), _p(cm), nfro,
+                              norb, 1.0, int(batch))
+    if rc != 0:
+        raise RuntimeError(f"ora_cho_fmcscf rc={rc}")
+    return f, w[:norb * naorb * nxy]
+
+
+def cd_incore(x: np.ndarray, thr: float, mxvec=None):
+    """Returns (rc, V (n, numcho)); x is a symmetric square matrix."""
+    a = _f64(x).copy()
+    n = a.shape[0]
+    mx = n if mxvec is None else int(mxvec)
+    vec = np.zeros((max(mx, 1), n))
+    num = np.zeros(1, dtype=np.int32)
+    rc = lib().ora_cd_incore(_p(a), n, _p(vec), mx, _p(num, _i32p), float(thr))
+    return int(rc), vec[:num[0]].T.copy()
+
+
+def energy(h: np.ndarray, dlt: np.ndarray, g: np.ndarray, enuc: float = 0.0) -> float:
+    h, d, g = _f64(h), _f64(dlt), _f64(g)
+    return float(lib().ora_energy(len(d), _p(h), _p(d), _p(g), float(enuc)))
