@@ -114,14 +114,14 @@ __global__ __launch_bounds__(kThreads, 2) void k_ht_f64(
       __syncthreads();
     }
   }
-  // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane%16 and rows
-  // i = 4*(lane/16) + r, r = 0..3.
+  // D[i][j] of v_mfma_f64_16x16x4f64 on gfx950: register r of lane l holds
+  // row i = 4*r + l/16, column j = l%16.
   const int a = a0 + 16 * w + (lane & 15);
   double *__restrict__ xj = X + jr * LDC * int64_t(n_pad);
 #pragma unroll
   for (int t = 0; t < OT; ++t)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) xj[int64_t(16 * t + 4 * (lane >> 4) + r) * n_pad + a] = acc[t][r];
+    for (int r = 0; r < 4; ++r) xj[int64_t(16 * t + 4 * r + (lane >> 4)) * n_pad + a] = acc[t][r];
 }
 
 // ---- SYRK ------------------------------------------------------------------
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_syrk_f64(
     for (int j = 0; j < MT; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int al = wm * WT + i * 16 + 4 * (lane >> 4) + r;
+        const int al = wm * WT + i * 16 + 4 * r + (lane >> 4);
         const int bl = wn * WT + j * 16 + (lane & 15);
         double *dst = kp + al * TM + bl;
         *dst = beta ? *dst + acc[i][j][r] : acc[i][j][r];

@@ -48,6 +48,14 @@ def _dev_f64(t: torch.Tensor, device) -> torch.Tensor:
     return t.to(device=device, dtype=torch.float64).contiguous()
 
 
+def _ld(t: torch.Tensor) -> int:
+    """Row stride of a contiguous 2-D tensor (stride(0) is arbitrary when the
+    leading dimension is 1)."""
+    if t.dim() != 2 or not t.is_contiguous():
+        raise ValueError("expected a contiguous 2-D tensor")
+    return t.shape[1]
+
+
 def _host_f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -179,7 +187,7 @@ class JKContext:
         nd = d.shape[0]
         j = torch.empty((nd, self.npair), dtype=torch.float64, device=self.device)
         v = torch.empty((self.nchol, nd), dtype=torch.float64, device=self.device) if want_v else None
-        self._check(self.lib.mjk_coulomb(self._h, nd, _ptr(d), d.stride(0), _ptr(j), j.stride(0), _ptr(v)))
+        self._check(self.lib.mjk_coulomb(self._h, nd, _ptr(d), _ld(d), _ptr(j), _ld(j), _ptr(v)))
         return (j, v) if want_v else j
 
     def exchange(self, corbs, precision="fp64") -> torch.Tensor:
@@ -190,7 +198,7 @@ class JKContext:
         nocc = np.array([c.shape[1] if c.dim() == 2 else 0 for c in cs], dtype=np.int32)
         ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
         k = torch.empty((nk, self.npair), dtype=torch.float64, device=self.device)
-        self._check(self.lib.mjk_exchange(self._h, nk, _ptr(nocc), ptrs, _ptr(k), k.stride(0),
+        self._check(self.lib.mjk_exchange(self._h, nk, _ptr(nocc), ptrs, _ptr(k), _ld(k),
                                           precision_code(precision)))
         return k
 
@@ -201,8 +209,8 @@ class JKContext:
         a = np.ascontiguousarray(fj if fj is not None else [0.0] * nd, dtype=np.float64)
         b = np.ascontiguousarray(fk if fk is not None else [0.0] * nd, dtype=np.float64)
         self._check(self.lib.mjk_fock_update(
-            self._h, nd, _ptr(flt), flt.stride(0), _ptr(jlt), jlt.stride(0) if jlt is not None else 0,
-            _ptr(js), _ptr(a), _ptr(klt), klt.stride(0) if klt is not None else 0, _ptr(b)))
+            self._h, nd, _ptr(flt), _ld(flt), _ptr(jlt), _ld(jlt) if jlt is not None else 0,
+            _ptr(js), _ptr(a), _ptr(klt), _ld(klt) if klt is not None else 0, _ptr(b)))
         return flt
 
     def energy(self, h, dlt, g) -> torch.Tensor:
