@@ -35,6 +35,7 @@ def parse():
     p.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--cpu-sample-vectors", type=int, default=256)
     return p.parse_args()
 
@@ -66,6 +67,47 @@ def cpu_baseline(n, M, o, seed, sample):
         "sample": f"oracle CHO_FSCF restatement (C, OpenMP) on {ms} of {M} vectors, "
                   f"nbas={n} nocc={o}, {dt:.2f} s, scaled x{M / ms:.1f}",
     }
+
+
+def energy_of(ctx, dlt, h, J, K):
+    """E = tr(D h) + 1/2 tr(D J) - 1/4 tr(D K) through the FP64 energy step:
+    G = J - 1/2 K (LT), E = sum h.D + 1/2 sum G.D."""
+    import torch
+
+    G = torch.zeros_like(J)
+    ctx.fock_update(G, J, [0], [1.0], K, [-0.5])
+    return ctx.energy(h, dlt, G).item()
+
+
+def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
+    """Times the exchange alone in FP64 and in each mixed mode (same inputs) and
+    reports speedups and |dE| against the FP64 build."""
+    import torch
+
+    J = ctx.coulomb(dlt)
+    res = {}
+    K64 = ctx.exchange([C], precision="fp64")
+    e64 = energy_of(ctx, dlt, h, J, K64)
+    tk = {}
+    for prec in ("fp64", "fp32"):
+        try:
+            ctx.exchange([C], precision=prec)
+        except Exception as exc:  # mode not built
+            res[prec] = {"error": str(exc)}
+            continue
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            K = ctx.exchange([C], precision=prec)
+        torch.cuda.synchronize()
+        tk[prec] = (time.perf_counter() - t0) / args.steps
+        e = energy_of(ctx, dlt, h, J, K)
+        res[prec] = {"exchange_ms": round(1e3 * tk[prec], 4), "E": e, "dE": abs(e - e64),
+                     "K_maxabs_err": float(torch.max(torch.abs(K - K64)).item())}
+    for prec in tk:
+        if prec != "fp64":
+            res[prec]["exchange_speedup_vs_fp64"] = round(tk["fp64"] / tk[prec], 3)
+    return res
 
 
 def main():
@@ -191,6 +233,8 @@ def main():
             "phase_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in phases.items()},
         },
     }
+    if args.precision == "fp64" and not args.no_mixed:
+        out["mixed"] = mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_step, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, M, o, args.seed, args.cpu_sample_vectors)
     print(json.dumps(out), flush=True)

@@ -159,7 +159,7 @@ int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
 int mjk_destroy(mjk_context *c) {
   if (!c) return MJK_OK;
   hipStreamSynchronize(c->stream);
-  for (DevBuf *b : {&c->ws_d, &c->ws_vpart, &c->ws_v, &c->ws_jpart, &c->ws_cbank, &c->ws_x,
+  for (DevBuf *b : {&c->ws_l32, &c->ws_d, &c->ws_vpart, &c->ws_v, &c->ws_jpart, &c->ws_cbank, &c->ws_x,
                     &c->ws_kpart, &c->ws_stage, &c->ws_j, &c->ws_k, &c->ws_red, &c->ws_misc})
     free_buf(*b);
   for (auto &r : c->rsets)
@@ -277,6 +277,7 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
     }
   }
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "upload sync"))) return rc;
+  c->l32_valid = false;
   for (int64_t j = jvec; j < jvec + jnum; ++j) {
     if (!c->loaded[size_t(j)]) {
       c->loaded[size_t(j)] = 1;

@@ -29,9 +29,32 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ inline v4d mfma_f64(double a, double b, v4d c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+// Precision traits of the exchange kernels.  row(lane, r) is the row of the
+// 16x16 result held in register r of `lane` (column = lane % 16); the FP64 and
+// FP32 16x16x4 instructions of gfx950 differ here.  The A operand of lane l is
+// A[l%16][l/16], the B operand B[l/16][l%16], for both.
+struct F64 {
+  typedef double T;
+  typedef v4d Acc;
+  static constexpr bool kFlush = false;  // accumulate in FP64 throughout
+  __device__ static Acc mma(T a, T b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+struct F32 {
+  typedef float T;
+  typedef v4f Acc;
+  static constexpr bool kFlush = true;  // FP32 partial sums folded into FP64
+  __device__ static Acc mma(T a, T b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// FP32 SYRK partial sums are folded into FP64 registers every kFlushRows
+// contraction rows, so no FP32 sum runs over more than kFlushRows products.
+constexpr int kFlushRows = 512;
 
 __device__ inline int64_t tri0(int64_t a) { return a * (a + 1) / 2; }
 
@@ -46,82 +69,124 @@ __device__ inline int64_t xcd_logical(int64_t nblocks) {
 
 // ---- orbital bank --------------------------------------------------------
 // Cb[a*ldc + off + k] = corb[k + a*nocc] for a < n, k < nocc.
+template <typename T>
 __global__ void k_pack_orb(const double *__restrict__ corb, int nocc, int n, int ldc, int off,
-                           double *__restrict__ cb) {
+                           T *__restrict__ cb) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= int64_t(nocc) * n) return;
   const int64_t a = i / nocc, k = i % nocc;
-  cb[a * ldc + off + k] = corb[i];
+  cb[a * ldc + off + k] = T(corb[i]);
+}
+
+// FP32 shadow of the resident vectors for the mixed-precision exchange.
+__global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, int64_t cnt) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < cnt;
+       i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = float(src[i]);
 }
 
 // ---- half-transform ------------------------------------------------------
-// grid.x = round_up(jc * ntile_a, 8), block = 256.
-// X[(jr*ldc + k)*n_pad + a] = sum_b Cb[b*ldc + k] * L^{j0+jr}_{ab}.
-template <int OT, int BK>
-__global__ __launch_bounds__(kThreads, 2) void k_ht_f64(
-    const double *__restrict__ L, int64_t ldl, int n, int n_pad, const double *__restrict__ Cb,
-    int64_t j0, int ntile_a, int64_t nblocks, double *__restrict__ X) {
+// grid.x = round_up(jc * ntile_a, 8), block = 256.  Workgroup (jr, a-tile of
+// TA columns) computes X[(jr*LDC + k)*n_pad + a] = sum_b Cb[b*LDC + k] L_ab for
+// all LDC = 16*OT bank columns k.  Wave w owns AO columns
+// a0 + w*TA/4 .. a0 + (w+1)*TA/4 - 1 (TA/64 MFMA column tiles) and all OT row
+// tiles.  The next b-chunk is fetched into registers while the current one is
+// multiplied out of LDS.
+template <class P, int OT, int TA, int BK>
+__global__ __launch_bounds__(kThreads) void k_ht(
+    const typename P::T *__restrict__ L, int64_t ldl, int n, int n_pad,
+    const typename P::T *__restrict__ Cb, int64_t j0, int ntile_a, int64_t nblocks,
+    typename P::T *__restrict__ X) {
+  typedef typename P::T T;
+  typedef typename P::Acc Acc;
   constexpr int LDC = 16 * OT;
-  __shared__ double sC[BK][LDC];
-  __shared__ double sL[BK][kTileA + 1];
+  constexpr int NB = TA / 64;            // 16-column MFMA tiles per wave
+  constexpr int SLD = TA + 1;            // LDS row stride of the L tile
+  constexpr int CN = BK * LDC / kThreads;
+  constexpr int LN = BK * TA / kThreads;
+  static_assert(BK * LDC % kThreads == 0 && BK * TA % kThreads == 0, "tile shape");
+  __shared__ T sC[BK * LDC];
+  __shared__ T sL[BK * SLD];
   const int64_t lb = xcd_logical(nblocks);
   if (lb < 0) return;
   const int64_t jr = lb / ntile_a;
-  const int a0 = int(lb % ntile_a) * kTileA;
+  const int a0 = int(lb % ntile_a) * TA;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const double *__restrict__ Lj = L + (j0 + jr) * ldl;
+  const T *__restrict__ Lj = L + (j0 + jr) * ldl;
 
-  v4d acc[OT];
-#pragma unroll
-  for (int t = 0; t < OT; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-
-  if (a0 < n) {
-    for (int b0 = 0; b0 < n; b0 += BK) {
-      // stage bank rows b0..b0+BK-1 (contiguous, zero padded beyond n)
-      const double *__restrict__ cs = Cb + int64_t(b0) * LDC;
-#pragma unroll
-      for (int i = tid; i < BK * LDC; i += kThreads) sC[i / LDC][i % LDC] = cs[i];
-      // unpack L^J[b0:b0+BK, a0:a0+64] into sL[b][a]
-      if (b0 >= a0 + kTileA) {  // every b > a: row b holds a contiguous
-#pragma unroll
-        for (int i = tid; i < BK * kTileA; i += kThreads) {
-          const int bb = i / kTileA, aa = i % kTileA, b = b0 + bb, a = a0 + aa;
-          sL[bb][aa] = (b < n && a < n) ? Lj[tri0(b) + a] : 0.0;
-        }
-      } else if (b0 + BK <= a0) {  // every b < a: row a holds b contiguous
-#pragma unroll
-        for (int i = tid; i < BK * kTileA; i += kThreads) {
-          const int bb = i % BK, aa = i / BK, b = b0 + bb, a = a0 + aa;
-          sL[bb][aa] = (a < n) ? Lj[tri0(a) + b] : 0.0;
-        }
-      } else {  // block straddles the diagonal
-#pragma unroll
-        for (int i = tid; i < BK * kTileA; i += kThreads) {
-          const int bb = i / kTileA, aa = i % kTileA, b = b0 + bb, a = a0 + aa;
-          double v = 0.0;
-          if (b < n && a < n) v = (b >= a) ? Lj[tri0(b) + a] : Lj[tri0(a) + b];
-          sL[bb][aa] = v;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int ks = 0; ks < BK / 4; ++ks) {
-        const int kr = ks * 4 + (lane >> 4);
-        const double bf = sL[kr][16 * w + (lane & 15)];
-#pragma unroll
-        for (int t = 0; t < OT; ++t) acc[t] = mfma_f64(sC[kr][16 * t + (lane & 15)], bf, acc[t]);
-      }
-      __syncthreads();
-    }
-  }
-  // D[i][j] of v_mfma_f64_16x16x4f64 on gfx950: register r of lane l holds
-  // row i = 4*r + l/16, column j = l%16.
-  const int a = a0 + 16 * w + (lane & 15);
-  double *__restrict__ xj = X + jr * LDC * int64_t(n_pad);
+  Acc acc[OT][NB];
 #pragma unroll
   for (int t = 0; t < OT; ++t)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) xj[int64_t(16 * t + 4 * r + (lane >> 4)) * n_pad + a] = acc[t][r];
+    for (int j = 0; j < NB; ++j) acc[t][j] = Acc{0, 0, 0, 0};
+
+  if (a0 < n) {
+    T rc[CN], rl[LN];
+    int lo[LN];
+    // global -> registers for the chunk starting at b0
+    auto fetch = [&](int b0) {
+      const T *__restrict__ cs = Cb + int64_t(b0) * LDC;
+#pragma unroll
+      for (int i = 0; i < CN; ++i) rc[i] = cs[tid + kThreads * i];
+      if (b0 >= a0 + TA) {  // every b > a: row b holds a contiguous
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
+          rl[i] = (b < n && a < n) ? Lj[tri0(b) + a] : T(0);
+          lo[i] = bb * SLD + aa;
+        }
+      } else if (b0 + BK <= a0) {  // every b < a: row a holds b contiguous
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+          const int e = tid + kThreads * i, bb = e % BK, aa = e / BK, b = b0 + bb, a = a0 + aa;
+          rl[i] = (a < n) ? Lj[tri0(a) + b] : T(0);
+          lo[i] = bb * SLD + aa;
+        }
+      } else {  // block straddles the diagonal
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
+          T v = T(0);
+          if (b < n && a < n) v = (b >= a) ? Lj[tri0(b) + a] : Lj[tri0(a) + b];
+          rl[i] = v;
+          lo[i] = bb * SLD + aa;
+        }
+      }
+    };
+    fetch(0);
+    for (int b0 = 0; b0 < n; b0 += BK) {
+      if (b0) __syncthreads();  // every wave is done with the previous chunk
+#pragma unroll
+      for (int i = 0; i < CN; ++i) sC[tid + kThreads * i] = rc[i];
+#pragma unroll
+      for (int i = 0; i < LN; ++i) sL[lo[i]] = rl[i];
+      __syncthreads();
+      if (b0 + BK < n) fetch(b0 + BK);
+#pragma unroll
+      for (int ks = 0; ks < BK / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        T bf[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) bf[j] = sL[kr * SLD + w * (TA / 4) + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int t = 0; t < OT; ++t) {
+          const T af = sC[kr * LDC + 16 * t + (lane & 15)];
+#pragma unroll
+          for (int j = 0; j < NB; ++j) acc[t][j] = P::mma(af, bf[j], acc[t][j]);
+        }
+      }
+    }
+  }
+  T *__restrict__ xj = X + jr * LDC * int64_t(n_pad);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int a = a0 + w * (TA / 4) + j * 16 + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < OT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        xj[int64_t(16 * t + P::row(lane, r)) * n_pad + a] = acc[t][j][r];
+  }
 }
 
 // ---- SYRK ------------------------------------------------------------------
@@ -129,74 +194,109 @@ __global__ __launch_bounds__(kThreads, 2) void k_ht_f64(
 // covers chunk vectors [s*jper, min(jc,(s+1)*jper)).  The contraction runs
 // over f = (J, k), k in [off, off+opad) of the bank columns.
 // kpart[(s*ntiles + t)*TM*TM + al*TM + bl] (+)= sum_f X[f][A*TM+al] X[f][B*TM+bl]
-template <int TM>
-__global__ __launch_bounds__(kThreads, 2) void k_syrk_f64(
-    const double *__restrict__ X, int n_pad, int ldc, int off, int opad, int64_t jc,
+// On diagonal tiles (A == B) only one operand is staged and the MFMA tiles
+// strictly above the diagonal are skipped (those entries are never read).
+template <class P, int TM, int BKF>
+__global__ __launch_bounds__(kThreads) void k_syrk(
+    const typename P::T *__restrict__ X, int n_pad, int ldc, int off, int opad, int64_t jc,
     int64_t jper, int ntiles, double *__restrict__ kpart, int beta) {
-  constexpr int BKF = 32;
+  typedef typename P::T T;
+  typedef typename P::Acc Acc;
   constexpr int WT = TM / 2;
   constexpr int MT = WT / 16;
-  __shared__ double sA[BKF][TM];
-  __shared__ double sB[BKF][TM];
-  __shared__ int64_t srow[BKF];
+  constexpr int PN = BKF * TM / kThreads;
+  static_assert(BKF * TM % kThreads == 0, "tile shape");
+  __shared__ T sA[BKF * TM];
+  __shared__ T sB[BKF * TM];
   const int tile = blockIdx.x % ntiles;
   const int s = blockIdx.x / ntiles;
   int A = int((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
   while (A * (A + 1) / 2 > tile) --A;
   while ((A + 1) * (A + 2) / 2 <= tile) ++A;
   const int B = tile - A * (A + 1) / 2;
+  const bool diag = (A == B);
   const int64_t ja = int64_t(s) * jper;
   const int64_t jz = min(jc, ja + jper);
-  const int64_t nf = (jz > ja ? jz - ja : 0) * opad;
+  const int nf = int((jz > ja ? jz - ja : 0) * opad);  // < 2^31, checked on the host
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
+  const bool idle = diag && wm < wn;  // wave tile entirely above the diagonal
 
-  v4d acc[MT][MT];
+  Acc acc[MT][MT];
+  double accd[P::kFlush ? MT : 1][P::kFlush ? MT : 1][4];
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
-    for (int j = 0; j < MT; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-
-  for (int64_t f0 = 0; f0 < nf; f0 += BKF) {
-    // X row offset of each contraction index f = (J, k) of this chunk
-    if (tid < BKF) {
-      const int64_t f = f0 + tid;
-      int64_t ro = -1;
-      if (f < nf) {
-        const int fi = int(f);  // nf < 2^31 (checked on the host)
-        ro = ((ja + fi / opad) * ldc + off + fi % opad) * int64_t(n_pad);
-      }
-      srow[tid] = ro;
-    }
-    __syncthreads();
+    for (int j = 0; j < MT; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+  if constexpr (P::kFlush) {
 #pragma unroll
-    for (int i = tid; i < BKF * TM; i += kThreads) {
-      const int ff = i / TM, cc = i % TM;
-      const int64_t ro = srow[ff];
-      double va = 0.0, vb = 0.0;
-      if (ro >= 0) {
-        va = X[ro + A * TM + cc];
-        vb = X[ro + B * TM + cc];
-      }
-      sA[ff][cc] = va;
-      sB[ff][cc] = vb;
-    }
-    __syncthreads();
+    for (int i = 0; i < MT; ++i)
 #pragma unroll
-    for (int ks = 0; ks < BKF / 4; ++ks) {
-      const int kr = ks * 4 + (lane >> 4);
-      double af[MT], bf[MT];
+      for (int j = 0; j < MT; ++j)
 #pragma unroll
-      for (int i = 0; i < MT; ++i) af[i] = sA[kr][wm * WT + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < MT; ++j) bf[j] = sB[kr][wn * WT + j * 16 + (lane & 15)];
+        for (int r = 0; r < 4; ++r) accd[i][j][r] = 0.0;
+  }
+  auto flush = [&]() {
+    if constexpr (P::kFlush) {
 #pragma unroll
       for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int j = 0; j < MT; ++j) acc[i][j] = mfma_f64(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < MT; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) accd[i][j][r] += double(acc[i][j][r]);
+          acc[i][j] = Acc{0, 0, 0, 0};
+        }
+    }
+  };
+
+  T ra[PN], rb[PN];
+  auto fetch = [&](int f0) {
+#pragma unroll
+    for (int i = 0; i < PN; ++i) {
+      const int e = tid + kThreads * i, ff = e / TM, cc = e % TM, f = f0 + ff;
+      T va = T(0), vb = T(0);
+      if (f < nf) {
+        const int64_t ro = ((ja + f / opad) * ldc + off + f % opad) * int64_t(n_pad);
+        va = X[ro + A * TM + cc];
+        if (!diag) vb = X[ro + B * TM + cc];
+      }
+      ra[i] = va;
+      rb[i] = vb;
+    }
+  };
+  if (nf > 0) fetch(0);
+  for (int f0 = 0; f0 < nf; f0 += BKF) {
+    if (f0) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PN; ++i) sA[tid + kThreads * i] = ra[i];
+    if (!diag) {
+#pragma unroll
+      for (int i = 0; i < PN; ++i) sB[tid + kThreads * i] = rb[i];
     }
     __syncthreads();
+    if (f0 + BKF < nf) fetch(f0 + BKF);
+    if (!idle) {
+      const T *__restrict__ sb = diag ? sA : sB;
+#pragma unroll
+      for (int ks = 0; ks < BKF / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        T af[MT], bf[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) af[i] = sA[kr * TM + wm * WT + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < MT; ++j) bf[j] = sb[kr * TM + wn * WT + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+          for (int j = 0; j < MT; ++j)
+            if (!diag || wm > wn || i >= j) acc[i][j] = P::mma(af[i], bf[j], acc[i][j]);
+      }
+    }
+    if constexpr (P::kFlush) {
+      if ((f0 + BKF) % kFlushRows == 0) flush();
+    }
   }
+  flush();
   double *__restrict__ kp = kpart + (int64_t(s) * ntiles + tile) * TM * TM;
 #pragma unroll
   for (int i = 0; i < MT; ++i)
@@ -204,10 +304,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_syrk_f64(
     for (int j = 0; j < MT; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int al = wm * WT + i * 16 + 4 * r + (lane >> 4);
+        const int al = wm * WT + i * 16 + P::row(lane, r);
         const int bl = wn * WT + j * 16 + (lane & 15);
+        double v;
+        if constexpr (P::kFlush) v = accd[i][j][r];
+        else v = acc[i][j][r];
         double *dst = kp + al * TM + bl;
-        *dst = beta ? *dst + acc[i][j][r] : acc[i][j][r];
+        *dst = beta ? *dst + v : v;
       }
 }
 
@@ -229,35 +332,47 @@ __global__ void k_reduce_k(const double *__restrict__ kpart, int n, int tm, int 
   }
 }
 
-template <int OT>
-void launch_ht(Context *c, const double *cb, int64_t j0, int64_t jc, double *X) {
-  const int ntile_a = c->n_pad / kTileA;
+template <class P, int OT, int TA, int BK>
+void launch_ht(Context *c, const typename P::T *L, const typename P::T *cb, int64_t j0, int64_t jc,
+               typename P::T *X) {
+  const int ntile_a = c->n_pad / TA;
   const int64_t nblocks = jc * ntile_a;
   const int64_t grid = round_up(nblocks, 8);
-  constexpr int BK = OT <= 10 ? 32 : 16;
-  hipLaunchKernelGGL((k_ht_f64<OT, BK>), dim3(unsigned(grid)), dim3(kThreads), 0, c->stream, c->L,
+  hipLaunchKernelGGL((k_ht<P, OT, TA, BK>), dim3(unsigned(grid)), dim3(kThreads), 0, c->stream, L,
                      c->ldl, c->n, c->n_pad, cb, j0, ntile_a, nblocks, X);
 }
 
-
-}  // namespace
-
-void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, double *X) {
+// Wide AO tiles while the accumulators fit, narrow ones for many orbitals.
+template <class P>
+void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T *cb, int64_t j0,
+                 int64_t jn, typename P::T *X) {
   switch (ot) {
-    case 1: launch_ht<1>(c, cb, j0, jn, X); break;
-    case 2: launch_ht<2>(c, cb, j0, jn, X); break;
-    case 3: launch_ht<3>(c, cb, j0, jn, X); break;
-    case 4: launch_ht<4>(c, cb, j0, jn, X); break;
-    case 6: launch_ht<6>(c, cb, j0, jn, X); break;
-    case 8: launch_ht<8>(c, cb, j0, jn, X); break;
-    case 10: launch_ht<10>(c, cb, j0, jn, X); break;
-    case 12: launch_ht<12>(c, cb, j0, jn, X); break;
-    default: launch_ht<16>(c, cb, j0, jn, X); break;
+    case 1: launch_ht<P, 1, 128, 32>(c, L, cb, j0, jn, X); break;
+    case 2: launch_ht<P, 2, 128, 32>(c, L, cb, j0, jn, X); break;
+    case 3: launch_ht<P, 3, 128, 32>(c, L, cb, j0, jn, X); break;
+    case 4: launch_ht<P, 4, 128, 32>(c, L, cb, j0, jn, X); break;
+    case 6: launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X); break;
+    case 8: launch_ht<P, 8, 64, 32>(c, L, cb, j0, jn, X); break;
+    case 10: launch_ht<P, 10, 64, 32>(c, L, cb, j0, jn, X); break;
+    case 12: launch_ht<P, 12, 64, 16>(c, L, cb, j0, jn, X); break;
+    default: launch_ht<P, 16, 64, 16>(c, L, cb, j0, jn, X); break;
   }
 }
 
-int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
-              std::vector<int> &off, std::vector<int> &opad, int *ot_out, int *tot_out) {
+template <class P>
+void syrk_launch(Context *c, int tm, dim3 grid, const typename P::T *X, int ldc, int off, int opad,
+                 int64_t jn, int64_t jper, int ntiles, double *kpart, int beta) {
+  if (tm == 128)
+    hipLaunchKernelGGL((k_syrk<P, 128, 16>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc,
+                       off, opad, jn, jper, ntiles, kpart, beta);
+  else
+    hipLaunchKernelGGL((k_syrk<P, 64, 32>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc,
+                       off, opad, jn, jper, ntiles, kpart, beta);
+}
+
+// Layout of the bank for nk sets; returns MJK_OK and ot = 0 when empty.
+int bank_layout(Context *c, int32_t nk, const int32_t *nocc, std::vector<int> &off,
+                std::vector<int> &opad, int *ot_out, int *tot_out) {
   off.assign(size_t(nk), 0);
   opad.assign(size_t(nk), 0);
   int tot = 0;
@@ -272,49 +387,58 @@ int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *
   *ot_out = 0;
   if (tot == 0) return MJK_OK;
   static const int kOT[] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
-  int ot = 0;
   for (int v : kOT)
     if (16 * v >= tot) {
-      ot = v;
-      break;
+      *ot_out = v;
+      return MJK_OK;
     }
-  if (!ot) return set_error(c, MJK_ERR_ARG, "more than 256 orbital columns in one call");
-  const int ldc = 16 * ot;
+  return set_error(c, MJK_ERR_ARG, "more than 256 orbital columns in one call");
+}
+
+template <typename T>
+int pack_bank_t(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                const std::vector<int> &off, int ldc) {
   const int n = c->n, n_pad = c->n_pad;
+  const size_t bytes = sizeof(T) * size_t(n_pad) * ldc;
   int rc;
-  if ((rc = ensure(c, c->ws_cbank, sizeof(double) * int64_t(n_pad) * ldc))) return rc;
-  double *cb = static_cast<double *>(c->ws_cbank.ptr);
+  if ((rc = ensure(c, c->ws_cbank, bytes))) return rc;
+  T *cb = static_cast<T *>(c->ws_cbank.ptr);
   phase_begin(c);
-  if ((rc = hip_check(c, hipMemsetAsync(cb, 0, sizeof(double) * int64_t(n_pad) * ldc, c->stream),
-                      "memset bank")))
-    return rc;
+  if ((rc = hip_check(c, hipMemsetAsync(cb, 0, bytes, c->stream), "memset bank"))) return rc;
   for (int d = 0; d < nk; ++d) {
     if (nocc[d] == 0) continue;
     const int64_t cnt = int64_t(nocc[d]) * n;
-    hipLaunchKernelGGL(k_pack_orb, dim3(unsigned(ceil_div(cnt, 256))), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_pack_orb<T>, dim3(unsigned(ceil_div(cnt, 256))), dim3(256), 0, c->stream,
                        corb[d], nocc[d], n, ldc, off[size_t(d)], cb);
   }
   phase_end(c, MJK_PHASE_PACK);
-  *ot_out = ot;
   return MJK_OK;
 }
 
-int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
-             int64_t ldk, int32_t precision) {
-  if (nk < 1) return set_error(c, MJK_ERR_ARG, "exchange: nk < 1");
-  if (ldk < c->np) return set_error(c, MJK_ERR_ARG, "exchange: ldk < nbas(nbas+1)/2");
-  if (precision != MJK_PREC_FP64)
-    return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
-  if (c->nloaded != c->nchol)
-    return set_error(c, MJK_ERR_SHORT_READ, "exchange: not all Cholesky vectors were uploaded");
+// FP32 copy of L, made once after the vectors change.
+int ensure_l32(Context *c) {
+  if (c->l32_valid) return MJK_OK;
+  const int64_t cnt = c->nchol * c->ldl;
+  int rc;
+  if ((rc = ensure(c, c->ws_l32, sizeof(float) * size_t(cnt)))) return rc;
+  phase_begin(c);
+  hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, c->stream, c->L,
+                     static_cast<float *>(c->ws_l32.ptr), cnt);
+  phase_end(c, MJK_PHASE_PACK);
+  c->l32_valid = true;
+  return hip_check(c, hipGetLastError(), "L to FP32");
+}
+
+template <class P>
+int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
+               int64_t ldk) {
+  typedef typename P::T T;
   const int n = c->n, n_pad = c->n_pad;
   const int64_t M = c->nchol;
   int rc;
-
-  // ---- orbital bank (all sets of this call, one pass over L)
   std::vector<int> off, opad;
   int ot = 0, tot = 0;
-  if ((rc = pack_bank(c, nk, nocc, corb, off, opad, &ot, &tot))) return rc;
+  if ((rc = bank_layout(c, nk, nocc, off, opad, &ot, &tot))) return rc;
   if (tot == 0) {  // no orbitals: K = 0
     for (int d = 0; d < nk; ++d)
       if ((rc = hip_check(c, hipMemsetAsync(klt + d * ldk, 0, sizeof(double) * c->np, c->stream),
@@ -323,68 +447,98 @@ int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *c
     return MJK_OK;
   }
   const int ldc = 16 * ot;
-  const double *cb = static_cast<const double *>(c->ws_cbank.ptr);
+  if ((rc = pack_bank_t<T>(c, nk, nocc, corb, off, ldc))) return rc;
+  const T *cb = static_cast<const T *>(c->ws_cbank.ptr);
+  const T *L;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    L = c->L;
+  } else {
+    if ((rc = ensure_l32(c))) return rc;
+    L = static_cast<const float *>(c->ws_l32.ptr);
+  }
 
   // ---- chunking over vectors (X budget)
-  const int64_t xvec_bytes = sizeof(double) * int64_t(ldc) * n_pad;
+  const int64_t xvec_bytes = int64_t(sizeof(T)) * ldc * n_pad;
   int64_t jc = std::max<int64_t>(1, std::min<int64_t>(M, c->x_budget / xvec_bytes));
   const int64_t nchunk = ceil_div(M, jc);
   jc = ceil_div(M, nchunk);
   if (jc * ldc >= (int64_t(1) << 31))
     return set_error(c, MJK_ERR_ARG, "exchange: chunk contraction length exceeds 2^31");
   if ((rc = ensure(c, c->ws_x, size_t(xvec_bytes) * jc))) return rc;
-  double *X = static_cast<double *>(c->ws_x.ptr);
+  T *X = static_cast<T *>(c->ws_x.ptr);
 
   // ---- SYRK tiling and slices
   int tm = c->tile_syrk ? c->tile_syrk : (n_pad >= 512 && n_pad % 128 == 0 ? 128 : 64);
   if (n_pad % tm) tm = 64;
   const int ntile1 = n_pad / tm;
   const int ntiles = ntile1 * (ntile1 + 1) / 2;
-  std::vector<int> nsplit(nk);
-  std::vector<int64_t> kbase(nk + 1, 0);
+  std::vector<int> nsplit(static_cast<size_t>(nk));
+  std::vector<int64_t> kbase(size_t(nk) + 1, 0);
   int active = 0;
   for (int d = 0; d < nk; ++d) active += nocc[d] > 0;
   for (int d = 0; d < nk; ++d) {
-    int64_t want = ceil_div(1024, int64_t(ntiles) * std::max(active, 1));
-    nsplit[d] = nocc[d] > 0 ? int(std::max<int64_t>(1, std::min<int64_t>(want, jc))) : 0;
-    kbase[d + 1] = kbase[d] + int64_t(nsplit[d]) * ntiles * tm * tm;
+    const int64_t want = ceil_div(1024, int64_t(ntiles) * std::max(active, 1));
+    nsplit[size_t(d)] = nocc[d] > 0 ? int(std::max<int64_t>(1, std::min<int64_t>(want, jc))) : 0;
+    kbase[size_t(d) + 1] = kbase[size_t(d)] + int64_t(nsplit[size_t(d)]) * ntiles * tm * tm;
   }
-  if ((rc = ensure(c, c->ws_kpart, sizeof(double) * std::max<int64_t>(kbase[nk], 1)))) return rc;
+  if ((rc = ensure(c, c->ws_kpart, sizeof(double) * std::max<int64_t>(kbase[size_t(nk)], 1))))
+    return rc;
   double *kpart = static_cast<double *>(c->ws_kpart.ptr);
 
   for (int64_t ch = 0; ch < nchunk; ++ch) {
     const int64_t j0 = ch * jc;
     const int64_t jn = std::min(jc, M - j0);
     phase_begin(c);
-    ht_launch(c, ot, cb, j0, jn, X);
+    ht_dispatch<P>(c, ot, L, cb, j0, jn, X);
     phase_end(c, MJK_PHASE_HALF_TRANSFORM);
     phase_begin(c);
     for (int d = 0; d < nk; ++d) {
-      if (!nsplit[d]) continue;
-      const int64_t jper = ceil_div(jn, nsplit[d]);
-      const dim3 grid(unsigned(ntiles * nsplit[d]));
-      if (tm == 128)
-        hipLaunchKernelGGL(k_syrk_f64<128>, grid, dim3(kThreads), 0, c->stream, X, n_pad, ldc,
-                           off[d], opad[d], jn, jper, ntiles, kpart + kbase[d], ch > 0 ? 1 : 0);
-      else
-        hipLaunchKernelGGL(k_syrk_f64<64>, grid, dim3(kThreads), 0, c->stream, X, n_pad, ldc,
-                           off[d], opad[d], jn, jper, ntiles, kpart + kbase[d], ch > 0 ? 1 : 0);
+      if (!nsplit[size_t(d)]) continue;
+      const int64_t jper = ceil_div(jn, nsplit[size_t(d)]);
+      syrk_launch<P>(c, tm, dim3(unsigned(ntiles * nsplit[size_t(d)])), X, ldc, off[size_t(d)],
+                     opad[size_t(d)], jn, jper, ntiles, kpart + kbase[size_t(d)], ch > 0 ? 1 : 0);
     }
     phase_end(c, MJK_PHASE_EXCHANGE);
   }
   phase_begin(c);
   for (int d = 0; d < nk; ++d) {
-    if (!nsplit[d]) {
+    if (!nsplit[size_t(d)]) {
       if ((rc = hip_check(c, hipMemsetAsync(klt + d * ldk, 0, sizeof(double) * c->np, c->stream),
                           "memset K")))
         return rc;
       continue;
     }
-    hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n)), dim3(256), 0, c->stream, kpart + kbase[d], n,
-                       tm, ntiles, nsplit[d], klt + d * ldk);
+    hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n)), dim3(256), 0, c->stream,
+                       kpart + kbase[size_t(d)], n, tm, ntiles, nsplit[size_t(d)], klt + d * ldk);
   }
   phase_end(c, MJK_PHASE_REDUCE);
   return hip_check(c, hipGetLastError(), "exchange launch");
+}
+
+}  // namespace
+
+int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+              std::vector<int> &off, std::vector<int> &opad, int *ot, int *tot) {
+  int rc = bank_layout(c, nk, nocc, off, opad, ot, tot);
+  if (rc || *tot == 0) return rc;
+  return pack_bank_t<double>(c, nk, nocc, corb, off, 16 * *ot);
+}
+
+void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, double *X) {
+  ht_dispatch<F64>(c, ot, c->L, cb, j0, jn, X);
+}
+
+int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
+             int64_t ldk, int32_t precision) {
+  if (nk < 1) return set_error(c, MJK_ERR_ARG, "exchange: nk < 1");
+  if (ldk < c->np) return set_error(c, MJK_ERR_ARG, "exchange: ldk < nbas(nbas+1)/2");
+  if (c->nloaded != c->nchol)
+    return set_error(c, MJK_ERR_SHORT_READ, "exchange: not all Cholesky vectors were uploaded");
+  switch (precision) {
+    case MJK_PREC_FP64: return exchange_t<F64>(c, nk, nocc, corb, klt, ldk);
+    case MJK_PREC_FP32: return exchange_t<F32>(c, nk, nocc, corb, klt, ldk);
+    default: return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
+  }
 }
 
 }  // namespace mjk

@@ -54,6 +54,8 @@ struct Context {
   int32_t tile_syrk = 0;                  // 0 = automatic, else 64 or 128
 
   // workspaces
+  DevBuf ws_l32;          // FP32 shadow of L (mixed-precision exchange)
+  bool l32_valid = false;
   DevBuf ws_d, ws_vpart, ws_v, ws_jpart, ws_cbank, ws_x, ws_kpart, ws_stage, ws_j,
       ws_k, ws_red, ws_misc;
 
