@@ -160,6 +160,30 @@ int mjk_energy(mjk_context *ctx, const double *h, const double *dlt,
 int mjk_puvx(mjk_context *ctx, int32_t naorb, const double *porb3, const double *cmo,
              int32_t nfro, int32_t norb, double *w);
 
+/* ---- energy gradient (device pointers; no reference counterpart) -------
+ * For E = tr(P h) + 1/2 sum_J V_J^2 - 1/4 sum_J tr(P L^J P L^J), P = 2 C C^T
+ * (the energy of src/scf/eneclc.F90 with the Fock matrix of CHO_FSCF):
+ * dE/dC = 4 F C and dE/dL^J = V_J P - 1/2 P L^J P. */
+
+/* Folded LT density dlt = fold(scale * C C^T) for C = corb (nbas x nocc,
+ * orbital index fastest): the Fold step of src/misc_util/fold.F90 applied to
+ * the density the SCF builds from its orbitals (src/scf/dmat.F90). */
+int mjk_density(mjk_context *ctx, int32_t nocc, const double *corb, double scale,
+                double *dlt);
+
+/* dc = scale * F C, F in plain LT storage, corb and dc nbas x nocc (orbital
+ * index fastest).  With scale = 4 and F = h + J - 1/2 K this is dE/dC. */
+int mjk_grad_orbitals(mjk_context *ctx, int32_t nocc, const double *corb, const double *flt,
+                      double scale, double *dc);
+
+/* dl(J*ldd + LT(a,b)) = s_ab * (alpha V_J P_ab + beta (C Z^J C^T)_ab),
+ * Z^J = C^T L^J C, s_ab = 2 for a != b and 1 on the diagonal (packed
+ * parameters).  p_lt is P in plain LT storage, or NULL for P = 2 C C^T; v is
+ * V_J (nchol values, e.g. from mjk_coulomb).  alpha = 1, beta = -2 gives
+ * dE/dL.  X^J = C^T L^J is recomputed in chunks bounded by the X budget. */
+int mjk_grad_vectors(mjk_context *ctx, int32_t nocc, const double *corb, const double *p_lt,
+                     const double *v, double alpha, double beta, double *dl, int64_t ldd);
+
 /* Waits for all work queued on the context stream. */
 int mjk_sync(mjk_context *ctx);
 

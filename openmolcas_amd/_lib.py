@@ -47,6 +47,9 @@ SIGNATURES = {
     "mjk_fock_update": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mjk_energy": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "mjk_puvx": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _i32, _vp]),
+    "mjk_density": (_c.c_int, [_vp, _i32, _vp, _c.c_double, _vp]),
+    "mjk_grad_orbitals": (_c.c_int, [_vp, _i32, _vp, _vp, _c.c_double, _vp]),
+    "mjk_grad_vectors": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _c.c_double, _c.c_double, _vp, _i64]),
     "mjk_sync": (_c.c_int, [_vp]),
     "mjk_set_profiling": (_c.c_int, [_vp, _i32]),
     "mjk_phase_times": (_c.c_int, [_vp, _vp, _vp]),

@@ -229,6 +229,38 @@ class JKContext:
         self._check(self.lib.mjk_puvx(self._h, na, _ptr(c3), _ptr(cm), int(nfro), int(norb), _ptr(w)))
         return w
 
+    def density(self, corb: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+        """Folded LT of scale * C C^T for C (nbas, nocc) on the device."""
+        c = _dev_f64(corb, self.device)
+        d = torch.empty((1, self.npair), dtype=torch.float64, device=self.device)
+        self._check(self.lib.mjk_density(self._h, c.shape[1], _ptr(c), float(scale), _ptr(d)))
+        return d
+
+    def grad_orbitals(self, corb: torch.Tensor, flt: torch.Tensor, scale: float) -> torch.Tensor:
+        """scale * F C with F in plain LT storage."""
+        c = _dev_f64(corb, self.device)
+        dc = torch.empty_like(c)
+        self._check(self.lib.mjk_grad_orbitals(self._h, c.shape[1], _ptr(c), _ptr(flt), float(scale), _ptr(dc)))
+        return dc
+
+    def grad_vectors(self, corb: torch.Tensor, v: torch.Tensor, alpha: float, beta: float,
+                     p_lt=None, out=None) -> torch.Tensor:
+        """(nchol, npair) packed gradient s_ab (alpha V_J P_ab + beta (C Z^J C^T)_ab)."""
+        c = _dev_f64(corb, self.device)
+        vv = v.to(torch.float64).contiguous()
+        dl = out if out is not None else torch.empty((self.nchol, self.npair), dtype=torch.float64,
+                                                     device=self.device)
+        self._check(self.lib.mjk_grad_vectors(self._h, c.shape[1], _ptr(c), _ptr(p_lt), _ptr(vv),
+                                              float(alpha), float(beta), _ptr(dl), _ld(dl)))
+        return dl
+
+    def ensure_vectors(self, L: torch.Tensor) -> None:
+        """Uploads L (nchol, npair) unless this exact tensor version is resident."""
+        key = (L.data_ptr(), L._version, tuple(L.shape))
+        if getattr(self, "_resident", None) != key:
+            self.upload_packed(L.detach())
+            self._resident = key
+
     # ------------------------------------------------------------ measurement
     def set_profiling(self, on: bool):
         self._check(self.lib.mjk_set_profiling(self._h, int(bool(on))))

@@ -1,0 +1,140 @@
+"""Backward of the closed-shell energy (hand-written HIP gradient kernels).
+
+Checked against central finite differences of the oracle energy (step 1e-5)
+and against the analytic orbital gradient 4 F C.  Tolerances are written per
+check: FD error of a step-1e-5 central difference is O(eps^2 * E''') plus
+O(roundoff / eps), so |fd - analytic| <= 1e-6 + 1e-6 |analytic| at oracle
+sizes; at full size (n=1200, nchol=6000, nocc=150) FD is taken on the GPU's
+FP64 energy and the bound is 2e-4 + 1e-6 |analytic| (E ~ 5e6 Eh, so roundoff
+of E / step ~ 1e-4).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from openmolcas_amd.autograd import closed_shell_energy
+from openmolcas_amd.fock import JKContext
+from openmolcas_amd.synthetic import (chol_vectors_np, chol_vectors_torch, fold, hcore_np, npair,
+                                      orbitals_np, orbitals_torch, pack, unpack)
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_energy(C, L, h):
+    n = C.shape[0]
+    P = 2.0 * C @ C.T
+    st = oracle.CholStore.full(L)
+    g = oracle.cho_fscf(st, np.zeros((1, npair(n))), [C.shape[1]], [np.sqrt(2.0) * C], fold(P), 0.5)[0]
+    return oracle.energy(h, fold(P), g), g
+
+
+def test_energy_gradients_match_finite_differences(gpu):
+    n, M, o = 24, 20, 4
+    L = chol_vectors_np(n, M, seed=70)
+    C = orbitals_np(n, o, seed=71)
+    h = hcore_np(n, seed=72)
+    jk = JKContext(n, M, device=gpu)
+    Ct = torch.tensor(C, device=gpu, requires_grad=True)
+    Lt = torch.tensor(L, device=gpu, requires_grad=True)
+    ht = torch.tensor(h, device=gpu, requires_grad=True)
+    E = closed_shell_energy(Ct, Lt, ht, jk)
+    E.backward()
+    e_ref, g_ref = oracle_energy(C, L, h)
+    assert E.item() == pytest.approx(e_ref, rel=1e-12, abs=1e-10)
+    # dE/dC = 4 F C (analytic) with F = h + G from the oracle
+    F = unpack(h + g_ref, n)
+    assert np.allclose(Ct.grad.cpu().numpy(), 4.0 * F @ C, rtol=1e-10, atol=1e-10)
+    # dE/dh = fold(P)
+    assert np.allclose(ht.grad.cpu().numpy(), fold(2.0 * C @ C.T), rtol=1e-12, atol=1e-13)
+    eps = 1e-5
+    rng = np.random.default_rng(73)
+    dC = Ct.grad.cpu().numpy()
+    for _ in range(8):
+        a, k = int(rng.integers(n)), int(rng.integers(o))
+        Cp, Cm = C.copy(), C.copy()
+        Cp[a, k] += eps
+        Cm[a, k] -= eps
+        fd = (oracle_energy(Cp, L, h)[0] - oracle_energy(Cm, L, h)[0]) / (2 * eps)
+        assert abs(fd - dC[a, k]) <= 1e-6 + 1e-6 * abs(dC[a, k])
+    dL = Lt.grad.cpu().numpy()
+    diag = np.array([p * (p + 1) // 2 + p for p in range(n)])
+    picks = [(int(rng.integers(M)), int(rng.integers(npair(n)))) for _ in range(10)]
+    picks += [(int(rng.integers(M)), int(diag[rng.integers(n)])) for _ in range(4)]
+    for j, p in picks:
+        Lp, Lm = L.copy(), L.copy()
+        Lp[j, p] += eps
+        Lm[j, p] -= eps
+        fd = (oracle_energy(C, Lp, h)[0] - oracle_energy(C, Lm, h)[0]) / (2 * eps)
+        assert abs(fd - dL[j, p]) <= 1e-6 + 1e-6 * abs(dL[j, p]), (j, p, fd, dL[j, p])
+
+
+def test_vector_gradient_chunked_equals_single_pass(gpu):
+    n, M, o = 70, 37, 9
+    L = torch.tensor(chol_vectors_np(n, M, seed=80), device=gpu)
+    C = torch.tensor(orbitals_np(n, o, seed=81), device=gpu)
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(L)
+    dlt = jk.density(C, 2.0)
+    _, V = jk.coulomb(dlt, want_v=True)
+    g1 = jk.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0).cpu().numpy()
+    jk.set_x_budget(8 * 6 * (2 * 16 * 128 + 16 * 16))  # 6 vectors per chunk
+    g2 = jk.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0).cpu().numpy()
+    assert np.max(np.abs(g1 - g2)) <= 1e-13 * max(1.0, np.max(np.abs(g1)))
+
+
+def test_full_size_backward_n1200_m6000_o150(gpu):
+    """Mixed precision plus backward workload: nbas=1200, nchol=6000, nocc=150.
+    autograd dE/dC and dE/dL against central differences (step 1e-5) on 32
+    random entries each, using the FP64 forward on the GPU."""
+    n, M, o = 1200, 6000, 150
+    L = chol_vectors_torch(n, M, seed=3, device=gpu)
+    C = orbitals_torch(n, o, seed=4, device=gpu)
+    g = torch.Generator(device=gpu)
+    g.manual_seed(5)
+    hs = torch.randn((n, n), generator=g, dtype=torch.float64, device=gpu)
+    ii, jj = torch.tril_indices(n, n, device=gpu)
+    h = ((hs + hs.T) / (2.0 * n))[ii, jj].contiguous()
+    jk = JKContext(n, M, device=gpu)
+    Ct = C.clone().requires_grad_(True)
+    Lt = L.requires_grad_(True)
+    E = closed_shell_energy(Ct, Lt, h, jk)
+    E.backward()
+    dC = Ct.grad
+    dL = Lt.grad
+    eps = 1e-5
+    rng = np.random.default_rng(6)
+
+    def energy_C(Cm):
+        return closed_shell_energy(Cm, Lt.detach(), h, jk).item()
+
+    with torch.no_grad():
+        for _ in range(32):
+            a, k = int(rng.integers(n)), int(rng.integers(o))
+            Cp = C.clone(); Cp[a, k] += eps
+            Cm = C.clone(); Cm[a, k] -= eps
+            fd = (energy_C(Cp) - energy_C(Cm)) / (2 * eps)
+            an = dC[a, k].item()
+            assert abs(fd - an) <= 2e-4 + 1e-6 * abs(an), (a, k, fd, an)
+        for _ in range(32):
+            j, p = int(rng.integers(M)), int(rng.integers(npair(n)))
+            row = Lt[j].detach().clone()
+            vals = []
+            for s in (+1.0, -1.0):
+                r = row.clone(); r[p] += s * eps
+                jk.upload_vectors(0, j, r.unsqueeze(0))
+                vals.append(_energy_resident(jk, C, h))
+            jk.upload_vectors(0, j, row.unsqueeze(0))
+            fd = (vals[0] - vals[1]) / (2 * eps)
+            an = dL[j, p].item()
+            assert abs(fd - an) <= 2e-4 + 1e-6 * abs(an), (j, p, fd, an)
+
+
+def _energy_resident(jk, C, h):
+    """Energy with the vectors currently resident in the context (FP64)."""
+    dlt = jk.density(C, scale=2.0)
+    J = jk.coulomb(dlt)
+    K = jk.exchange([C])
+    G = torch.zeros_like(J)
+    jk.fock_update(G, J, [0], [1.0], K, [-1.0])
+    return jk.energy(h.reshape(1, -1), dlt, G).item()
