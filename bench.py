@@ -36,13 +36,16 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
-    p.add_argument("--cpu-sample-vectors", type=int, default=256)
+    p.add_argument("--copy-probe", action="store_true",
+                   help="after the timed region, copy 1 GiB device to device (calibrates HBM counters)")
+    p.add_argument("--cpu-sample-vectors", type=int, default=1 << 30)
     return p.parse_args()
 
 
-def cpu_baseline(n, M, o, seed, sample):
-    """The CPU oracle (C, OpenMP) on the first `sample` vectors of the same
-    workload, scaled to the full nchol: builds/s of CHO_FSCF-shaped work."""
+def cpu_baseline(n, M, o, seed, sample, repeats=5):
+    """The CPU oracle (C restatement of CHO_FSCF, OpenMP over the host cores) on
+    the same workload; `sample` vectors (all by default), median of `repeats`
+    builds, scaled to the full nchol when sampled."""
     import numpy as np
 
     import oracle
@@ -54,18 +57,22 @@ def cpu_baseline(n, M, o, seed, sample):
     dlt = fold(C @ C.T)
     st = oracle.CholStore.full(L)
     flt = np.zeros((1, npair(n)))
-    oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=64)  # warm
-    t0 = time.perf_counter()
-    oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=64)
-    dt = time.perf_counter() - t0
+    oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)  # warm
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": round(1.0 / (dt * M / ms), 6),
         "unit": "builds/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle CHO_FSCF restatement (C, OpenMP) on {ms} of {M} vectors, "
-                  f"nbas={n} nocc={o}, {dt:.2f} s, scaled x{M / ms:.1f}",
+        "sample": f"oracle CHO_FSCF restatement (C, OpenMP, {threads} threads) on {ms} of {M} vectors, "
+                  f"nbas={n} nocc={o}, batch 256 vectors, median of {repeats}: {dt:.2f} s per build"
+                  + (f", scaled x{M / ms:.1f}" if ms < M else ""),
     }
 
 
@@ -108,6 +115,24 @@ def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
         if prec != "fp64":
             res[prec]["exchange_speedup_vs_fp64"] = round(tk["fp64"] / tk[prec], 3)
     return res
+
+
+def profiled_traffic(kernel_prefix, n, M, o, precision):
+    """HBM bytes per dispatch of `kernel_prefix` from the committed rocprofv3
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"jk_build_n{n}_m{M}_o{o}_{precision}.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    try:
+        blob = text.split("```json", 1)[1].split("```", 1)[0]
+        data = json.loads(blob)
+    except (IndexError, ValueError):
+        return None
+    for k, v in data.items():
+        if k.startswith(kernel_prefix):
+            return v["fetch_bytes"] + v["write_bytes"]
+    return None
 
 
 def main():
@@ -188,10 +213,14 @@ def main():
     k_ms = phases["half_transform"][0] / args.steps + phases["exchange"][0] / args.steps
     k_tf = (flops_ht + flops_syrk) / (k_ms * 1e-3) / 1e12
     # dominant kernel: the larger of the two exchange kernels by time
+    ptag = {"fp64": "F64", "fp32": "F32", "bf16x3": "BF16"}[args.precision]
     if ht_ms * ht_calls >= sy_ms * sy_calls:
-        dom = ("k_ht_f64 (half-transform X = C^T L)", ht_tf)
+        dom = (f"k_ht<{ptag}> (half-transform X = C^T L)", ht_tf, f"k_ht<{ptag},")
     else:
-        dom = ("k_syrk_f64 (K = sum X^T X)", sy_tf)
+        dom = (f"k_syrk<{ptag}> (K = sum X^T X)", sy_tf, f"k_syrk<{ptag},")
+    peaks = {name: round(ctx.measure_peak(kind), 2) for name, kind in
+             (("mfma_f64_TFs", 0), ("mfma_f32_TFs", 1), ("mfma_bf16_TFs", 2), ("fma_f64_TFs", 3),
+              ("copy_TBs", 4))}
 
     out = {
         "metric": "J+K builds/sec (HBM GB/s on J, MFMA utilisation on K)",
@@ -218,7 +247,8 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(dom[1] / peak, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(dom[2], n, M, o, args.precision),
+            "peak_measured": peaks,
         },
         "kernels": {
             "coulomb_J_GBs": round(j_gbs, 1),
@@ -233,6 +263,14 @@ def main():
             "phase_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in phases.items()},
         },
     }
+    if args.copy_probe:
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.precision == "fp64" and not args.no_mixed:
         out["mixed"] = mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_step, args)
     if not args.no_cpu_baseline:

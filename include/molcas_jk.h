@@ -192,6 +192,11 @@ int mjk_sync(mjk_context *ctx);
 /* Turns per-phase HIP-event timing on (1) or off (0); resets the counters. */
 int mjk_set_profiling(mjk_context *ctx, int32_t on);
 
+/* Measured peak of the current device (no reference counterpart):
+ * kind 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16 MFMA, 3 FP64 VALU FMA (TFLOP/s),
+ * 4 device-to-device copy (TB/s, bytes read + written). */
+int mjk_measure_peak(mjk_context *ctx, int32_t kind, double *value);
+
 /* Copies accumulated milliseconds and launch counts per phase (arrays of
  * MJK_NUM_PHASES entries; either may be NULL). */
 int mjk_phase_times(mjk_context *ctx, double *ms, int64_t *calls);

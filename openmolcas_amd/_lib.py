@@ -53,6 +53,7 @@ SIGNATURES = {
     "mjk_sync": (_c.c_int, [_vp]),
     "mjk_set_profiling": (_c.c_int, [_vp, _i32]),
     "mjk_phase_times": (_c.c_int, [_vp, _vp, _vp]),
+    "mjk_measure_peak": (_c.c_int, [_vp, _i32, _vp]),
 }
 
 _lib = None

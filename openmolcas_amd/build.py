@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "csrc", "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmolcas_jk.so")
 
-SOURCES = ["capi.hip", "coulomb.hip", "exchange.hip", "assembly.hip", "puvx.hip", "backward.hip"]
+SOURCES = ["capi.hip", "coulomb.hip", "exchange.hip", "assembly.hip", "puvx.hip", "backward.hip", "peaks.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -38,6 +38,37 @@ def _compile(src: str) -> str:
     return obj
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_BIN = os.path.join(FORTRAN, "bin", "fscf_driver")
+
+
+def build_fortran(verbose: bool = False):
+    """ISO_C_BINDING interface module + the Fortran driver of the CHO_FSCF
+    drop-in (amdflang).  Returns the driver path, or None without a compiler."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("amdflang not found: Fortran interface not built")
+        return None
+    obj = os.path.join(FORTRAN, "_build")
+    os.makedirs(obj, exist_ok=True)
+    os.makedirs(os.path.dirname(FORTRAN_BIN), exist_ok=True)
+    iface_o = os.path.join(obj, "molcas_jk_iface.o")
+    cmds = [
+        [FLANG, "-O2", "-J", obj, "-c", os.path.join(FORTRAN, "molcas_jk_iface.F90"), "-o", iface_o],
+        [FLANG, "-O2", "-J", obj, os.path.join(FORTRAN, "fscf_driver.F90"), iface_o, "-L" + LIBDIR,
+         "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_BIN],
+    ]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("amdflang failed")
+    if verbose:
+        print("built", FORTRAN_BIN)
+    return FORTRAN_BIN
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -52,6 +83,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -265,6 +265,12 @@ class JKContext:
     def set_profiling(self, on: bool):
         self._check(self.lib.mjk_set_profiling(self._h, int(bool(on))))
 
+    def measure_peak(self, kind: int) -> float:
+        """Measured device peak: 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16 MFMA, 3 FP64 FMA (TFLOP/s), 4 copy (TB/s)."""
+        v = ctypes.c_double()
+        self._check(self.lib.mjk_measure_peak(self._h, int(kind), ctypes.byref(v)))
+        return v.value
+
     def phase_times(self):
         ms = (ctypes.c_double * len(_lib.PHASES))()
         calls = (ctypes.c_int64 * len(_lib.PHASES))()

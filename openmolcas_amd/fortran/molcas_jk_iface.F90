@@ -1,0 +1,118 @@
+!-----------------------------------------------------------------------
+! ISO_C_BINDING interface to libmolcas_jk (include/molcas_jk.h) and
+! Fortran drop-ins with the argument lists of OpenMolcas's
+!   CHO_FSCF   (src/fock_util/cho_fscf.F90:14)
+!   CHO_FMCSCF (src/fock_util/cho_fmcscf.F90:14)
+! for nSym = 1.  The reference passes DSBA_Type arrays; a caller passes
+! their %A0 storage (LT packed per irrep, orbital matrices A2(nOcc,nBas)).
+! rc keeps the reference's meaning (0 ok, 33 memory, 77 short read); the
+! GPU context plays the role of the vector buffer of Cho_X_Init.
+!-----------------------------------------------------------------------
+module molcas_jk_iface
+  use, intrinsic :: iso_c_binding, only: c_ptr, c_int, c_int32_t, c_int64_t, c_double, c_null_ptr, &
+                                         c_char, c_loc
+  implicit none
+  private
+  public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_cho_fscf, &
+            mjk_cho_fmcscf, cho_fscf_gpu, cho_fmcscf_gpu, jk_gpu_init, jk_gpu_final
+
+  integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1
+  type(c_ptr), save, public :: jk_ctx = c_null_ptr
+
+  interface
+    integer(c_int) function mjk_create(ctx, nbas, nchol) bind(C, name='mjk_create')
+      import :: c_ptr, c_int, c_int32_t, c_int64_t
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int32_t), value :: nbas
+      integer(c_int64_t), value :: nchol
+    end function mjk_create
+    integer(c_int) function mjk_destroy(ctx) bind(C, name='mjk_destroy')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function mjk_destroy
+    integer(c_int) function mjk_set_reduced_set(ctx, jred, nrs, rs_to_lt) bind(C, name='mjk_set_reduced_set')
+      import :: c_ptr, c_int, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: jred
+      integer(c_int64_t), value :: nrs
+      integer(c_int64_t), intent(in) :: rs_to_lt(*)
+    end function mjk_set_reduced_set
+    integer(c_int) function mjk_upload_vectors(ctx, jred, jvec, jnum, lrs, on_device) &
+        bind(C, name='mjk_upload_vectors')
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: jred, on_device
+      integer(c_int64_t), value :: jvec, jnum
+      real(c_double), intent(in) :: lrs(*)
+    end function mjk_upload_vectors
+    integer(c_int) function mjk_cho_fscf(ctx, nden, flt, nforb, niorb, porb, dlt, exfac, prec) &
+        bind(C, name='mjk_cho_fscf')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nden, prec
+      real(c_double), intent(inout) :: flt(*)
+      integer(c_int32_t), intent(in) :: nforb(*), niorb(*)
+      real(c_double), intent(in) :: porb(*), dlt(*)
+      real(c_double), value :: exfac
+    end function mjk_cho_fscf
+    integer(c_int) function mjk_cho_fmcscf(ctx, flt, nforb, niorb, naorb, factxi, dlt, do_active, &
+        porb1, nchm, porb2, porb3, w_pwxy, cmo, nfro, norb, exfac, prec) bind(C, name='mjk_cho_fmcscf')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: flt(*)
+      integer(c_int32_t), value :: nforb, niorb, naorb, do_active, nchm, nfro, norb, prec
+      real(c_double), value :: factxi, exfac
+      real(c_double), intent(in) :: dlt(*), porb1(*), porb2(*), porb3(*), cmo(*)
+      real(c_double), intent(inout) :: w_pwxy(*)
+    end function mjk_cho_fmcscf
+  end interface
+
+contains
+
+  ! Cho_X_Init counterpart: one context for nBas functions and NumCho vectors.
+  subroutine jk_gpu_init(rc, nBas, NumCho)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nBas, NumCho
+    rc = int(mjk_create(jk_ctx, int(nBas, c_int32_t), int(NumCho, c_int64_t)))
+  end subroutine jk_gpu_init
+
+  ! Cho_X_Final counterpart.
+  subroutine jk_gpu_final(rc)
+    integer, intent(out) :: rc
+    rc = int(mjk_destroy(jk_ctx))
+    jk_ctx = c_null_ptr
+  end subroutine jk_gpu_final
+
+  ! CHO_FSCF(rc,nDen,FLT,nForb,nIorb,Porb,DLT,ExFac), nSym = 1:
+  ! FLT(:,d) += J[DLT(:,1)] - ExFac * sum_Jk X X^T with Porb(d) = A2(nOcc,nBas).
+  subroutine cho_fscf_gpu(rc, nDen, FLT, nForb, nIorb, Porb, DLT, ExFac)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nDen, nForb(nDen), nIorb(nDen)
+    real(8), intent(inout) :: FLT(*)
+    real(8), intent(in) :: Porb(*), DLT(*), ExFac
+    integer(c_int32_t) :: nf(2), ni(2)
+    nf = 0
+    ni = 0
+    nf(1:min(nDen, 2)) = int(nForb(1:min(nDen, 2)), c_int32_t)
+    ni(1:min(nDen, 2)) = int(nIorb(1:min(nDen, 2)), c_int32_t)
+    rc = int(mjk_cho_fscf(jk_ctx, int(nDen, c_int32_t), FLT, nf, ni, Porb, DLT, real(ExFac, c_double), &
+                          MJK_PREC_FP64))
+  end subroutine cho_fscf_gpu
+
+  ! CHO_FMCSCF(rc,FLT,nForb,nIorb,nAorb,FactXI,DLT,DoActive,POrb,nChM,W_PWXY,CMO,ExFac), nSym = 1.
+  subroutine cho_fmcscf_gpu(rc, FLT, nForb, nIorb, nAorb, FactXI, DLT, DoActive, POrb1, nChM, POrb2, &
+                            POrb3, W_PWXY, CMO, nFro, nOrb, ExFac)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nForb, nIorb, nAorb, nChM, nFro, nOrb
+    logical, intent(in) :: DoActive
+    real(8), intent(inout) :: FLT(*), W_PWXY(*)
+    real(8), intent(in) :: FactXI, DLT(*), POrb1(*), POrb2(*), POrb3(*), CMO(*), ExFac
+    integer(c_int32_t) :: act
+    act = 0
+    if (DoActive) act = 1
+    rc = int(mjk_cho_fmcscf(jk_ctx, FLT, int(nForb, c_int32_t), int(nIorb, c_int32_t), int(nAorb, c_int32_t), &
+                            real(FactXI, c_double), DLT, act, POrb1, int(nChM, c_int32_t), POrb2, POrb3, W_PWXY, &
+                            CMO, int(nFro, c_int32_t), int(nOrb, c_int32_t), real(ExFac, c_double), MJK_PREC_FP64))
+  end subroutine cho_fmcscf_gpu
+
+end module molcas_jk_iface
