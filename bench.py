@@ -96,7 +96,7 @@ def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
     K64 = ctx.exchange([C], precision="fp64")
     e64 = energy_of(ctx, dlt, h, J, K64)
     tk = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16x3"):
         try:
             ctx.exchange([C], precision=prec)
         except Exception as exc:  # mode not built

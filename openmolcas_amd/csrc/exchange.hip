@@ -22,6 +22,8 @@
 //    vectors; it contracts over (J, k in the density's bank columns) with
 //    v_mfma_f64_16x16x4f64 and writes an FP64 partial tile.  A final
 //    deterministic reduction sums the slices into LT storage.
+#include <type_traits>
+
 #include "mjk_internal.h"
 
 namespace mjk {
@@ -314,6 +316,271 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
       }
 }
 
+// ---- split-BF16, three products ---------------------------------------------
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi); a product is taken as
+// lo*hi + hi*lo + hi*hi (lo*lo dropped) with v_mfma_f32_16x16x32_bf16 and FP32
+// accumulation.  That MFMA takes 8 consecutive contraction indices per lane
+// (lane l: row/column l%16, indices 8*(l/16)..8*(l/16)+7), so both operands are
+// staged contraction-contiguous in LDS: rows of kB3S bf16 (32 used, padded to
+// keep 16-byte alignment).  Result layout: row 4*(l/16) + r, column l%16.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct B3 {
+  typedef float T;
+  static constexpr bool kFlush = true;
+};
+
+constexpr int kB3K = 32;          // contraction indices per MFMA
+constexpr int kB3S = kB3K + 8;    // LDS row stride in bf16 elements
+
+__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = __bf16(x);
+  lo = __bf16(x - float(hi));
+}
+
+__device__ inline v4f mma_b3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, v4f acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+// Transposed split bank: ch/cl[(off + k)*n_pad + a] = hi/lo(corb[k + a*nocc]).
+__global__ void k_pack_orb_b3(const double *__restrict__ corb, int nocc, int n, int n_pad, int off,
+                              __bf16 *__restrict__ ch, __bf16 *__restrict__ cl) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(nocc) * n) return;
+  const int64_t a = i / nocc, k = i % nocc;
+  __bf16 h, l;
+  split_bf16(float(corb[i]), h, l);
+  ch[(off + k) * int64_t(n_pad) + a] = h;
+  cl[(off + k) * int64_t(n_pad) + a] = l;
+}
+
+// Half-transform, same work decomposition as k_ht: X[(jr*LDC + k)*n_pad + a].
+template <int OT, int TA>
+__global__ __launch_bounds__(kThreads) void k_ht_b3(
+    const float *__restrict__ L, int64_t ldl, int n, int n_pad, const __bf16 *__restrict__ CbH,
+    const __bf16 *__restrict__ CbL, int64_t j0, int ntile_a, int64_t nblocks,
+    float *__restrict__ X) {
+  constexpr int LDC = 16 * OT;
+  constexpr int NB = TA / 64;
+  constexpr int LN = kB3K * TA / kThreads;
+  constexpr int CCH = LDC * kB3K / 8;  // 16-byte pieces of one bank array per chunk
+  constexpr int CN = (CCH + kThreads - 1) / kThreads;
+  static_assert(kB3K * TA % kThreads == 0, "tile shape");
+  __shared__ __attribute__((aligned(16))) __bf16 sCh[LDC * kB3S];
+  __shared__ __attribute__((aligned(16))) __bf16 sCl[LDC * kB3S];
+  __shared__ __attribute__((aligned(16))) __bf16 sLh[TA * kB3S];
+  __shared__ __attribute__((aligned(16))) __bf16 sLl[TA * kB3S];
+  const int64_t lb = xcd_logical(nblocks);
+  if (lb < 0) return;
+  const int64_t jr = lb / ntile_a;
+  const int a0 = int(lb % ntile_a) * TA;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const float *__restrict__ Lj = L + (j0 + jr) * ldl;
+
+  v4f acc[OT][NB];
+#pragma unroll
+  for (int t = 0; t < OT; ++t)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[t][j] = v4f{0, 0, 0, 0};
+
+  if (a0 < n) {
+    float rl[LN];
+    int lo[LN];
+    uint4 rch[CN], rcl[CN];
+    auto fetch = [&](int b0) {
+#pragma unroll
+      for (int i = 0; i < CN; ++i) {
+        const int p = tid + kThreads * i;
+        if (p < CCH) {
+          const int64_t src = int64_t(p / 4) * n_pad + b0 + (p % 4) * 8;
+          rch[i] = *reinterpret_cast<const uint4 *>(CbH + src);
+          rcl[i] = *reinterpret_cast<const uint4 *>(CbL + src);
+        }
+      }
+      if (b0 >= a0 + TA) {  // every b > a
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
+          rl[i] = (b < n && a < n) ? Lj[tri0(b) + a] : 0.f;
+          lo[i] = aa * kB3S + bb;
+        }
+      } else if (b0 + kB3K <= a0) {  // every b < a
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+          const int e = tid + kThreads * i, bb = e % kB3K, aa = e / kB3K, b = b0 + bb, a = a0 + aa;
+          rl[i] = (a < n) ? Lj[tri0(a) + b] : 0.f;
+          lo[i] = aa * kB3S + bb;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
+          float v = 0.f;
+          if (b < n && a < n) v = (b >= a) ? Lj[tri0(b) + a] : Lj[tri0(a) + b];
+          rl[i] = v;
+          lo[i] = aa * kB3S + bb;
+        }
+      }
+    };
+    fetch(0);
+    for (int b0 = 0; b0 < n; b0 += kB3K) {
+      if (b0) __syncthreads();
+#pragma unroll
+      for (int i = 0; i < CN; ++i) {
+        const int p = tid + kThreads * i;
+        if (p < CCH) {
+          const int dst = (p / 4) * kB3S + (p % 4) * 8;
+          *reinterpret_cast<uint4 *>(sCh + dst) = rch[i];
+          *reinterpret_cast<uint4 *>(sCl + dst) = rcl[i];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < LN; ++i) split_bf16(rl[i], sLh[lo[i]], sLl[lo[i]]);
+      __syncthreads();
+      if (b0 + kB3K < n) fetch(b0 + kB3K);
+      const int ko = 8 * (lane >> 4);
+      bf16x8 bh[NB], bl[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int r = w * (TA / 4) + 16 * j + (lane & 15);
+        bh[j] = *reinterpret_cast<const bf16x8 *>(sLh + r * kB3S + ko);
+        bl[j] = *reinterpret_cast<const bf16x8 *>(sLl + r * kB3S + ko);
+      }
+#pragma unroll
+      for (int t = 0; t < OT; ++t) {
+        const int r = 16 * t + (lane & 15);
+        const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(sCh + r * kB3S + ko);
+        const bf16x8 al = *reinterpret_cast<const bf16x8 *>(sCl + r * kB3S + ko);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[t][j] = mma_b3(ah, al, bh[j], bl[j], acc[t][j]);
+      }
+    }
+  }
+  float *__restrict__ xj = X + jr * LDC * int64_t(n_pad);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int a = a0 + w * (TA / 4) + j * 16 + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < OT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xj[int64_t(16 * t + 4 * (lane >> 4) + r) * n_pad + a] = acc[t][j][r];
+  }
+}
+
+// SYRK on FP32 X with split-BF16 products; same tiles, slices and FP64 flush
+// as k_syrk.
+template <int TM>
+__global__ __launch_bounds__(kThreads) void k_syrk_b3(
+    const float *__restrict__ X, int n_pad, int ldc, int off, int opad, int64_t jc, int64_t jper,
+    int ntiles, double *__restrict__ kpart, int beta) {
+  constexpr int WT = TM / 2;
+  constexpr int MT = WT / 16;
+  constexpr int PN = kB3K * TM / kThreads;
+  __shared__ __attribute__((aligned(16))) __bf16 sAh[TM * kB3S];
+  __shared__ __attribute__((aligned(16))) __bf16 sAl[TM * kB3S];
+  __shared__ __attribute__((aligned(16))) __bf16 sBh[TM * kB3S];
+  __shared__ __attribute__((aligned(16))) __bf16 sBl[TM * kB3S];
+  const int tile = blockIdx.x % ntiles;
+  const int s = blockIdx.x / ntiles;
+  int A = int((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+  while (A * (A + 1) / 2 > tile) --A;
+  while ((A + 1) * (A + 2) / 2 <= tile) ++A;
+  const int B = tile - A * (A + 1) / 2;
+  const bool diag = (A == B);
+  const int64_t ja = int64_t(s) * jper;
+  const int64_t jz = min(jc, ja + jper);
+  const int nf = int((jz > ja ? jz - ja : 0) * opad);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const bool idle = diag && wm < wn;
+
+  v4f acc[MT][MT];
+  double accd[MT][MT][4];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+      acc[i][j] = v4f{0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) accd[i][j][r] = 0.0;
+    }
+  float ra[PN], rb[PN];
+  auto fetch = [&](int f0) {
+#pragma unroll
+    for (int i = 0; i < PN; ++i) {
+      const int e = tid + kThreads * i, ff = e / TM, cc = e % TM, f = f0 + ff;
+      float va = 0.f, vb = 0.f;
+      if (f < nf) {
+        const int64_t ro = ((ja + f / opad) * ldc + off + f % opad) * int64_t(n_pad);
+        va = X[ro + A * TM + cc];
+        if (!diag) vb = X[ro + B * TM + cc];
+      }
+      ra[i] = va;
+      rb[i] = vb;
+    }
+  };
+  if (nf > 0) fetch(0);
+  for (int f0 = 0; f0 < nf; f0 += kB3K) {
+    if (f0) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PN; ++i) {
+      const int e = tid + kThreads * i, ff = e / TM, cc = e % TM;
+      split_bf16(ra[i], sAh[cc * kB3S + ff], sAl[cc * kB3S + ff]);
+      if (!diag) split_bf16(rb[i], sBh[cc * kB3S + ff], sBl[cc * kB3S + ff]);
+    }
+    __syncthreads();
+    if (f0 + kB3K < nf) fetch(f0 + kB3K);
+    if (!idle) {
+      const __bf16 *sbh = diag ? sAh : sBh;
+      const __bf16 *sbl = diag ? sAl : sBl;
+      const int ko = 8 * (lane >> 4);
+      bf16x8 ah[MT], al[MT], bh[MT], bl[MT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i) {
+        const int r = wm * WT + 16 * i + (lane & 15);
+        ah[i] = *reinterpret_cast<const bf16x8 *>(sAh + r * kB3S + ko);
+        al[i] = *reinterpret_cast<const bf16x8 *>(sAl + r * kB3S + ko);
+      }
+#pragma unroll
+      for (int j = 0; j < MT; ++j) {
+        const int r = wn * WT + 16 * j + (lane & 15);
+        bh[j] = *reinterpret_cast<const bf16x8 *>(sbh + r * kB3S + ko);
+        bl[j] = *reinterpret_cast<const bf16x8 *>(sbl + r * kB3S + ko);
+      }
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j)
+          if (!diag || wm > wn || i >= j) acc[i][j] = mma_b3(ah[i], al[i], bh[j], bl[j], acc[i][j]);
+    }
+    if ((f0 + kB3K) % kFlushRows == 0) {
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < MT; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) accd[i][j][r] += double(acc[i][j][r]);
+          acc[i][j] = v4f{0, 0, 0, 0};
+        }
+    }
+  }
+  double *__restrict__ kp = kpart + (int64_t(s) * ntiles + tile) * TM * TM;
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < MT; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int al_ = wm * WT + i * 16 + 4 * (lane >> 4) + r;
+        const int bl_ = wn * WT + j * 16 + (lane & 15);
+        const double v = accd[i][j][r] + double(acc[i][j][r]);
+        double *dst = kp + al_ * TM + bl_;
+        *dst = beta ? *dst + v : v;
+      }
+}
+
 // klt[a*(a+1)/2 + b] = sum_s kpart[(s*ntiles + tile(a,b))*TM*TM + (a%TM)*TM + b%TM]
 // grid.x = n (row a), block 256.
 __global__ void k_reduce_k(const double *__restrict__ kpart, int n, int tm, int ntiles,
@@ -368,6 +635,60 @@ void syrk_launch(Context *c, int tm, dim3 grid, const typename P::T *X, int ldc,
   else
     hipLaunchKernelGGL((k_syrk<P, 64, 32>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc,
                        off, opad, jn, jper, ntiles, kpart, beta);
+}
+
+template <int OT, int TA>
+void launch_ht_b3(Context *c, const float *L, const __bf16 *ch, const __bf16 *cl, int64_t j0,
+                  int64_t jc, float *X) {
+  const int ntile_a = c->n_pad / TA;
+  const int64_t nblocks = jc * ntile_a;
+  hipLaunchKernelGGL((k_ht_b3<OT, TA>), dim3(unsigned(round_up(nblocks, 8))), dim3(kThreads), 0,
+                     c->stream, L, c->ldl, c->n, c->n_pad, ch, cl, j0, ntile_a, nblocks, X);
+}
+
+void ht_dispatch_b3(Context *c, int ot, const float *L, const __bf16 *ch, const __bf16 *cl,
+                    int64_t j0, int64_t jn, float *X) {
+  switch (ot) {
+    case 1: launch_ht_b3<1, 128>(c, L, ch, cl, j0, jn, X); break;
+    case 2: launch_ht_b3<2, 128>(c, L, ch, cl, j0, jn, X); break;
+    case 3: launch_ht_b3<3, 128>(c, L, ch, cl, j0, jn, X); break;
+    case 4: launch_ht_b3<4, 128>(c, L, ch, cl, j0, jn, X); break;
+    case 6: launch_ht_b3<6, 128>(c, L, ch, cl, j0, jn, X); break;
+    case 8: launch_ht_b3<8, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 10: launch_ht_b3<10, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 12: launch_ht_b3<12, 64>(c, L, ch, cl, j0, jn, X); break;
+    default: launch_ht_b3<16, 64>(c, L, ch, cl, j0, jn, X); break;
+  }
+}
+
+void syrk_launch_b3(Context *c, int tm, dim3 grid, const float *X, int ldc, int off, int opad,
+                    int64_t jn, int64_t jper, int ntiles, double *kpart, int beta) {
+  if (tm == 128)
+    hipLaunchKernelGGL((k_syrk_b3<128>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc, off,
+                       opad, jn, jper, ntiles, kpart, beta);
+  else
+    hipLaunchKernelGGL((k_syrk_b3<64>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc, off,
+                       opad, jn, jper, ntiles, kpart, beta);
+}
+
+int pack_bank_b3(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                 const std::vector<int> &off, int ldc) {
+  const int n = c->n, n_pad = c->n_pad;
+  const size_t half = sizeof(__bf16) * size_t(ldc) * n_pad;
+  int rc;
+  if ((rc = ensure(c, c->ws_cbank, 2 * half))) return rc;
+  __bf16 *ch = static_cast<__bf16 *>(c->ws_cbank.ptr);
+  __bf16 *cl = ch + size_t(ldc) * n_pad;
+  phase_begin(c);
+  if ((rc = hip_check(c, hipMemsetAsync(ch, 0, 2 * half, c->stream), "memset bank"))) return rc;
+  for (int d = 0; d < nk; ++d) {
+    if (nocc[d] == 0) continue;
+    const int64_t cnt = int64_t(nocc[d]) * n;
+    hipLaunchKernelGGL(k_pack_orb_b3, dim3(unsigned(ceil_div(cnt, 256))), dim3(256), 0, c->stream,
+                       corb[d], nocc[d], n, n_pad, off[size_t(d)], ch, cl);
+  }
+  phase_end(c, MJK_PHASE_PACK);
+  return MJK_OK;
 }
 
 // Layout of the bank for nk sets; returns MJK_OK and ot = 0 when empty.
@@ -447,8 +768,15 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
     return MJK_OK;
   }
   const int ldc = 16 * ot;
-  if ((rc = pack_bank_t<T>(c, nk, nocc, corb, off, ldc))) return rc;
+  constexpr bool kB3 = std::is_same<P, B3>::value;
+  if constexpr (kB3) {
+    if ((rc = pack_bank_b3(c, nk, nocc, corb, off, ldc))) return rc;
+  } else {
+    if ((rc = pack_bank_t<T>(c, nk, nocc, corb, off, ldc))) return rc;
+  }
   const T *cb = static_cast<const T *>(c->ws_cbank.ptr);
+  const __bf16 *cbh = static_cast<const __bf16 *>(c->ws_cbank.ptr);
+  const __bf16 *cbl = cbh + size_t(ldc) * n_pad;
   const T *L;
   if constexpr (sizeof(T) == sizeof(double)) {
     L = c->L;
@@ -489,14 +817,22 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
     const int64_t j0 = ch * jc;
     const int64_t jn = std::min(jc, M - j0);
     phase_begin(c);
-    ht_dispatch<P>(c, ot, L, cb, j0, jn, X);
+    if constexpr (kB3)
+      ht_dispatch_b3(c, ot, L, cbh, cbl, j0, jn, X);
+    else
+      ht_dispatch<P>(c, ot, L, cb, j0, jn, X);
     phase_end(c, MJK_PHASE_HALF_TRANSFORM);
     phase_begin(c);
     for (int d = 0; d < nk; ++d) {
       if (!nsplit[size_t(d)]) continue;
       const int64_t jper = ceil_div(jn, nsplit[size_t(d)]);
-      syrk_launch<P>(c, tm, dim3(unsigned(ntiles * nsplit[size_t(d)])), X, ldc, off[size_t(d)],
-                     opad[size_t(d)], jn, jper, ntiles, kpart + kbase[size_t(d)], ch > 0 ? 1 : 0);
+      const dim3 grid(unsigned(ntiles * nsplit[size_t(d)]));
+      if constexpr (kB3)
+        syrk_launch_b3(c, tm, grid, X, ldc, off[size_t(d)], opad[size_t(d)], jn, jper, ntiles,
+                       kpart + kbase[size_t(d)], ch > 0 ? 1 : 0);
+      else
+        syrk_launch<P>(c, tm, grid, X, ldc, off[size_t(d)], opad[size_t(d)], jn, jper, ntiles,
+                       kpart + kbase[size_t(d)], ch > 0 ? 1 : 0);
     }
     phase_end(c, MJK_PHASE_EXCHANGE);
   }
@@ -537,6 +873,7 @@ int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *c
   switch (precision) {
     case MJK_PREC_FP64: return exchange_t<F64>(c, nk, nocc, corb, klt, ldk);
     case MJK_PREC_FP32: return exchange_t<F32>(c, nk, nocc, corb, klt, ldk);
+    case MJK_PREC_BF16X3: return exchange_t<B3>(c, nk, nocc, corb, klt, ldk);
     default: return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
   }
 }

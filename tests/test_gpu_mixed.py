@@ -13,6 +13,9 @@ from openmolcas_amd.synthetic import chol_vectors_np, orbitals_np
 pytestmark = pytest.mark.gpu
 
 FP32_RTOL = 2e-6
+# split-BF16: inputs carried to 16 significant bits (hi + lo), lo*lo dropped,
+# FP32 accumulation: relative max-abs error 1e-4 of max|K|
+B3_RTOL = 1e-4
 
 
 def oracle_K(L, C):
@@ -48,3 +51,16 @@ def test_fp32_shadow_refreshed_after_upload(gpu):
     ctx.upload_packed(2.0 * L)  # the FP32 copy of L must follow the new vectors
     k2 = ctx.exchange([torch.from_numpy(C)], precision="fp32").cpu().numpy()
     assert np.allclose(k2, 4.0 * k1, rtol=1e-6, atol=1e-9 * np.max(np.abs(k1)))
+
+
+@pytest.mark.parametrize("n,M,occs", [(64, 33, [16, 1]), (200, 40, [64, 13]), (300, 64, [150])])
+def test_exchange_bf16x3_matches_fp64_oracle(gpu, n, M, occs):
+    L = chol_vectors_np(n, M, seed=5 * n + M)
+    Cs = [orbitals_np(n, o, seed=700 + i) for i, o in enumerate(occs)]
+    ctx = JKContext(n, M, device=gpu)
+    ctx.upload_packed(L)
+    K3 = ctx.exchange([torch.from_numpy(c) for c in Cs], precision="bf16x3").cpu().numpy()
+    for d, C in enumerate(Cs):
+        ref = oracle_K(L, C)
+        err = np.max(np.abs(K3[d] - ref)) / np.max(np.abs(ref))
+        assert err < B3_RTOL, err
