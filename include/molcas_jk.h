@@ -70,6 +70,28 @@ int mjk_create(mjk_context **ctx, int32_t nbas, int64_t nchol);
  * (called at src/scf/wfctl_scf.F90:1202). */
 int mjk_destroy(mjk_context *ctx);
 
+/* As mjk_create, but the new context shares the device workspaces and the
+ * stream of `share` (many small systems, one set of scratch buffers; calls on
+ * contexts that share must not overlap).  Library-internal design: no
+ * reference counterpart. */
+int mjk_create_shared(mjk_context **ctx, int32_t nbas, int64_t nchol, mjk_context *share);
+
+/* Device pointer and row stride (doubles) of the resident vector store, for
+ * callers that write the vectors in place (zero copy): row J holds the packed
+ * LT of vector J in its first nbas(nbas+1)/2 entries; the rest of the row
+ * must stay zero.  Marks every vector as present.  Counterpart of the vector
+ * buffer that Cho_X_Init fills (src/cholesky_util/cho_x_init.F90:16-52). */
+int mjk_device_vectors(mjk_context *ctx, double **L, int64_t *ldl);
+
+/* Exchange tiling: SYRK tile (0 automatic, 64, 128), SYRK workgroups per
+ * launch to aim for (0 automatic), half-transform AO tile (0, 64, 128). */
+int mjk_set_tuning(mjk_context *ctx, int32_t tile_syrk, int32_t split_target, int32_t ht_ta);
+
+/* Times every exchange tiling for these orbital sets and keeps the fastest
+ * in the context (best time in *best_ms); klt receives the last K. */
+int mjk_autotune(mjk_context *ctx, int32_t nk, const int32_t *nocc, const double *const *corb,
+                 double *klt, int64_t ldk, int32_t precision, double *best_ms);
+
 /* Text of the last error on this context ("" if none). */
 const char *mjk_last_error(const mjk_context *ctx);
 

@@ -35,6 +35,10 @@ SIGNATURES = {
     "mjk_create": (_c.c_int, [_c.POINTER(_vp), _i32, _i64]),
     "mjk_destroy": (_c.c_int, [_vp]),
     "mjk_last_error": (_c.c_char_p, [_vp]),
+    "mjk_create_shared": (_c.c_int, [_c.POINTER(_vp), _i32, _i64, _vp]),
+    "mjk_device_vectors": (_c.c_int, [_vp, _c.POINTER(_vp), _c.POINTER(_i64)]),
+    "mjk_set_tuning": (_c.c_int, [_vp, _i32, _i32, _i32]),
+    "mjk_autotune": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32, _vp]),
     "mjk_set_stream": (_c.c_int, [_vp, _vp]),
     "mjk_set_x_budget": (_c.c_int, [_vp, _i64]),
     "mjk_set_reduced_set": (_c.c_int, [_vp, _i32, _i64, _vp]),

@@ -84,8 +84,8 @@ int fock_update(Context *c, int32_t nd, double *flt, int64_t ldf, const double *
 int energy(Context *c, const double *h, const double *dlt, const double *g, double *e_out_dev) {
   int rc;
   if (!dlt || !e_out_dev) return set_error(c, MJK_ERR_ARG, "energy: null pointer");
-  if ((rc = ensure(c, c->ws_red, sizeof(double) * kRedBlocks))) return rc;
-  double *part = static_cast<double *>(c->ws_red.ptr);
+  if ((rc = ensure(c, c->w->red, sizeof(double) * kRedBlocks))) return rc;
+  double *part = static_cast<double *>(c->w->red.ptr);
   phase_begin(c);
   hipLaunchKernelGGL(k_energy_part, dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream, h, dlt, g,
                      c->np, part);

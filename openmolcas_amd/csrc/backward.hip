@@ -160,7 +160,7 @@ extern "C" int mjk_density(mjk_context *c, int32_t nocc, const double *corb, dou
   if (!c) return MJK_ERR_ARG;
   if (nocc < 0 || (nocc > 0 && !corb) || !dlt) return set_error(c, MJK_ERR_ARG, "density: bad arguments");
   if (nocc == 0)
-    return hip_check(c, hipMemsetAsync(dlt, 0, sizeof(double) * size_t(c->np), c->stream), "density");
+    return fill_zero(c, dlt, sizeof(double) * size_t(c->np));
   phase_begin(c);
   hipLaunchKernelGGL(k_density_lt, dim3(unsigned(c->n)), dim3(256), 0, c->stream, corb, nocc, c->n,
                      scale, 1, dlt);
@@ -174,8 +174,8 @@ extern "C" int mjk_grad_orbitals(mjk_context *c, int32_t nocc, const double *cor
   if (nocc < 1 || !corb || !flt || !dc) return set_error(c, MJK_ERR_ARG, "grad_orbitals: bad arguments");
   const int n = c->n;
   int rc;
-  if ((rc = ensure(c, c->ws_misc, sizeof(double) * size_t(n) * n))) return rc;
-  double *fd = static_cast<double *>(c->ws_misc.ptr);
+  if ((rc = ensure(c, c->w->misc, sizeof(double) * size_t(n) * n))) return rc;
+  double *fd = static_cast<double *>(c->w->misc.ptr);
   phase_begin(c);
   hipLaunchKernelGGL(k_unpack_lt, dim3(unsigned(n)), dim3(256), 0, c->stream, flt, n, fd, int64_t(n));
   GemmBatch g{};
@@ -205,21 +205,21 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
   const double *sets[1] = {corb};
   if ((rc = pack_bank(c, 1, &nocc, sets, off, opad, &ot, &tot))) return rc;
   const int ldc = 16 * ot;
-  const double *cb = static_cast<const double *>(c->ws_cbank.ptr);
+  const double *cb = static_cast<const double *>(c->w->cbank.ptr);
   // per vector: X (ldc x n_pad) + Z (ldc x ldc) + W (n_pad x ldc)
   const int64_t per_vec = int64_t(ldc) * n_pad * 2 + int64_t(ldc) * ldc;
   int64_t jc = std::max<int64_t>(1, std::min<int64_t>(M, c->x_budget / (8 * per_vec)));
   const int64_t nchunk = ceil_div(M, jc);
   jc = ceil_div(M, nchunk);
-  if ((rc = ensure(c, c->ws_x, sizeof(double) * size_t(jc * per_vec)))) return rc;
-  double *X = static_cast<double *>(c->ws_x.ptr);
+  if ((rc = ensure(c, c->w->x, sizeof(double) * size_t(jc * per_vec)))) return rc;
+  double *X = static_cast<double *>(c->w->x.ptr);
   double *Z = X + jc * ldc * int64_t(n_pad);
   double *W = Z + jc * ldc * int64_t(ldc);
   // P: caller's plain LT, or 2 C C^T
   const double *P = p_lt;
   if (!P) {
-    if ((rc = ensure(c, c->ws_misc, sizeof(double) * size_t(np)))) return rc;
-    double *pw = static_cast<double *>(c->ws_misc.ptr);
+    if ((rc = ensure(c, c->w->misc, sizeof(double) * size_t(np)))) return rc;
+    double *pw = static_cast<double *>(c->w->misc.ptr);
     hipLaunchKernelGGL(k_density_lt, dim3(unsigned(n)), dim3(256), 0, c->stream, corb, nocc, n,
                        2.0, 0, pw);
     P = pw;

@@ -41,6 +41,37 @@ int ensure(Context *c, DevBuf &b, size_t bytes) {
   return MJK_OK;
 }
 
+namespace {
+// Zero fill in 16-byte words plus a 4-byte tail (all buffers are 4-byte multiples).
+__global__ void k_fill_zero(uint4 *__restrict__ p16, int64_t n16, uint32_t *__restrict__ p4,
+                            int64_t n4) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride)
+    p16[i] = make_uint4(0, 0, 0, 0);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += stride) p4[i] = 0u;
+}
+}  // namespace
+
+int fill_zero(Context *c, void *ptr, size_t bytes) {
+  if (bytes == 0) return MJK_OK;
+  if ((reinterpret_cast<uintptr_t>(ptr) & 3) != 0 || (bytes & 3) != 0)
+    return set_error(c, MJK_ERR_ARG, "fill_zero: buffer not 4-byte aligned");
+  // 4-byte head up to 16-byte alignment, 16-byte body, 4-byte tail
+  char *p = static_cast<char *>(ptr);
+  const size_t head = std::min(bytes, (16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15);
+  const size_t body = (bytes - head) / 16 * 16;
+  const size_t tail = bytes - head - body;
+  const int64_t n16 = int64_t(body / 16);
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(4096, ceil_div(n16, 256)));
+  if (head)
+    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, c->stream, static_cast<uint4 *>(nullptr),
+                       int64_t(0), reinterpret_cast<uint32_t *>(p), int64_t(head / 4));
+  hipLaunchKernelGGL(k_fill_zero, dim3(unsigned(blocks)), dim3(256), 0, c->stream,
+                     reinterpret_cast<uint4 *>(p + head), n16,
+                     reinterpret_cast<uint32_t *>(p + head + body), int64_t(tail / 4));
+  return hip_check(c, hipGetLastError(), "fill_zero");
+}
+
 static hipEvent_t take_event(Context *c, int *idx) {
   if (c->ev_used == c->ev_pool.size()) {
     hipEvent_t e;
@@ -128,6 +159,8 @@ int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
   c->n_pad = int32_t(round_up(nbas, 128));
   c->nchol = nchol;
   c->loaded.assign(size_t(nchol), 0);
+  c->wsp = std::make_shared<Workspace>();
+  c->w = c->wsp.get();
   int rc;
   if ((rc = hip_check(c, hipGetDevice(&c->device), "hipGetDevice")) ||
       (rc = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking),
@@ -159,9 +192,7 @@ int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
 int mjk_destroy(mjk_context *c) {
   if (!c) return MJK_OK;
   hipStreamSynchronize(c->stream);
-  for (DevBuf *b : {&c->ws_l32, &c->ws_d, &c->ws_vpart, &c->ws_v, &c->ws_jpart, &c->ws_cbank, &c->ws_x,
-                    &c->ws_kpart, &c->ws_stage, &c->ws_j, &c->ws_k, &c->ws_red, &c->ws_misc})
-    free_buf(*b);
+  free_buf(c->ws_l32);  // the shared workspace goes with its last owner
   for (auto &r : c->rsets)
     if (r.d_map) hipFree(r.d_map);
   if (c->L) hipFree(c->L);
@@ -188,6 +219,41 @@ int mjk_set_stream(mjk_context *c, void *s) {
 int mjk_set_x_budget(mjk_context *c, int64_t bytes) {
   if (!c || bytes < 1) return MJK_ERR_ARG;
   c->x_budget = bytes;
+  return MJK_OK;
+}
+
+int mjk_create_shared(mjk_context **out, int32_t nbas, int64_t nchol, mjk_context *share) {
+  if (!out || !share) return MJK_ERR_ARG;
+  int rc = mjk_create(out, nbas, nchol);
+  if (rc) return rc;
+  mjk_context *c = *out;
+  (void)hipStreamDestroy(c->stream);
+  c->stream = share->stream;  // shared workspaces need one stream
+  c->own_stream = false;
+  c->wsp = share->wsp;
+  c->w = c->wsp.get();
+  c->x_budget = share->x_budget;
+  return MJK_OK;
+}
+
+int mjk_device_vectors(mjk_context *c, double **L, int64_t *ldl) {
+  if (!c || !L || !ldl) return MJK_ERR_ARG;
+  *L = c->L;
+  *ldl = c->ldl;
+  std::fill(c->loaded.begin(), c->loaded.end(), uint8_t(1));
+  c->nloaded = c->nchol;
+  c->l32_valid = false;
+  return MJK_OK;
+}
+
+int mjk_set_tuning(mjk_context *c, int32_t tile_syrk, int32_t split_target, int32_t ht_ta) {
+  if (!c) return MJK_ERR_ARG;
+  if ((tile_syrk != 0 && tile_syrk != 64 && tile_syrk != 128) || split_target < 0 ||
+      (ht_ta != 0 && ht_ta != 64 && ht_ta != 128))
+    return set_error(c, MJK_ERR_ARG, "set_tuning: tile 0/64/128, split >= 0, ht_ta 0/64/128");
+  c->tile_syrk = tile_syrk;
+  c->split_target = split_target;
+  c->ht_ta = ht_ta;
   return MJK_OK;
 }
 
@@ -259,13 +325,13 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
       for (int64_t j0 = 0; j0 < jnum; j0 += per) {
         const int64_t jn = std::min(per, jnum - j0);
         if (!on_device) {
-          if ((rc = ensure(c, c->ws_stage, sizeof(double) * nrs * jn))) return rc;
-          if ((rc = hip_check(c, hipMemcpyAsync(c->ws_stage.ptr, lrs + j0 * nrs,
+          if ((rc = ensure(c, c->w->stage, sizeof(double) * nrs * jn))) return rc;
+          if ((rc = hip_check(c, hipMemcpyAsync(c->w->stage.ptr, lrs + j0 * nrs,
                                                 sizeof(double) * nrs * jn, hipMemcpyHostToDevice,
                                                 c->stream),
                               "stage vectors")))
             return rc;
-          src = static_cast<const double *>(c->ws_stage.ptr);
+          src = static_cast<const double *>(c->w->stage.ptr);
         } else {
           src = lrs + j0 * nrs;
         }
@@ -368,8 +434,8 @@ int mjk_cho_fscf(mjk_context *c, int32_t nden, double *flt, const int32_t *nforb
   // device scratch: D | J | K(nden) | F(nden) | orbitals
   const size_t need = sizeof(double) * size_t(np * (2 + 2 * nden) + tot);
   int rc;
-  if ((rc = ensure(c, c->ws_misc, need))) return rc;
-  double *dD = static_cast<double *>(c->ws_misc.ptr);
+  if ((rc = ensure(c, c->w->misc, need))) return rc;
+  double *dD = static_cast<double *>(c->w->misc.ptr);
   double *dJ = dD + np, *dK = dJ + np, *dF = dK + nden * np, *dC = dF + nden * np;
   if ((rc = hip_check(c, hipMemcpyAsync(dD, dlt, sizeof(double) * np, hipMemcpyHostToDevice,
                                         c->stream), "copy DLT")) ||
@@ -417,8 +483,8 @@ int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, in
   const size_t need = sizeof(double) * size_t(np * 4 * nden + (nk1 + nk2) * n +
                                               (do_puvx ? naorb * n + n * n + nw : 0));
   int rc;
-  if ((rc = ensure(c, c->ws_misc, need))) return rc;
-  double *dD = static_cast<double *>(c->ws_misc.ptr);
+  if ((rc = ensure(c, c->w->misc, need))) return rc;
+  double *dD = static_cast<double *>(c->w->misc.ptr);
   double *dJ = dD + nden * np, *dK = dJ + nden * np, *dF = dK + nden * np, *dC1 = dF + nden * np;
   double *dC2 = dC1 + int64_t(nk1) * n, *dC3 = dC2 + int64_t(nk2) * n;
   double *dCMO = dC3 + (do_puvx ? int64_t(naorb) * n : 0), *dW = dCMO + (do_puvx ? n * n : 0);

@@ -175,8 +175,8 @@ int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
   int rc;
 
   // staging copy of the densities (zero padded to ldl)
-  if ((rc = ensure(c, c->ws_d, sizeof(double) * nd * ldl))) return rc;
-  double *Dp = static_cast<double *>(c->ws_d.ptr);
+  if ((rc = ensure(c, c->w->d, sizeof(double) * nd * ldl))) return rc;
+  double *Dp = static_cast<double *>(c->w->d.ptr);
   phase_begin(c);
   hipLaunchKernelGGL(k_pack_dens, dim3(unsigned(ceil_div(ldl, 256)), unsigned(nd)), dim3(256), 0,
                      c->stream, dlt, ldd, c->np, ldl, Dp);
@@ -190,10 +190,10 @@ int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
   int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nblk), 1), nseg));
   const int64_t pchunk = ceil_div(nseg, nsplit1) * kSeg;
   nsplit1 = int(ceil_div(ldl, pchunk));
-  if ((rc = ensure(c, c->ws_vpart, sizeof(double) * nsplit1 * M * nd))) return rc;
-  if ((rc = ensure(c, c->ws_v, sizeof(double) * M * nd))) return rc;
-  double *vpart = static_cast<double *>(c->ws_vpart.ptr);
-  double *V = v_out ? v_out : static_cast<double *>(c->ws_v.ptr);
+  if ((rc = ensure(c, c->w->vpart, sizeof(double) * nsplit1 * M * nd))) return rc;
+  if ((rc = ensure(c, c->w->v, sizeof(double) * M * nd))) return rc;
+  double *vpart = static_cast<double *>(c->w->vpart.ptr);
+  double *V = v_out ? v_out : static_cast<double *>(c->w->v.ptr);
   phase_begin(c);
   switch (ndt) {
     case 1: launch_v<8, 1>(c, Dp, nd, vpart, nsplit1, pchunk); break;
@@ -212,8 +212,8 @@ int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
   int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nseg), 1), M));
   const int64_t jchunk = ceil_div(M, nsplit2);
   nsplit2 = int(ceil_div(M, jchunk));
-  if ((rc = ensure(c, c->ws_jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
-  double *jpart = static_cast<double *>(c->ws_jpart.ptr);
+  if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
+  double *jpart = static_cast<double *>(c->w->jpart.ptr);
   phase_begin(c);
   switch (ndt) {
     case 1: launch_j<1>(c, V, nd, jpart, nsplit2, jchunk); break;

@@ -58,6 +58,26 @@ struct F32 {
 // contraction rows, so no FP32 sum runs over more than kFlushRows products.
 constexpr int kFlushRows = 512;
 
+// Densities of one SYRK launch: workgroup b belongs to the set d with
+// item0[d] <= b < item0[d+1]; inside it, tile = local % ntiles and slice
+// s = local / ntiles.  Set d contracts over bank columns off..off+opad-1 and
+// writes partial tiles at kpart + kbase[d].
+constexpr int kMaxSets = 16;
+struct SyrkSets {
+  int n;
+  int off[kMaxSets];
+  int opad[kMaxSets];
+  int nsplit[kMaxSets];
+  int item0[kMaxSets + 1];
+  int64_t kbase[kMaxSets];
+};
+
+__device__ inline int find_set(const SyrkSets &sets, int b) {
+  int d = 0;
+  while (d + 1 < sets.n && b >= sets.item0[d + 1]) ++d;
+  return d;
+}
+
 __device__ inline int64_t tri0(int64_t a) { return a * (a + 1) / 2; }
 
 // Maps blockIdx.x so that consecutive logical blocks land on the same XCD
@@ -200,8 +220,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 // strictly above the diagonal are skipped (those entries are never read).
 template <class P, int TM, int BKF>
 __global__ __launch_bounds__(kThreads) void k_syrk(
-    const typename P::T *__restrict__ X, int n_pad, int ldc, int off, int opad, int64_t jc,
-    int64_t jper, int ntiles, double *__restrict__ kpart, int beta) {
+    const typename P::T *__restrict__ X, int n_pad, int ldc, int64_t jc, int ntiles,
+    const SyrkSets sets, double *__restrict__ kpart_all, int beta) {
   typedef typename P::T T;
   typedef typename P::Acc Acc;
   constexpr int WT = TM / 2;
@@ -210,8 +230,13 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   static_assert(BKF * TM % kThreads == 0, "tile shape");
   __shared__ T sA[BKF * TM];
   __shared__ T sB[BKF * TM];
-  const int tile = blockIdx.x % ntiles;
-  const int s = blockIdx.x / ntiles;
+  const int set = find_set(sets, blockIdx.x);
+  const int local = blockIdx.x - sets.item0[set];
+  const int tile = local % ntiles;
+  const int s = local / ntiles;
+  const int off = sets.off[set], opad = sets.opad[set];
+  const int64_t jper = (jc + sets.nsplit[set] - 1) / sets.nsplit[set];
+  double *__restrict__ kpart = kpart_all + sets.kbase[set];
   int A = int((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
   while (A * (A + 1) / 2 > tile) --A;
   while ((A + 1) * (A + 2) / 2 <= tile) ++A;
@@ -473,8 +498,8 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
 // as k_syrk.
 template <int TM>
 __global__ __launch_bounds__(kThreads) void k_syrk_b3(
-    const float *__restrict__ X, int n_pad, int ldc, int off, int opad, int64_t jc, int64_t jper,
-    int ntiles, double *__restrict__ kpart, int beta) {
+    const float *__restrict__ X, int n_pad, int ldc, int64_t jc, int ntiles, const SyrkSets sets,
+    double *__restrict__ kpart_all, int beta) {
   constexpr int WT = TM / 2;
   constexpr int MT = WT / 16;
   constexpr int PN = kB3K * TM / kThreads;
@@ -482,8 +507,13 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
   __shared__ __attribute__((aligned(16))) __bf16 sAl[TM * kB3S];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[TM * kB3S];
   __shared__ __attribute__((aligned(16))) __bf16 sBl[TM * kB3S];
-  const int tile = blockIdx.x % ntiles;
-  const int s = blockIdx.x / ntiles;
+  const int set = find_set(sets, blockIdx.x);
+  const int local = blockIdx.x - sets.item0[set];
+  const int tile = local % ntiles;
+  const int s = local / ntiles;
+  const int off = sets.off[set], opad = sets.opad[set];
+  const int64_t jper = (jc + sets.nsplit[set] - 1) / sets.nsplit[set];
+  double *__restrict__ kpart = kpart_all + sets.kbase[set];
   int A = int((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
   while (A * (A + 1) / 2 > tile) --A;
   while ((A + 1) * (A + 2) / 2 <= tile) ++A;
@@ -581,14 +611,21 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
       }
 }
 
-// klt[a*(a+1)/2 + b] = sum_s kpart[(s*ntiles + tile(a,b))*TM*TM + (a%TM)*TM + b%TM]
-// grid.x = n (row a), block 256.
-__global__ void k_reduce_k(const double *__restrict__ kpart, int n, int tm, int ntiles,
-                           int nsplit, double *__restrict__ klt) {
-  const int a = blockIdx.x;
+// klt[d*ldk + a*(a+1)/2 + b] = sum_s kpart[kbase[d] + (s*ntiles + tile(a,b))*TM*TM
+//                                           + (a%TM)*TM + b%TM]
+// grid = (n rows a, nsets), block 256.
+__global__ void k_reduce_k(const double *__restrict__ kpart_all, int n, int tm, int ntiles,
+                           const SyrkSets sets, double *__restrict__ klt, int64_t ldk) {
+  const int a = blockIdx.x, d = blockIdx.y;
   const int A = a / tm, al = a % tm;
-  double *__restrict__ out = klt + tri0(a);
+  const int nsplit = sets.nsplit[d];
+  const double *__restrict__ kpart = kpart_all + sets.kbase[d];
+  double *__restrict__ out = klt + d * ldk + tri0(a);
   for (int b = threadIdx.x; b <= a; b += blockDim.x) {
+    if (nsplit == 0) {
+      out[b] = 0.0;
+      continue;
+    }
     const int B = b / tm, bl = b % tm;
     const int64_t t = int64_t(A) * (A + 1) / 2 + B;
     const double *__restrict__ src = kpart + t * tm * tm + al * tm + bl;
@@ -609,16 +646,24 @@ void launch_ht(Context *c, const typename P::T *L, const typename P::T *cb, int6
                      c->ldl, c->n, c->n_pad, cb, j0, ntile_a, nblocks, X);
 }
 
-// Wide AO tiles while the accumulators fit, narrow ones for many orbitals.
+// AO tile width: 128 while the accumulators fit (ot <= 6), 64 beyond; the
+// context's ht_ta overrides (autotuner).
+inline int ht_tile(const Context *c, int ot) {
+  if (c->ht_ta == 64) return 64;
+  if (c->ht_ta == 128 && ot <= 6) return 128;
+  return ot <= 6 ? 128 : 64;
+}
+
 template <class P>
 void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T *cb, int64_t j0,
                  int64_t jn, typename P::T *X) {
+  const bool wide = ht_tile(c, ot) == 128;
   switch (ot) {
-    case 1: launch_ht<P, 1, 128, 32>(c, L, cb, j0, jn, X); break;
-    case 2: launch_ht<P, 2, 128, 32>(c, L, cb, j0, jn, X); break;
-    case 3: launch_ht<P, 3, 128, 32>(c, L, cb, j0, jn, X); break;
-    case 4: launch_ht<P, 4, 128, 32>(c, L, cb, j0, jn, X); break;
-    case 6: launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X); break;
+    case 1: wide ? launch_ht<P, 1, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 1, 64, 32>(c, L, cb, j0, jn, X); break;
+    case 2: wide ? launch_ht<P, 2, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 2, 64, 32>(c, L, cb, j0, jn, X); break;
+    case 3: wide ? launch_ht<P, 3, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 3, 64, 32>(c, L, cb, j0, jn, X); break;
+    case 4: wide ? launch_ht<P, 4, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 4, 64, 32>(c, L, cb, j0, jn, X); break;
+    case 6: wide ? launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 6, 64, 32>(c, L, cb, j0, jn, X); break;
     case 8: launch_ht<P, 8, 64, 32>(c, L, cb, j0, jn, X); break;
     case 10: launch_ht<P, 10, 64, 32>(c, L, cb, j0, jn, X); break;
     case 12: launch_ht<P, 12, 64, 16>(c, L, cb, j0, jn, X); break;
@@ -627,14 +672,14 @@ void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T
 }
 
 template <class P>
-void syrk_launch(Context *c, int tm, dim3 grid, const typename P::T *X, int ldc, int off, int opad,
-                 int64_t jn, int64_t jper, int ntiles, double *kpart, int beta) {
+void syrk_launch(Context *c, int tm, int items, const typename P::T *X, int ldc, int64_t jn,
+                 int ntiles, const SyrkSets &sets, double *kpart, int beta) {
   if (tm == 128)
-    hipLaunchKernelGGL((k_syrk<P, 128, 16>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc,
-                       off, opad, jn, jper, ntiles, kpart, beta);
+    hipLaunchKernelGGL((k_syrk<P, 128, 16>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
   else
-    hipLaunchKernelGGL((k_syrk<P, 64, 32>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc,
-                       off, opad, jn, jper, ntiles, kpart, beta);
+    hipLaunchKernelGGL((k_syrk<P, 64, 32>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
 }
 
 template <int OT, int TA>
@@ -648,12 +693,13 @@ void launch_ht_b3(Context *c, const float *L, const __bf16 *ch, const __bf16 *cl
 
 void ht_dispatch_b3(Context *c, int ot, const float *L, const __bf16 *ch, const __bf16 *cl,
                     int64_t j0, int64_t jn, float *X) {
+  const bool wide = ht_tile(c, ot) == 128;
   switch (ot) {
-    case 1: launch_ht_b3<1, 128>(c, L, ch, cl, j0, jn, X); break;
-    case 2: launch_ht_b3<2, 128>(c, L, ch, cl, j0, jn, X); break;
-    case 3: launch_ht_b3<3, 128>(c, L, ch, cl, j0, jn, X); break;
-    case 4: launch_ht_b3<4, 128>(c, L, ch, cl, j0, jn, X); break;
-    case 6: launch_ht_b3<6, 128>(c, L, ch, cl, j0, jn, X); break;
+    case 1: wide ? launch_ht_b3<1, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<1, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 2: wide ? launch_ht_b3<2, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<2, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 3: wide ? launch_ht_b3<3, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<3, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 4: wide ? launch_ht_b3<4, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<4, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 6: wide ? launch_ht_b3<6, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<6, 64>(c, L, ch, cl, j0, jn, X); break;
     case 8: launch_ht_b3<8, 64>(c, L, ch, cl, j0, jn, X); break;
     case 10: launch_ht_b3<10, 64>(c, L, ch, cl, j0, jn, X); break;
     case 12: launch_ht_b3<12, 64>(c, L, ch, cl, j0, jn, X); break;
@@ -661,14 +707,14 @@ void ht_dispatch_b3(Context *c, int ot, const float *L, const __bf16 *ch, const 
   }
 }
 
-void syrk_launch_b3(Context *c, int tm, dim3 grid, const float *X, int ldc, int off, int opad,
-                    int64_t jn, int64_t jper, int ntiles, double *kpart, int beta) {
+void syrk_launch_b3(Context *c, int tm, int items, const float *X, int ldc, int64_t jn, int ntiles,
+                    const SyrkSets &sets, double *kpart, int beta) {
   if (tm == 128)
-    hipLaunchKernelGGL((k_syrk_b3<128>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc, off,
-                       opad, jn, jper, ntiles, kpart, beta);
+    hipLaunchKernelGGL((k_syrk_b3<128>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
   else
-    hipLaunchKernelGGL((k_syrk_b3<64>), grid, dim3(kThreads), 0, c->stream, X, c->n_pad, ldc, off,
-                       opad, jn, jper, ntiles, kpart, beta);
+    hipLaunchKernelGGL((k_syrk_b3<64>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
 }
 
 int pack_bank_b3(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
@@ -676,11 +722,11 @@ int pack_bank_b3(Context *c, int32_t nk, const int32_t *nocc, const double *cons
   const int n = c->n, n_pad = c->n_pad;
   const size_t half = sizeof(__bf16) * size_t(ldc) * n_pad;
   int rc;
-  if ((rc = ensure(c, c->ws_cbank, 2 * half))) return rc;
-  __bf16 *ch = static_cast<__bf16 *>(c->ws_cbank.ptr);
+  if ((rc = ensure(c, c->w->cbank, 2 * half))) return rc;
+  __bf16 *ch = static_cast<__bf16 *>(c->w->cbank.ptr);
   __bf16 *cl = ch + size_t(ldc) * n_pad;
   phase_begin(c);
-  if ((rc = hip_check(c, hipMemsetAsync(ch, 0, 2 * half, c->stream), "memset bank"))) return rc;
+  if ((rc = fill_zero(c, ch, 2 * half))) return rc;
   for (int d = 0; d < nk; ++d) {
     if (nocc[d] == 0) continue;
     const int64_t cnt = int64_t(nocc[d]) * n;
@@ -722,10 +768,10 @@ int pack_bank_t(Context *c, int32_t nk, const int32_t *nocc, const double *const
   const int n = c->n, n_pad = c->n_pad;
   const size_t bytes = sizeof(T) * size_t(n_pad) * ldc;
   int rc;
-  if ((rc = ensure(c, c->ws_cbank, bytes))) return rc;
-  T *cb = static_cast<T *>(c->ws_cbank.ptr);
+  if ((rc = ensure(c, c->w->cbank, bytes))) return rc;
+  T *cb = static_cast<T *>(c->w->cbank.ptr);
   phase_begin(c);
-  if ((rc = hip_check(c, hipMemsetAsync(cb, 0, bytes, c->stream), "memset bank"))) return rc;
+  if ((rc = fill_zero(c, cb, bytes))) return rc;
   for (int d = 0; d < nk; ++d) {
     if (nocc[d] == 0) continue;
     const int64_t cnt = int64_t(nocc[d]) * n;
@@ -762,9 +808,7 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
   if ((rc = bank_layout(c, nk, nocc, off, opad, &ot, &tot))) return rc;
   if (tot == 0) {  // no orbitals: K = 0
     for (int d = 0; d < nk; ++d)
-      if ((rc = hip_check(c, hipMemsetAsync(klt + d * ldk, 0, sizeof(double) * c->np, c->stream),
-                          "memset K")))
-        return rc;
+      if ((rc = fill_zero(c, klt + d * ldk, sizeof(double) * c->np))) return rc;
     return MJK_OK;
   }
   const int ldc = 16 * ot;
@@ -774,8 +818,8 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
   } else {
     if ((rc = pack_bank_t<T>(c, nk, nocc, corb, off, ldc))) return rc;
   }
-  const T *cb = static_cast<const T *>(c->ws_cbank.ptr);
-  const __bf16 *cbh = static_cast<const __bf16 *>(c->ws_cbank.ptr);
+  const T *cb = static_cast<const T *>(c->w->cbank.ptr);
+  const __bf16 *cbh = static_cast<const __bf16 *>(c->w->cbank.ptr);
   const __bf16 *cbl = cbh + size_t(ldc) * n_pad;
   const T *L;
   if constexpr (sizeof(T) == sizeof(double)) {
@@ -792,26 +836,34 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
   jc = ceil_div(M, nchunk);
   if (jc * ldc >= (int64_t(1) << 31))
     return set_error(c, MJK_ERR_ARG, "exchange: chunk contraction length exceeds 2^31");
-  if ((rc = ensure(c, c->ws_x, size_t(xvec_bytes) * jc))) return rc;
-  T *X = static_cast<T *>(c->ws_x.ptr);
+  if ((rc = ensure(c, c->w->x, size_t(xvec_bytes) * jc))) return rc;
+  T *X = static_cast<T *>(c->w->x.ptr);
 
-  // ---- SYRK tiling and slices
+  // ---- SYRK tiling and slices (one launch covers every density)
+  if (nk > kMaxSets) return set_error(c, MJK_ERR_ARG, "exchange: more than 16 orbital sets");
   int tm = c->tile_syrk ? c->tile_syrk : (n_pad >= 512 && n_pad % 128 == 0 ? 128 : 64);
   if (n_pad % tm) tm = 64;
   const int ntile1 = n_pad / tm;
   const int ntiles = ntile1 * (ntile1 + 1) / 2;
-  std::vector<int> nsplit(static_cast<size_t>(nk));
-  std::vector<int64_t> kbase(size_t(nk) + 1, 0);
+  const int target = c->split_target ? c->split_target : 1024;
   int active = 0;
   for (int d = 0; d < nk; ++d) active += nocc[d] > 0;
+  SyrkSets sets{};
+  sets.n = nk;
+  int64_t kb = 0;
+  sets.item0[0] = 0;
   for (int d = 0; d < nk; ++d) {
-    const int64_t want = ceil_div(1024, int64_t(ntiles) * std::max(active, 1));
-    nsplit[size_t(d)] = nocc[d] > 0 ? int(std::max<int64_t>(1, std::min<int64_t>(want, jc))) : 0;
-    kbase[size_t(d) + 1] = kbase[size_t(d)] + int64_t(nsplit[size_t(d)]) * ntiles * tm * tm;
+    const int64_t want = ceil_div(target, int64_t(ntiles) * std::max(active, 1));
+    sets.nsplit[d] = nocc[d] > 0 ? int(std::max<int64_t>(1, std::min<int64_t>(want, jc))) : 0;
+    sets.off[d] = off[size_t(d)];
+    sets.opad[d] = std::max(opad[size_t(d)], 1);
+    sets.kbase[d] = kb;
+    kb += int64_t(sets.nsplit[d]) * ntiles * tm * tm;
+    sets.item0[d + 1] = sets.item0[d] + sets.nsplit[d] * ntiles;
   }
-  if ((rc = ensure(c, c->ws_kpart, sizeof(double) * std::max<int64_t>(kbase[size_t(nk)], 1))))
-    return rc;
-  double *kpart = static_cast<double *>(c->ws_kpart.ptr);
+  const int items = sets.item0[nk];
+  if ((rc = ensure(c, c->w->kpart, sizeof(double) * size_t(std::max<int64_t>(kb, 1))))) return rc;
+  double *kpart = static_cast<double *>(c->w->kpart.ptr);
 
   for (int64_t ch = 0; ch < nchunk; ++ch) {
     const int64_t j0 = ch * jc;
@@ -823,30 +875,15 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
       ht_dispatch<P>(c, ot, L, cb, j0, jn, X);
     phase_end(c, MJK_PHASE_HALF_TRANSFORM);
     phase_begin(c);
-    for (int d = 0; d < nk; ++d) {
-      if (!nsplit[size_t(d)]) continue;
-      const int64_t jper = ceil_div(jn, nsplit[size_t(d)]);
-      const dim3 grid(unsigned(ntiles * nsplit[size_t(d)]));
-      if constexpr (kB3)
-        syrk_launch_b3(c, tm, grid, X, ldc, off[size_t(d)], opad[size_t(d)], jn, jper, ntiles,
-                       kpart + kbase[size_t(d)], ch > 0 ? 1 : 0);
-      else
-        syrk_launch<P>(c, tm, grid, X, ldc, off[size_t(d)], opad[size_t(d)], jn, jper, ntiles,
-                       kpart + kbase[size_t(d)], ch > 0 ? 1 : 0);
-    }
+    if constexpr (kB3)
+      syrk_launch_b3(c, tm, items, X, ldc, jn, ntiles, sets, kpart, ch > 0 ? 1 : 0);
+    else
+      syrk_launch<P>(c, tm, items, X, ldc, jn, ntiles, sets, kpart, ch > 0 ? 1 : 0);
     phase_end(c, MJK_PHASE_EXCHANGE);
   }
   phase_begin(c);
-  for (int d = 0; d < nk; ++d) {
-    if (!nsplit[size_t(d)]) {
-      if ((rc = hip_check(c, hipMemsetAsync(klt + d * ldk, 0, sizeof(double) * c->np, c->stream),
-                          "memset K")))
-        return rc;
-      continue;
-    }
-    hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n)), dim3(256), 0, c->stream,
-                       kpart + kbase[size_t(d)], n, tm, ntiles, nsplit[size_t(d)], klt + d * ldk);
-  }
+  hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n), unsigned(nk)), dim3(256), 0, c->stream, kpart, n,
+                     tm, ntiles, sets, klt, ldk);
   phase_end(c, MJK_PHASE_REDUCE);
   return hip_check(c, hipGetLastError(), "exchange launch");
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -29,6 +30,16 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// Device workspaces.  Contexts created with mjk_create_shared use the same
+// Workspace and must queue their work on the same stream.
+struct Workspace {
+  DevBuf d, vpart, v, jpart, cbank, x, kpart, stage, j, k, red, misc, sets;
+  ~Workspace() {
+    for (DevBuf *b : {&d, &vpart, &v, &jpart, &cbank, &x, &kpart, &stage, &j, &k, &red, &misc, &sets})
+      if (b->ptr) (void)hipFree(b->ptr);
+  }
+};
+
 struct ReducedSet {
   int64_t nrs = 0;
   bool identity = false;      // rs position r maps to LT index r for all r
@@ -51,13 +62,17 @@ struct Context {
   int device = 0;
 
   int64_t x_budget = int64_t(16) << 30;  // bytes for one chunk of X
-  int32_t tile_syrk = 0;                  // 0 = automatic, else 64 or 128
 
   // workspaces
   DevBuf ws_l32;          // FP32 shadow of L (mixed-precision exchange)
   bool l32_valid = false;
-  DevBuf ws_d, ws_vpart, ws_v, ws_jpart, ws_cbank, ws_x, ws_kpart, ws_stage, ws_j,
-      ws_k, ws_red, ws_misc;
+  std::shared_ptr<Workspace> wsp;  // workspaces, possibly shared with other contexts
+  Workspace *w = nullptr;
+
+  // tunables (0 = automatic); set by mjk_set_tuning or the autotuner
+  int32_t tile_syrk = 0;     // SYRK tile: 64 or 128
+  int32_t split_target = 0;  // SYRK workgroups per launch to aim for
+  int32_t ht_ta = 0;         // half-transform AO tile: 64 or 128
 
   // profiling: per-phase accumulated milliseconds and launch counts
   bool profile = false;
@@ -82,6 +97,10 @@ void phase_end(Context *c, int phase);
 int resolve_phases(Context *c);
 
 int set_error(Context *c, int rc, const std::string &msg);
+
+// Zero-fills device memory with a kernel (no hipMemsetAsync, so the call can
+// sit inside a captured HIP graph).
+int fill_zero(Context *c, void *ptr, size_t bytes);
 int hip_check(Context *c, hipError_t e, const char *what);
 
 // ---- kernels (launchers) -------------------------------------------------

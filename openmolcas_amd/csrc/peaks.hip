@@ -75,9 +75,9 @@ extern "C" int mjk_measure_peak(mjk_context *c, int32_t kind, double *value) {
   if (!c || !value) return MJK_ERR_ARG;
   const int64_t copy_bytes = int64_t(1) << 30;
   int rc;
-  if ((rc = ensure(c, c->ws_misc, kind == 4 ? size_t(2 * copy_bytes) : sizeof(double) * kBlocks)))
+  if ((rc = ensure(c, c->w->misc, kind == 4 ? size_t(2 * copy_bytes) : sizeof(double) * kBlocks)))
     return rc;
-  double *buf = static_cast<double *>(c->ws_misc.ptr);
+  double *buf = static_cast<double *>(c->w->misc.ptr);
   hipEvent_t e0, e1;
   if ((rc = hip_check(c, hipEventCreate(&e0), "event")) || (rc = hip_check(c, hipEventCreate(&e1), "event")))
     return rc;

@@ -90,19 +90,18 @@ extern "C" int mjk_puvx(mjk_context *c, int32_t naorb, const double *porb3, cons
   const double *sets[1] = {porb3};
   if ((rc = pack_bank(c, 1, &naorb, sets, off, opad, &ot, &tot))) return rc;
   const int ldc = 16 * ot;
-  const double *cb = static_cast<const double *>(c->ws_cbank.ptr);
+  const double *cb = static_cast<const double *>(c->w->cbank.ptr);
   const int64_t yvec = sizeof(double) * int64_t(ldc) * n_pad;
   int64_t jc = std::max<int64_t>(1, std::min<int64_t>(M, c->x_budget / yvec));
   const int64_t nchunk = ceil_div(M, jc);
   jc = ceil_div(M, nchunk);
-  if ((rc = ensure(c, c->ws_x, size_t(yvec * jc)))) return rc;
-  double *Y = static_cast<double *>(c->ws_x.ptr);
+  if ((rc = ensure(c, c->w->x, size_t(yvec * jc)))) return rc;
+  double *Y = static_cast<double *>(c->w->x.ptr);
   const size_t scr_n = size_t(nxy) * naorb * n_pad;
-  if ((rc = ensure(c, c->ws_k, sizeof(double) * (scr_n + size_t(jc) * nxy)))) return rc;
-  double *scr = static_cast<double *>(c->ws_k.ptr);
+  if ((rc = ensure(c, c->w->k, sizeof(double) * (scr_n + size_t(jc) * nxy)))) return rc;
+  double *scr = static_cast<double *>(c->w->k.ptr);
   double *lxy = scr + scr_n;
-  if ((rc = hip_check(c, hipMemsetAsync(scr, 0, sizeof(double) * scr_n, c->stream), "zero Scr")))
-    return rc;
+  if ((rc = fill_zero(c, scr, sizeof(double) * scr_n))) return rc;
   for (int64_t ch = 0; ch < nchunk; ++ch) {
     const int64_t j0 = ch * jc, jn = std::min(jc, M - j0);
     phase_begin(c);

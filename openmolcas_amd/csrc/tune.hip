@@ -1,0 +1,60 @@
+// Autotuner of the exchange tiling: for the orbital sets of one call it times
+// every combination of SYRK tile (64, 128), SYRK workgroups per launch (512 ..
+// 4096) and half-transform AO tile (64, 128) with HIP events and keeps the
+// fastest in the context.  Shapes differ enough between molecules (n 100-600,
+// o 10-120) that no single tiling wins everywhere.
+#include "mjk_internal.h"
+
+using namespace mjk;
+
+extern "C" int mjk_autotune(mjk_context *c, int32_t nk, const int32_t *nocc,
+                            const double *const *corb, double *klt, int64_t ldk,
+                            int32_t precision, double *best_ms) {
+  if (!c) return MJK_ERR_ARG;
+  if (!nocc || !corb || !klt) return set_error(c, MJK_ERR_ARG, "autotune: null pointer");
+  const bool was_profiling = c->profile;
+  c->profile = false;
+  hipEvent_t e0, e1;
+  int rc;
+  if ((rc = hip_check(c, hipEventCreate(&e0), "event")) || (rc = hip_check(c, hipEventCreate(&e1), "event")))
+    return rc;
+  const int tms[] = {64, 128};
+  const int splits[] = {512, 1024, 2048, 4096};
+  const int tas[] = {64, 128};
+  float best = 1e30f;
+  int bt = 0, bs = 0, ba = 0;
+  // warm-up with the automatic choice (allocates the workspaces once)
+  c->tile_syrk = c->split_target = c->ht_ta = 0;
+  if ((rc = exchange(c, nk, nocc, corb, klt, ldk, precision))) goto done;
+  for (int tm : tms) {
+    if (c->n_pad % tm) continue;
+    for (int sp : splits) {
+      for (int ta : tas) {
+        c->tile_syrk = tm;
+        c->split_target = sp;
+        c->ht_ta = ta;
+        if ((rc = hip_check(c, hipEventRecord(e0, c->stream), "record"))) goto done;
+        if ((rc = exchange(c, nk, nocc, corb, klt, ldk, precision))) goto done;
+        if ((rc = hip_check(c, hipEventRecord(e1, c->stream), "record"))) goto done;
+        if ((rc = hip_check(c, hipEventSynchronize(e1), "sync"))) goto done;
+        float ms = 0.f;
+        if ((rc = hip_check(c, hipEventElapsedTime(&ms, e0, e1), "elapsed"))) goto done;
+        if (ms < best) {
+          best = ms;
+          bt = tm;
+          bs = sp;
+          ba = ta;
+        }
+      }
+    }
+  }
+  c->tile_syrk = bt;
+  c->split_target = bs;
+  c->ht_ta = ba;
+  if (best_ms) *best_ms = best;
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->profile = was_profiling;
+  return rc;
+}

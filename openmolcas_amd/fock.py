@@ -63,7 +63,7 @@ def _host_f64(a) -> np.ndarray:
 class JKContext:
     """One device context for nbas basis functions and nchol vectors (nSym=1)."""
 
-    def __init__(self, nbas: int, nchol: int, device=None):
+    def __init__(self, nbas: int, nchol: int, device=None, share: "JKContext" = None):
         if not torch.cuda.is_available():
             raise RuntimeError("JKContext needs a HIP device (MI355X); there is no CPU fallback")
         self.lib = _lib.load()
@@ -73,11 +73,18 @@ class JKContext:
         self.npair = self.nbas * (self.nbas + 1) // 2
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            rc = self.lib.mjk_create(ctypes.byref(h), self.nbas, self.nchol)
+            if share is None:
+                rc = self.lib.mjk_create(ctypes.byref(h), self.nbas, self.nchol)
+            else:
+                rc = self.lib.mjk_create_shared(ctypes.byref(h), self.nbas, self.nchol, share._h)
         if rc != 0:
             raise MjkError(rc, "mjk_create failed")
         self._h = h
-        self.bind_stream(torch.cuda.current_stream(self.device))
+        self._share = share  # keeps the workspace owner alive
+        if share is None:
+            self.bind_stream(torch.cuda.current_stream(self.device))
+        else:
+            self.stream = share.stream
 
     # ------------------------------------------------------------ plumbing
     def _check(self, rc: int):
@@ -254,10 +261,44 @@ class JKContext:
                                               float(alpha), float(beta), _ptr(dl), _ld(dl)))
         return dl
 
+    def device_vectors(self) -> torch.Tensor:
+        """(nchol, npair) float64 view of the resident vector store (zero copy);
+        writing it replaces the vectors."""
+        p = ctypes.c_void_p()
+        ld = ctypes.c_int64()
+        self._check(self.lib.mjk_device_vectors(self._h, ctypes.byref(p), ctypes.byref(ld)))
+
+        class _View:
+            __cuda_array_interface__ = {
+                "shape": (self.nchol, self.npair), "strides": (8 * ld.value, 8),
+                "typestr": "<f8", "data": (p.value, False), "version": 3}
+
+        t = torch.as_tensor(_View(), device=self.device)
+        self._resident = ("device", t.data_ptr())
+        return t
+
+    def set_tuning(self, tile_syrk: int = 0, split_target: int = 0, ht_ta: int = 0) -> None:
+        self._check(self.lib.mjk_set_tuning(self._h, int(tile_syrk), int(split_target), int(ht_ta)))
+
+    def autotune(self, corbs, precision="fp64") -> float:
+        """Times the exchange tilings for these orbital sets; keeps the fastest."""
+        cs = [_dev_f64(c, self.device) for c in corbs]
+        nk = len(cs)
+        nocc = np.array([c.shape[1] for c in cs], dtype=np.int32)
+        ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
+        k = torch.empty((nk, self.npair), dtype=torch.float64, device=self.device)
+        best = ctypes.c_double()
+        self._check(self.lib.mjk_autotune(self._h, nk, _ptr(nocc), ptrs, _ptr(k), _ld(k),
+                                          precision_code(precision), ctypes.byref(best)))
+        return best.value
+
     def ensure_vectors(self, L: torch.Tensor) -> None:
         """Uploads L (nchol, npair) unless this exact tensor version is resident."""
+        res = getattr(self, "_resident", None)
+        if res is not None and res[0] == "device" and res[1] == L.data_ptr():
+            return  # L is the context's own store
         key = (L.data_ptr(), L._version, tuple(L.shape))
-        if getattr(self, "_resident", None) != key:
+        if res != key:
             self.upload_packed(L.detach())
             self._resident = key
 

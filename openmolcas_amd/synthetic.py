@@ -80,15 +80,17 @@ def fold(a: np.ndarray) -> np.ndarray:
 
 
 # ------------------------------------------------------------------ torch (HBM)
-def chol_vectors_torch(n: int, M: int, seed: int, device="cuda", dtype=None, chunk: int = 256):
-    """(M, n(n+1)/2) packed vectors generated on the device in chunks."""
+def chol_vectors_torch(n: int, M: int, seed: int, device="cuda", dtype=None, chunk: int = 256,
+                       out=None):
+    """(M, n(n+1)/2) packed vectors generated on the device in chunks (into
+    `out` when given, e.g. a context's resident store)."""
     import torch
 
     dtype = dtype or torch.float64
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     npr = npair(n)
-    L = torch.empty((M, npr), dtype=dtype, device=device)
+    L = torch.empty((M, npr), dtype=dtype, device=device) if out is None else out
     scales = torch.tensor(vector_scales(M), dtype=dtype, device=device)
     di = torch.tensor(diag_index(n), dtype=torch.long, device=device)
     for j0 in range(0, M, chunk):
