@@ -36,6 +36,9 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
+    p.add_argument("--extra", action="store_true",
+                   help="also time the other workloads (n=1200 build + backward, 9-density batch, "
+                        "64-system graph batch); adds an 'extra' object")
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed region, copy 1 GiB device to device (calibrates HBM counters)")
     p.add_argument("--cpu-sample-vectors", type=int, default=1 << 30)
@@ -133,6 +136,124 @@ def profiled_traffic(kernel_prefix, n, M, o, precision):
         if k.startswith(kernel_prefix):
             return v["fetch_bytes"] + v["write_bytes"]
     return None
+
+
+def _timeit(fn, reps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def extra_workloads(dev):
+    """Secondary measurements: the n=1200/nchol=6000/nocc=150 build with its
+    backward, the 9-density state-averaged batch (n=800, nchol=3600) and the
+    64-system uneven batch as one graph replay.  FP64 and mixed."""
+    import numpy as np
+    import torch
+
+    from openmolcas_amd.batch import MoleculeBatch, uneven_shapes
+    from openmolcas_amd.fock import JKContext
+    from openmolcas_amd.synthetic import chol_vectors_torch, npair, orbitals_torch
+
+    out = {}
+    # -- n=1200, nchol=6000, nocc=150: forward J+K and backward
+    n, M, o = 1200, 6000, 150
+    jk = JKContext(n, M, device=dev)
+    L = jk.device_vectors()
+    chol_vectors_torch(n, M, 3, device=dev, out=L)
+    C = orbitals_torch(n, o, 4, device=dev)
+    dlt = jk.density(C, 2.0)
+    res = {"workload": "nbas=1200 nchol=6000 nocc=150, P = 2CC^T"}
+    J, V = jk.coulomb(dlt, want_v=True)
+    K64 = jk.exchange([C])
+    G = torch.zeros_like(J)
+    jk.fock_update(G, J, [0], [1.0], K64, [-1.0])
+    h = torch.zeros_like(J)
+    e64 = jk.energy(h, dlt, G).item()
+    res["coulomb_ms"] = round(1e3 * _timeit(lambda: jk.coulomb(dlt), 3), 3)
+    bytes_j = 2.0 * M * npair(n) * 8
+    res["coulomb_GBs"] = round(bytes_j / (res["coulomb_ms"] * 1e-3) / 1e9, 1)
+    fk = M * o * (2.0 * n * n + n * (n + 1))
+    for prec in ("fp64", "fp32", "bf16x3"):
+        t = _timeit(lambda: jk.exchange([C], precision=prec), 2)
+        K = jk.exchange([C], precision=prec)
+        Gp = torch.zeros_like(J)
+        jk.fock_update(Gp, J, [0], [1.0], K, [-1.0])
+        e = jk.energy(h, dlt, Gp).item()
+        res[f"exchange_{prec}_ms"] = round(1e3 * t, 3)
+        res[f"exchange_{prec}_TFs"] = round(fk / t / 1e12, 2)
+        res[f"dE_{prec}"] = abs(e - e64)
+        if prec != "fp64":
+            F = Gp
+            dC_mixed = jk.grad_orbitals(C, F, 4.0)
+            dC64 = jk.grad_orbitals(C, G, 4.0)
+            res[f"dEdC_maxabs_err_{prec}"] = float(torch.max(torch.abs(dC_mixed - dC64)).item())
+    dl = torch.empty((M, npair(n)), dtype=torch.float64, device=dev)
+    res["backward_dC_ms"] = round(1e3 * _timeit(lambda: jk.grad_orbitals(C, G, 4.0), 3), 3)
+    res["backward_dL_ms"] = round(1e3 * _timeit(
+        lambda: jk.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=dl), 2), 3)
+    out["nbas1200"] = res
+    del dl, L
+    jk.close()
+    torch.cuda.empty_cache()
+
+    # -- state-averaged batch: n=800, nchol=3600, 1 x rank 120 + 8 x U{4..16}
+    n, M = 800, 3600
+    rng = np.random.default_rng(11)
+    ranks = [120] + [int(r) for r in rng.integers(4, 17, size=8)]
+    jk = JKContext(n, M, device=dev)
+    L = jk.device_vectors()
+    chol_vectors_torch(n, M, 12, device=dev, out=L)
+    Q = orbitals_torch(n, sum(ranks), 13, device=dev)
+    Cs, dl, o0 = [], [], 0
+    ii, jj = torch.tril_indices(n, n, device=dev)
+    w = torch.where(ii == jj, 1.0, 2.0).to(torch.float64)
+    for d, r in enumerate(ranks):
+        occ = torch.full((r,), 2.0, dtype=torch.float64, device=dev) if d == 0 else \
+            torch.tensor(rng.uniform(0.05, 1.95, r), dtype=torch.float64, device=dev)
+        Cd = (Q[:, o0:o0 + r] * torch.sqrt(occ)).contiguous()
+        o0 += r
+        Cs.append(Cd)
+        dl.append(((Cd @ Cd.T)[ii, jj] * w).contiguous())
+    D = torch.stack(dl)
+    res = {"workload": f"nbas=800 nchol=3600, 9 densities ranks {ranks}"}
+    tj = _timeit(lambda: jk.coulomb(D), 3)
+    res["coulomb_9dens_ms"] = round(1e3 * tj, 3)
+    res["coulomb_GBs"] = round(2.0 * M * npair(n) * 8 / tj / 1e9, 1)
+    fk = sum(M * r * (2.0 * n * n + n * (n + 1)) for r in ranks)
+    for prec in ("fp64", "fp32", "bf16x3"):
+        t = _timeit(lambda: jk.exchange(Cs, precision=prec), 3)
+        res[f"exchange_{prec}_ms"] = round(1e3 * t, 3)
+        res[f"exchange_{prec}_TFs"] = round(fk / t / 1e12, 2)
+    res["builds_per_s_fp64"] = round(1.0 / (tj + res["exchange_fp64_ms"] * 1e-3), 3)
+    out["state_averaged_9"] = res
+    del L
+    jk.close()
+    torch.cuda.empty_cache()
+
+    # -- 64 uneven systems, forward + backward, one graph replay
+    shapes = uneven_shapes(64, seed=5)
+    batch = MoleculeBatch(shapes, device=dev, seed=3)
+    batch.autotune("fp64")
+    res = {"workload": "64 systems nbas U[100,600], nchol U[4,6] nbas, nocc U[0.1,0.2] nbas; "
+                       "forward + backward per graph replay"}
+    for prec in ("fp64", "fp32"):
+        batch.capture(prec)
+        t = _timeit(batch.replay, 3)
+        res[f"graph_replay_{prec}_ms"] = round(1e3 * t, 2)
+        res[f"E_{prec}"] = batch.E.clone()
+    res["fwd_exchange_TFs_fp64"] = round(batch.flops() / (res["graph_replay_fp64_ms"] * 1e-3) / 1e12, 2)
+    res["maxabs_dE_fp32"] = float(torch.max(torch.abs(res.pop("E_fp32") - res.pop("E_fp64"))).item())
+    res["speedup_fp32"] = round(res["graph_replay_fp64_ms"] / res["graph_replay_fp32_ms"], 3)
+    out["molecule_batch_64"] = res
+    batch.close()
+    return out
 
 
 def main():
@@ -271,12 +392,17 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.precision == "fp64" and not args.no_mixed:
+    if args.extra:
+        ctx.close()
+        torch.cuda.empty_cache()
+        out["extra"] = extra_workloads(dev)
+    if args.precision == "fp64" and not args.no_mixed and not args.extra:
         out["mixed"] = mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_step, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, M, o, args.seed, args.cpu_sample_vectors)
     print(json.dumps(out), flush=True)
-    ctx.close()
+    if not args.extra:
+        ctx.close()
 
 
 if __name__ == "__main__":

@@ -144,46 +144,40 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 
   if (a0 < n) {
     T rc[CN], rl[LN];
-    int lo[LN];
-    // global -> registers for the chunk starting at b0
-    auto fetch = [&](int b0) {
+    // Element i of a thread: "row-major" chunks (b > a and diagonal blocks)
+    // walk e -> (bb = e / TA, aa = e % TA); "column" chunks (b < a, where the
+    // packed row a holds b contiguous) walk e -> (bb = e % BK, aa = e / BK).
+    auto fetch = [&](int b0, bool colmajor) {
       const T *__restrict__ cs = Cb + int64_t(b0) * LDC;
 #pragma unroll
       for (int i = 0; i < CN; ++i) rc[i] = cs[tid + kThreads * i];
-      if (b0 >= a0 + TA) {  // every b > a: row b holds a contiguous
 #pragma unroll
-        for (int i = 0; i < LN; ++i) {
-          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
-          rl[i] = (b < n && a < n) ? Lj[tri0(b) + a] : T(0);
-          lo[i] = bb * SLD + aa;
-        }
-      } else if (b0 + BK <= a0) {  // every b < a: row a holds b contiguous
-#pragma unroll
-        for (int i = 0; i < LN; ++i) {
-          const int e = tid + kThreads * i, bb = e % BK, aa = e / BK, b = b0 + bb, a = a0 + aa;
-          rl[i] = (a < n) ? Lj[tri0(a) + b] : T(0);
-          lo[i] = bb * SLD + aa;
-        }
-      } else {  // block straddles the diagonal
-#pragma unroll
-        for (int i = 0; i < LN; ++i) {
-          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
-          T v = T(0);
-          if (b < n && a < n) v = (b >= a) ? Lj[tri0(b) + a] : Lj[tri0(a) + b];
-          rl[i] = v;
-          lo[i] = bb * SLD + aa;
-        }
+      for (int i = 0; i < LN; ++i) {
+        const int e = tid + kThreads * i;
+        const int bb = colmajor ? e % BK : e / TA, aa = colmajor ? e / BK : e % TA;
+        const int b = b0 + bb, a = a0 + aa;
+        const bool ok = (b < n) && (a < n);
+        const int hi = b >= a ? b : a, lo = b >= a ? a : b;
+        const int64_t idx = ok ? tri0(hi) + lo : 0;  // branch-free masked load
+        const T v = Lj[idx];
+        rl[i] = ok ? v : T(0);
       }
     };
-    fetch(0);
+    auto is_col = [&](int b0) { return b0 + BK <= a0; };
+    fetch(0, is_col(0));
     for (int b0 = 0; b0 < n; b0 += BK) {
+      const bool colmajor = is_col(b0);
       if (b0) __syncthreads();  // every wave is done with the previous chunk
 #pragma unroll
       for (int i = 0; i < CN; ++i) sC[tid + kThreads * i] = rc[i];
 #pragma unroll
-      for (int i = 0; i < LN; ++i) sL[lo[i]] = rl[i];
+      for (int i = 0; i < LN; ++i) {
+        const int e = tid + kThreads * i;
+        const int bb = colmajor ? e % BK : e / TA, aa = colmajor ? e / BK : e % TA;
+        sL[bb * SLD + aa] = rl[i];
+      }
       __syncthreads();
-      if (b0 + BK < n) fetch(b0 + BK);
+      if (b0 + BK < n) fetch(b0 + BK, is_col(b0 + BK));
 #pragma unroll
       for (int ks = 0; ks < BK / 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
@@ -221,7 +215,7 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 template <class P, int TM, int BKF>
 __global__ __launch_bounds__(kThreads) void k_syrk(
     const typename P::T *__restrict__ X, int n_pad, int ldc, int64_t jc, int ntiles,
-    const SyrkSets sets, double *__restrict__ kpart_all, int beta) {
+    const SyrkSets sets, double *__restrict__ kpart_all, int chunk) {
   typedef typename P::T T;
   typedef typename P::Acc Acc;
   constexpr int WT = TM / 2;
@@ -281,14 +275,13 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
 #pragma unroll
     for (int i = 0; i < PN; ++i) {
       const int e = tid + kThreads * i, ff = e / TM, cc = e % TM, f = f0 + ff;
-      T va = T(0), vb = T(0);
-      if (f < nf) {
-        const int64_t ro = ((ja + f / opad) * ldc + off + f % opad) * int64_t(n_pad);
-        va = X[ro + A * TM + cc];
-        if (!diag) vb = X[ro + B * TM + cc];
-      }
-      ra[i] = va;
-      rb[i] = vb;
+      const bool ok = f < nf;
+      const int fc = ok ? f : 0;  // branch-free masked load
+      const int64_t ro = ((ja + fc / opad) * ldc + off + fc % opad) * int64_t(n_pad);
+      const T va = X[ro + A * TM + cc];
+      const T vb = X[ro + B * TM + cc];
+      ra[i] = ok ? va : T(0);
+      rb[i] = ok ? vb : T(0);
     }
   };
   if (nf > 0) fetch(0);
@@ -324,7 +317,9 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
     }
   }
   flush();
-  double *__restrict__ kp = kpart + (int64_t(s) * ntiles + tile) * TM * TM;
+  // slot (chunk, slice): every chunk of vectors writes its own partial tiles
+  double *__restrict__ kp =
+      kpart + ((int64_t(chunk) * sets.nsplit[set] + s) * ntiles + tile) * TM * TM;
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -336,8 +331,7 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
         double v;
         if constexpr (P::kFlush) v = accd[i][j][r];
         else v = acc[i][j][r];
-        double *dst = kp + al * TM + bl;
-        *dst = beta ? *dst + v : v;
+        kp[al * TM + bl] = v;
       }
 }
 
@@ -499,7 +493,7 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
 template <int TM>
 __global__ __launch_bounds__(kThreads) void k_syrk_b3(
     const float *__restrict__ X, int n_pad, int ldc, int64_t jc, int ntiles, const SyrkSets sets,
-    double *__restrict__ kpart_all, int beta) {
+    double *__restrict__ kpart_all, int chunk) {
   constexpr int WT = TM / 2;
   constexpr int MT = WT / 16;
   constexpr int PN = kB3K * TM / kThreads;
@@ -596,7 +590,9 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
         }
     }
   }
-  double *__restrict__ kp = kpart + (int64_t(s) * ntiles + tile) * TM * TM;
+  // slot (chunk, slice): every chunk of vectors writes its own partial tiles
+  double *__restrict__ kp =
+      kpart + ((int64_t(chunk) * sets.nsplit[set] + s) * ntiles + tile) * TM * TM;
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -605,20 +601,19 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
       for (int r = 0; r < 4; ++r) {
         const int al_ = wm * WT + i * 16 + 4 * (lane >> 4) + r;
         const int bl_ = wn * WT + j * 16 + (lane & 15);
-        const double v = accd[i][j][r] + double(acc[i][j][r]);
-        double *dst = kp + al_ * TM + bl_;
-        *dst = beta ? *dst + v : v;
+        kp[al_ * TM + bl_] = accd[i][j][r] + double(acc[i][j][r]);
       }
 }
 
 // klt[d*ldk + a*(a+1)/2 + b] = sum_s kpart[kbase[d] + (s*ntiles + tile(a,b))*TM*TM
-//                                           + (a%TM)*TM + b%TM]
+//                                           + (a%TM)*TM + b%TM],  s over (chunk, slice)
 // grid = (n rows a, nsets), block 256.
 __global__ void k_reduce_k(const double *__restrict__ kpart_all, int n, int tm, int ntiles,
-                           const SyrkSets sets, double *__restrict__ klt, int64_t ldk) {
+                           const SyrkSets sets, int nchunk, double *__restrict__ klt,
+                           int64_t ldk) {
   const int a = blockIdx.x, d = blockIdx.y;
   const int A = a / tm, al = a % tm;
-  const int nsplit = sets.nsplit[d];
+  const int nsplit = sets.nsplit[d] * nchunk;  // slots (chunk, slice), chunk-major
   const double *__restrict__ kpart = kpart_all + sets.kbase[d];
   double *__restrict__ out = klt + d * ldk + tri0(a);
   for (int b = threadIdx.x; b <= a; b += blockDim.x) {
@@ -673,13 +668,13 @@ void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T
 
 template <class P>
 void syrk_launch(Context *c, int tm, int items, const typename P::T *X, int ldc, int64_t jn,
-                 int ntiles, const SyrkSets &sets, double *kpart, int beta) {
+                 int ntiles, const SyrkSets &sets, double *kpart, int chunk) {
   if (tm == 128)
     hipLaunchKernelGGL((k_syrk<P, 128, 16>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
-                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk);
   else
     hipLaunchKernelGGL((k_syrk<P, 64, 32>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
-                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk);
 }
 
 template <int OT, int TA>
@@ -708,13 +703,13 @@ void ht_dispatch_b3(Context *c, int ot, const float *L, const __bf16 *ch, const 
 }
 
 void syrk_launch_b3(Context *c, int tm, int items, const float *X, int ldc, int64_t jn, int ntiles,
-                    const SyrkSets &sets, double *kpart, int beta) {
+                    const SyrkSets &sets, double *kpart, int chunk) {
   if (tm == 128)
     hipLaunchKernelGGL((k_syrk_b3<128>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
-                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk);
   else
     hipLaunchKernelGGL((k_syrk_b3<64>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
-                       c->n_pad, ldc, jn, ntiles, sets, kpart, beta);
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk);
 }
 
 int pack_bank_b3(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
@@ -858,7 +853,7 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
     sets.off[d] = off[size_t(d)];
     sets.opad[d] = std::max(opad[size_t(d)], 1);
     sets.kbase[d] = kb;
-    kb += int64_t(sets.nsplit[d]) * ntiles * tm * tm;
+    kb += int64_t(sets.nsplit[d]) * nchunk * ntiles * tm * tm;
     sets.item0[d + 1] = sets.item0[d] + sets.nsplit[d] * ntiles;
   }
   const int items = sets.item0[nk];
@@ -876,14 +871,14 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
     phase_end(c, MJK_PHASE_HALF_TRANSFORM);
     phase_begin(c);
     if constexpr (kB3)
-      syrk_launch_b3(c, tm, items, X, ldc, jn, ntiles, sets, kpart, ch > 0 ? 1 : 0);
+      syrk_launch_b3(c, tm, items, X, ldc, jn, ntiles, sets, kpart, int(ch));
     else
-      syrk_launch<P>(c, tm, items, X, ldc, jn, ntiles, sets, kpart, ch > 0 ? 1 : 0);
+      syrk_launch<P>(c, tm, items, X, ldc, jn, ntiles, sets, kpart, int(ch));
     phase_end(c, MJK_PHASE_EXCHANGE);
   }
   phase_begin(c);
   hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n), unsigned(nk)), dim3(256), 0, c->stream, kpart, n,
-                     tm, ntiles, sets, klt, ldk);
+                     tm, ntiles, sets, int(nchunk), klt, ldk);
   phase_end(c, MJK_PHASE_REDUCE);
   return hip_check(c, hipGetLastError(), "exchange launch");
 }
