@@ -117,7 +117,35 @@ def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
     for prec in tk:
         if prec != "fp64":
             res[prec]["exchange_speedup_vs_fp64"] = round(tk["fp64"] / tk[prec], 3)
+    if "fp32" in tk and "bf16x3" in tk:
+        res["ladder"] = ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, tk["fp64"], res, args.steps)
     return res
+
+
+def ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, t64, res, steps):
+    """Precision ladder: kappa per mode from the full-precision-mode energy
+    errors just measured, split points planned for |dE| <= 1e-6 Eh, then the
+    ladder build timed and its energy checked against FP64."""
+    import torch
+
+    from openmolcas_amd import ladder
+
+    w = ctx.vector_norms()
+    total = float(torch.sum(w).item())
+    k32 = max(res["fp32"]["dE"], 1e-300) / total
+    kb3 = max(res["bf16x3"]["dE"], 1e-300) / total
+    plan = ladder.plan(w, k32, kb3, tol=1e-6)
+    K = ctx.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        K = ctx.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    e = energy_of(ctx, dlt, h, J, K)
+    return {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "nchol": int(w.numel()),
+            "predicted_dE": plan.predicted, "dE": abs(e - e64),
+            "exchange_ms": round(1e3 * t, 4), "exchange_speedup_vs_fp64": round(t64 / t, 3)}
 
 
 def profiled_traffic(kernel_prefix, n, M, o, precision):
@@ -194,6 +222,18 @@ def extra_workloads(dev):
             dC_mixed = jk.grad_orbitals(C, F, 4.0)
             dC64 = jk.grad_orbitals(C, G, 4.0)
             res[f"dEdC_maxabs_err_{prec}"] = float(torch.max(torch.abs(dC_mixed - dC64)).item())
+    # precision ladder planned from the measured full-mode errors (|dE| <= 1e-6 Eh)
+    from openmolcas_amd import ladder
+    wv = jk.vector_norms()
+    tot = float(torch.sum(wv).item())
+    plan = ladder.plan(wv, max(res["dE_fp32"], 1e-300) / tot, max(res["dE_bf16x3"], 1e-300) / tot)
+    t = _timeit(lambda: jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16), 2)
+    K = jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
+    Gp = torch.zeros_like(J)
+    jk.fock_update(Gp, J, [0], [1.0], K, [-1.0])
+    res["ladder"] = {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "exchange_ms": round(1e3 * t, 3),
+                     "dE": abs(jk.energy(h, dlt, Gp).item() - e64),
+                     "speedup_vs_fp64": round(res["exchange_fp64_ms"] / (1e3 * t), 3)}
     dl = torch.empty((M, npair(n)), dtype=torch.float64, device=dev)
     res["backward_dC_ms"] = round(1e3 * _timeit(lambda: jk.grad_orbitals(C, G, 4.0), 3), 3)
     res["backward_dL_ms"] = round(1e3 * _timeit(

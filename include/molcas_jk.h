@@ -161,6 +161,22 @@ int mjk_exchange(mjk_context *ctx, int32_t nk, const int32_t *nocc,
                  const double *const *corb, double *klt, int64_t ldk,
                  int32_t precision);
 
+/* Exchange as mjk_exchange with a precision ladder over the vector index:
+ * J in [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32 and [j_bf16, nchol) in
+ * split-BF16 (three products); every partial sum is reduced in FP64.  With
+ * vectors ordered by decreasing norm (the Cholesky pivot order) the low
+ * precisions only see the small tail.  0 <= j_fp32 <= j_bf16 <= nchol.
+ * No single counterpart in the reference: it replaces the same
+ * Cho_X_getVtra + DGEMM_Tri pair (src/fock_util/cho_fscf.F90:250,276). */
+int mjk_exchange_mixed(mjk_context *ctx, int32_t nk, const int32_t *nocc,
+                       const double *const *corb, double *klt, int64_t ldk,
+                       int64_t j_fp32, int64_t j_bf16);
+
+/* w_dev[J] = ||L^J||_F^2 of the full symmetric matrix (device, nchol
+ * doubles): the weights a caller uses to place the ladder split points.
+ * No counterpart in the reference. */
+int mjk_vector_norms(mjk_context *ctx, double *w_dev);
+
 /* flt(d) += fj[d] * jlt(jsrc[d]) + fk[d] * klt(d), d < nd (jsrc[d] < 0: no
  * J term).  The LT update of src/fock_util/cho_fscf.F90:276,301. */
 int mjk_fock_update(mjk_context *ctx, int32_t nd, double *flt, int64_t ldf,

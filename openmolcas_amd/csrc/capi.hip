@@ -369,6 +369,20 @@ int mjk_exchange(mjk_context *c, int32_t nk, const int32_t *nocc, const double *
   return exchange(c, nk, nocc, corb, klt, ldk, precision);
 }
 
+int mjk_exchange_mixed(mjk_context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                       double *klt, int64_t ldk, int64_t j_fp32, int64_t j_bf16) {
+  if (!c) return MJK_ERR_ARG;
+  if (!nocc || !corb || !klt) return set_error(c, MJK_ERR_ARG, "exchange: null pointer");
+  for (int d = 0; d < nk; ++d)
+    if (nocc[d] > 0 && !corb[d]) return set_error(c, MJK_ERR_ARG, "exchange: null orbital matrix");
+  return exchange_mixed(c, nk, nocc, corb, klt, ldk, j_fp32, j_bf16);
+}
+
+int mjk_vector_norms(mjk_context *c, double *w_dev) {
+  if (!c) return MJK_ERR_ARG;
+  return vector_norms(c, w_dev);
+}
+
 int mjk_fock_update(mjk_context *c, int32_t nd, double *flt, int64_t ldf, const double *jlt,
                     int64_t ldj, const int32_t *jsrc, const double *fj, const double *klt,
                     int64_t ldk, const double *fk) {

@@ -149,6 +149,140 @@ __global__ void k_pack_dens(const double *__restrict__ dlt, int64_t ldd, int64_t
   out[d * ldl + p] = (p < np) ? dlt[d * ldd + p] : 0.0;
 }
 
+
+// ---- many densities (3..16): both passes as FP64-MFMA GEMMs ---------------
+// With nd densities each vector element is used nd times; the per-vector
+// kernels above would re-read every density for every vector.  Here a
+// workgroup stages a tile of L and a tile of the densities (or of V) in LDS and
+// contracts them with v_mfma_f64_16x16x4f64 (A[l%16][l/16], B[l/16][l%16],
+// result row 4*r + l/16, column l%16), so each density element is read once
+// per 64 vectors and each L element once per pass: still HBM-bound, at the
+// single-density byte count.
+constexpr int kVM = 64;   // vectors per workgroup (pass 1)
+constexpr int kVK = 32;   // pairs per LDS chunk (pass 1)
+constexpr int kJP = 256;  // pairs per workgroup (pass 2)
+constexpr int kJK = 16;   // vectors per LDS chunk (pass 2)
+
+__device__ inline v4d mfma64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// grid = (ceil(M/64), nsplit); vpart[(s*M + J)*nd + d] = sum_{p in split s} L[J][p] D[d][p]
+__global__ __launch_bounds__(kThreads) void k_coulomb_v_mfma(
+    const double *__restrict__ L, int64_t ldl, int64_t M, const double *__restrict__ D, int nd,
+    int64_t pchunk, double *__restrict__ vpart) {
+  __shared__ double sL[kVM][kVK + 1];
+  __shared__ double sD[16][kVK + 1];
+  const int64_t J0 = int64_t(blockIdx.x) * kVM;
+  const int64_t p0 = int64_t(blockIdx.y) * pchunk;
+  const int64_t p1 = min(p0 + pchunk, ldl);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  constexpr int NL = kVM * kVK / kThreads, NDD = 16 * kVK / kThreads;
+  double rl[NL], rd[NDD];
+  auto fetch = [&](int64_t pc) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = tid + kThreads * i, r = e / kVK, k = e % kVK;
+      const bool ok = (J0 + r < M);
+      const double v = L[(ok ? J0 + r : 0) * ldl + pc + k];
+      rl[i] = ok ? v : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NDD; ++i) {
+      const int e = tid + kThreads * i, d = e / kVK, k = e % kVK;
+      const double v = D[(d < nd ? d : 0) * ldl + pc + k];
+      rd[i] = d < nd ? v : 0.0;
+    }
+  };
+  if (p0 < p1) fetch(p0);
+  for (int64_t pc = p0; pc < p1; pc += kVK) {
+    if (pc != p0) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = tid + kThreads * i;
+      sL[e / kVK][e % kVK] = rl[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NDD; ++i) {
+      const int e = tid + kThreads * i;
+      sD[e / kVK][e % kVK] = rd[i];
+    }
+    __syncthreads();
+    if (pc + kVK < p1) fetch(pc + kVK);
+#pragma unroll
+    for (int ks = 0; ks < kVK / 4; ++ks) {
+      const int k = ks * 4 + (lane >> 4);
+      acc = mfma64(sL[16 * w + (lane & 15)][k], sD[lane & 15][k], acc);
+    }
+  }
+  const int d = lane & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t J = J0 + 16 * w + 4 * r + (lane >> 4);
+    if (J < M && d < nd) vpart[(int64_t(blockIdx.y) * M + J) * nd + d] = acc[r];
+  }
+}
+
+// grid = (ldl/256, nsplit); jpart[(s*nd + d)*ldl + p] = sum_{J in split s} V[J][d] L[J][p]
+__global__ __launch_bounds__(kThreads) void k_coulomb_j_mfma(
+    const double *__restrict__ L, int64_t ldl, int64_t M, const double *__restrict__ V, int nd,
+    int64_t jchunk, double *__restrict__ jpart) {
+  __shared__ double sL[kJK][kJP];
+  __shared__ double sV[kJK][16];
+  const int64_t p0 = int64_t(blockIdx.x) * kJP;
+  const int64_t ja = int64_t(blockIdx.y) * jchunk;
+  const int64_t jz = min(ja + jchunk, M);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  v4d acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  constexpr int NL = kJK * kJP / kThreads;
+  double rl[NL], rv;
+  auto fetch = [&](int64_t jc) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = tid + kThreads * i, r = e / kJP, k = e % kJP;
+      const bool ok = jc + r < jz;
+      const double v = L[(ok ? jc + r : 0) * ldl + p0 + k];
+      rl[i] = ok ? v : 0.0;
+    }
+    const int r = tid / 16, d = tid % 16;  // 256 threads = 16 x 16
+    const bool ok = (jc + r < jz) && d < nd;
+    const double v = V[(ok ? (jc + r) * nd + d : 0)];
+    rv = ok ? v : 0.0;
+  };
+  if (ja < jz) fetch(ja);
+  for (int64_t jc = ja; jc < jz; jc += kJK) {
+    if (jc != ja) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = tid + kThreads * i;
+      sL[e / kJP][e % kJP] = rl[i];
+    }
+    sV[tid / 16][tid % 16] = rv;
+    __syncthreads();
+    if (jc + kJK < jz) fetch(jc + kJK);
+#pragma unroll
+    for (int ks = 0; ks < kJK / 4; ++ks) {
+      const int k = ks * 4 + (lane >> 4);
+      const double b = sV[k][lane & 15];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = mfma64(sL[k][64 * w + 16 * t + (lane & 15)], b, acc[t]);
+    }
+  }
+  const int d = lane & 15;
+  if (d < nd) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t p = p0 + 64 * w + 16 * t + 4 * r + (lane >> 4);
+        jpart[(int64_t(blockIdx.y) * nd + d) * ldl + p] = acc[t][r];
+      }
+  }
+}
+
 template <int JB, int ND>
 void launch_v(Context *c, const double *D, int nd, double *vpart, int nsplit, int64_t pchunk) {
   dim3 grid(unsigned(ceil_div(c->nchol, JB)), unsigned(nsplit));
@@ -163,7 +297,33 @@ void launch_j(Context *c, const double *V, int nd, double *jpart, int nsplit, in
                      c->nchol, V, nd, jchunk, jpart);
 }
 
+// w[J] = ||L^J||_F^2 of the full symmetric matrix = 2 sum_p L_p^2 - sum_a L_aa^2.
+// grid = M, one workgroup per vector.
+__global__ __launch_bounds__(kThreads) void k_vector_norms(const double *__restrict__ L,
+                                                           int64_t ldl, int64_t np, int n,
+                                                           double *__restrict__ w) {
+  __shared__ double lds[4];
+  const double *__restrict__ row = L + int64_t(blockIdx.x) * ldl;
+  double s[1] = {0.0};
+  for (int64_t p = threadIdx.x; p < np; p += kThreads) s[0] += 2.0 * row[p] * row[p];
+  for (int a = threadIdx.x; a < n; a += kThreads) {
+    const double x = row[int64_t(a) * (a + 1) / 2 + a];
+    s[0] -= x * x;
+  }
+  block_sum<1>(s, lds);
+  if (threadIdx.x == 0) w[blockIdx.x] = s[0];
+}
+
 }  // namespace
+
+int vector_norms(Context *c, double *w) {
+  if (!w) return set_error(c, MJK_ERR_ARG, "vector_norms: null pointer");
+  if (c->nloaded != c->nchol)
+    return set_error(c, MJK_ERR_SHORT_READ, "vector_norms: not all Cholesky vectors were uploaded");
+  hipLaunchKernelGGL(k_vector_norms, dim3(unsigned(c->nchol)), dim3(kThreads), 0, c->stream, c->L,
+                     c->ldl, c->np, c->n, w);
+  return hip_check(c, hipGetLastError(), "vector_norms launch");
+}
 
 int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
             double *v_out) {
@@ -182,11 +342,48 @@ int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
                      c->stream, dlt, ldd, c->np, ldl, Dp);
   phase_end(c, MJK_PHASE_PACK);
 
+  const int64_t nseg = ldl / kSeg;
+  if (nd >= 3) {
+    // ---- pass 1 (MFMA): V = L D^T, split over pairs to fill the device
+    const int64_t nblk = ceil_div(M, kVM);
+    const int64_t nk = ldl / kVK;
+    int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nblk), 1), nk));
+    const int64_t pchunk = ceil_div(nk, nsplit1) * kVK;
+    nsplit1 = int(ceil_div(ldl, pchunk));
+    if ((rc = ensure(c, c->w->vpart, sizeof(double) * nsplit1 * M * nd))) return rc;
+    if ((rc = ensure(c, c->w->v, sizeof(double) * M * nd))) return rc;
+    double *vpart = static_cast<double *>(c->w->vpart.ptr);
+    double *V = v_out ? v_out : static_cast<double *>(c->w->v.ptr);
+    phase_begin(c);
+    hipLaunchKernelGGL(k_coulomb_v_mfma, dim3(unsigned(nblk), unsigned(nsplit1)), dim3(kThreads), 0,
+                       c->stream, c->L, ldl, M, Dp, nd, pchunk, vpart);
+    phase_end(c, MJK_PHASE_COULOMB_V);
+    phase_begin(c);
+    hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M * nd, 256))), dim3(256), 0, c->stream,
+                       vpart, M, nd, nsplit1, V);
+    phase_end(c, MJK_PHASE_REDUCE);
+    // ---- pass 2 (MFMA): J = L^T V, split over vectors
+    const int64_t npc = ldl / kJP;
+    int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, npc), 1), ceil_div(M, kJK)));
+    const int64_t jchunk = round_up(ceil_div(M, nsplit2), kJK);
+    nsplit2 = int(ceil_div(M, jchunk));
+    if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
+    double *jpart = static_cast<double *>(c->w->jpart.ptr);
+    phase_begin(c);
+    hipLaunchKernelGGL(k_coulomb_j_mfma, dim3(unsigned(npc), unsigned(nsplit2)), dim3(kThreads), 0,
+                       c->stream, c->L, ldl, M, V, nd, jchunk, jpart);
+    phase_end(c, MJK_PHASE_COULOMB_J);
+    phase_begin(c);
+    hipLaunchKernelGGL(k_reduce_j, dim3(unsigned(ceil_div(c->np, 256)), unsigned(nd)), dim3(256), 0,
+                       c->stream, jpart, ldl, c->np, nd, nsplit2, jlt, ldj);
+    phase_end(c, MJK_PHASE_REDUCE);
+    return hip_check(c, hipGetLastError(), "coulomb launch");
+  }
+
   // ---- pass 1: V = L^T D
   const int ndt = nd <= 1 ? 1 : nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : 16;
   const int jb = ndt <= 2 ? 8 : ndt == 4 ? 4 : ndt == 8 ? 2 : 1;
   const int64_t nblk = ceil_div(M, jb);
-  const int64_t nseg = ldl / kSeg;
   int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nblk), 1), nseg));
   const int64_t pchunk = ceil_div(nseg, nsplit1) * kSeg;
   nsplit1 = int(ceil_div(ldl, pchunk));

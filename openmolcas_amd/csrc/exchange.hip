@@ -713,12 +713,12 @@ void syrk_launch_b3(Context *c, int tm, int items, const float *X, int ldc, int6
 }
 
 int pack_bank_b3(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
-                 const std::vector<int> &off, int ldc) {
+                 const std::vector<int> &off, int ldc, DevBuf &buf) {
   const int n = c->n, n_pad = c->n_pad;
   const size_t half = sizeof(__bf16) * size_t(ldc) * n_pad;
   int rc;
-  if ((rc = ensure(c, c->w->cbank, 2 * half))) return rc;
-  __bf16 *ch = static_cast<__bf16 *>(c->w->cbank.ptr);
+  if ((rc = ensure(c, buf, 2 * half))) return rc;
+  __bf16 *ch = static_cast<__bf16 *>(buf.ptr);
   __bf16 *cl = ch + size_t(ldc) * n_pad;
   phase_begin(c);
   if ((rc = fill_zero(c, ch, 2 * half))) return rc;
@@ -759,12 +759,12 @@ int bank_layout(Context *c, int32_t nk, const int32_t *nocc, std::vector<int> &o
 
 template <typename T>
 int pack_bank_t(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
-                const std::vector<int> &off, int ldc) {
+                const std::vector<int> &off, int ldc, DevBuf &buf) {
   const int n = c->n, n_pad = c->n_pad;
   const size_t bytes = sizeof(T) * size_t(n_pad) * ldc;
   int rc;
-  if ((rc = ensure(c, c->w->cbank, bytes))) return rc;
-  T *cb = static_cast<T *>(c->w->cbank.ptr);
+  if ((rc = ensure(c, buf, bytes))) return rc;
+  T *cb = static_cast<T *>(buf.ptr);
   phase_begin(c);
   if ((rc = fill_zero(c, cb, bytes))) return rc;
   for (int d = 0; d < nk; ++d) {
@@ -791,12 +791,15 @@ int ensure_l32(Context *c) {
   return hip_check(c, hipGetLastError(), "L to FP32");
 }
 
-template <class P>
-int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
-               int64_t ldk) {
-  typedef typename P::T T;
+// A run of vectors [j0, j1) computed in one precision.
+struct Segment {
+  int64_t j0, j1;
+  int32_t prec;
+};
+
+int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                      double *klt, int64_t ldk, const std::vector<Segment> &segs) {
   const int n = c->n, n_pad = c->n_pad;
-  const int64_t M = c->nchol;
   int rc;
   std::vector<int> off, opad;
   int ot = 0, tot = 0;
@@ -806,49 +809,49 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
       if ((rc = fill_zero(c, klt + d * ldk, sizeof(double) * c->np))) return rc;
     return MJK_OK;
   }
+  if (nk > kMaxSets) return set_error(c, MJK_ERR_ARG, "exchange: more than 16 orbital sets");
   const int ldc = 16 * ot;
-  constexpr bool kB3 = std::is_same<P, B3>::value;
-  if constexpr (kB3) {
-    if ((rc = pack_bank_b3(c, nk, nocc, corb, off, ldc))) return rc;
-  } else {
-    if ((rc = pack_bank_t<T>(c, nk, nocc, corb, off, ldc))) return rc;
-  }
-  const T *cb = static_cast<const T *>(c->w->cbank.ptr);
-  const __bf16 *cbh = static_cast<const __bf16 *>(c->w->cbank.ptr);
-  const __bf16 *cbl = cbh + size_t(ldc) * n_pad;
-  const T *L;
-  if constexpr (sizeof(T) == sizeof(double)) {
-    L = c->L;
-  } else {
-    if ((rc = ensure_l32(c))) return rc;
-    L = static_cast<const float *>(c->ws_l32.ptr);
-  }
+  bool use[3] = {false, false, false};
+  for (const Segment &s : segs)
+    if (s.j1 > s.j0) use[s.prec] = true;
+  // one bank per precision in use; the reduced-precision ones read the FP32 copy of L
+  if (use[MJK_PREC_FP64] && (rc = pack_bank_t<double>(c, nk, nocc, corb, off, ldc, c->w->cbank)))
+    return rc;
+  if (use[MJK_PREC_FP32] && (rc = pack_bank_t<float>(c, nk, nocc, corb, off, ldc, c->w->cb32)))
+    return rc;
+  if (use[MJK_PREC_BF16X3] && (rc = pack_bank_b3(c, nk, nocc, corb, off, ldc, c->w->cb3)))
+    return rc;
+  if ((use[MJK_PREC_FP32] || use[MJK_PREC_BF16X3]) && (rc = ensure_l32(c))) return rc;
+  const float *L32 = static_cast<const float *>(c->ws_l32.ptr);
 
-  // ---- chunking over vectors (X budget)
-  const int64_t xvec_bytes = int64_t(sizeof(T)) * ldc * n_pad;
-  int64_t jc = std::max<int64_t>(1, std::min<int64_t>(M, c->x_budget / xvec_bytes));
-  const int64_t nchunk = ceil_div(M, jc);
-  jc = ceil_div(M, nchunk);
+  // ---- chunking over vectors (X budget, sized for the widest element in use)
+  const int64_t elem = use[MJK_PREC_FP64] ? 8 : 4;
+  const int64_t xvec_bytes = elem * ldc * n_pad;
+  int64_t jc = std::max<int64_t>(1, std::min<int64_t>(c->nchol, c->x_budget / xvec_bytes));
+  int64_t nchunk = 0;
+  for (const Segment &s : segs)
+    if (s.j1 > s.j0) nchunk += ceil_div(s.j1 - s.j0, jc);
   if (jc * ldc >= (int64_t(1) << 31))
     return set_error(c, MJK_ERR_ARG, "exchange: chunk contraction length exceeds 2^31");
   if ((rc = ensure(c, c->w->x, size_t(xvec_bytes) * jc))) return rc;
-  T *X = static_cast<T *>(c->w->x.ptr);
 
-  // ---- SYRK tiling and slices (one launch covers every density)
-  if (nk > kMaxSets) return set_error(c, MJK_ERR_ARG, "exchange: more than 16 orbital sets");
+  // ---- SYRK tiling and slices (one launch per chunk covers every density)
   int tm = c->tile_syrk ? c->tile_syrk : (n_pad >= 512 && n_pad % 128 == 0 ? 128 : 64);
   if (n_pad % tm) tm = 64;
   const int ntile1 = n_pad / tm;
   const int ntiles = ntile1 * (ntile1 + 1) / 2;
   const int target = c->split_target ? c->split_target : 1024;
-  int active = 0;
-  for (int d = 0; d < nk; ++d) active += nocc[d] > 0;
   SyrkSets sets{};
   sets.n = nk;
   int64_t kb = 0;
   sets.item0[0] = 0;
+  // slices per density in proportion to its contraction length (opad rows per vector)
+  int64_t work = 0;
+  for (int d = 0; d < nk; ++d)
+    if (nocc[d] > 0) work += opad[size_t(d)];
   for (int d = 0; d < nk; ++d) {
-    const int64_t want = ceil_div(target, int64_t(ntiles) * std::max(active, 1));
+    const int64_t want = ceil_div(int64_t(target) * opad[size_t(d)],
+                                  int64_t(ntiles) * std::max<int64_t>(work, 1));
     sets.nsplit[d] = nocc[d] > 0 ? int(std::max<int64_t>(1, std::min<int64_t>(want, jc))) : 0;
     sets.off[d] = off[size_t(d)];
     sets.opad[d] = std::max(opad[size_t(d)], 1);
@@ -860,21 +863,43 @@ int exchange_t(Context *c, int32_t nk, const int32_t *nocc, const double *const 
   if ((rc = ensure(c, c->w->kpart, sizeof(double) * size_t(std::max<int64_t>(kb, 1))))) return rc;
   double *kpart = static_cast<double *>(c->w->kpart.ptr);
 
-  for (int64_t ch = 0; ch < nchunk; ++ch) {
-    const int64_t j0 = ch * jc;
-    const int64_t jn = std::min(jc, M - j0);
-    phase_begin(c);
-    if constexpr (kB3)
-      ht_dispatch_b3(c, ot, L, cbh, cbl, j0, jn, X);
-    else
-      ht_dispatch<P>(c, ot, L, cb, j0, jn, X);
-    phase_end(c, MJK_PHASE_HALF_TRANSFORM);
-    phase_begin(c);
-    if constexpr (kB3)
-      syrk_launch_b3(c, tm, items, X, ldc, jn, ntiles, sets, kpart, int(ch));
-    else
-      syrk_launch<P>(c, tm, items, X, ldc, jn, ntiles, sets, kpart, int(ch));
-    phase_end(c, MJK_PHASE_EXCHANGE);
+  int64_t slot = 0;  // running chunk index over all segments
+  for (const Segment &s : segs) {
+    for (int64_t j0 = s.j0; j0 < s.j1; j0 += jc, ++slot) {
+      const int64_t jn = std::min(jc, s.j1 - j0);
+      phase_begin(c);
+      switch (s.prec) {
+        case MJK_PREC_FP64:
+          ht_dispatch<F64>(c, ot, c->L, static_cast<const double *>(c->w->cbank.ptr), j0, jn,
+                           static_cast<double *>(c->w->x.ptr));
+          break;
+        case MJK_PREC_FP32:
+          ht_dispatch<F32>(c, ot, L32, static_cast<const float *>(c->w->cb32.ptr), j0, jn,
+                           static_cast<float *>(c->w->x.ptr));
+          break;
+        default: {
+          const __bf16 *cbh = static_cast<const __bf16 *>(c->w->cb3.ptr);
+          ht_dispatch_b3(c, ot, L32, cbh, cbh + size_t(ldc) * n_pad, j0, jn,
+                         static_cast<float *>(c->w->x.ptr));
+        }
+      }
+      phase_end(c, MJK_PHASE_HALF_TRANSFORM);
+      phase_begin(c);
+      switch (s.prec) {
+        case MJK_PREC_FP64:
+          syrk_launch<F64>(c, tm, items, static_cast<const double *>(c->w->x.ptr), ldc, jn, ntiles,
+                           sets, kpart, int(slot));
+          break;
+        case MJK_PREC_FP32:
+          syrk_launch<F32>(c, tm, items, static_cast<const float *>(c->w->x.ptr), ldc, jn, ntiles,
+                           sets, kpart, int(slot));
+          break;
+        default:
+          syrk_launch_b3(c, tm, items, static_cast<const float *>(c->w->x.ptr), ldc, jn, ntiles,
+                         sets, kpart, int(slot));
+      }
+      phase_end(c, MJK_PHASE_EXCHANGE);
+    }
   }
   phase_begin(c);
   hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n), unsigned(nk)), dim3(256), 0, c->stream, kpart, n,
@@ -889,7 +914,7 @@ int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *
               std::vector<int> &off, std::vector<int> &opad, int *ot, int *tot) {
   int rc = bank_layout(c, nk, nocc, off, opad, ot, tot);
   if (rc || *tot == 0) return rc;
-  return pack_bank_t<double>(c, nk, nocc, corb, off, 16 * *ot);
+  return pack_bank_t<double>(c, nk, nocc, corb, off, 16 * *ot, c->w->cbank);
 }
 
 void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, double *X) {
@@ -898,16 +923,26 @@ void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, dou
 
 int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
              int64_t ldk, int32_t precision) {
+  if (precision < MJK_PREC_FP64 || precision > MJK_PREC_BF16X3)
+    return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
+  const int64_t M = c->nchol;
+  return exchange_mixed(c, nk, nocc, corb, klt, ldk, precision == MJK_PREC_FP64 ? M : 0,
+                        precision == MJK_PREC_BF16X3 ? 0 : M);
+}
+
+int exchange_mixed(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                   double *klt, int64_t ldk, int64_t j_fp32, int64_t j_bf16) {
   if (nk < 1) return set_error(c, MJK_ERR_ARG, "exchange: nk < 1");
   if (ldk < c->np) return set_error(c, MJK_ERR_ARG, "exchange: ldk < nbas(nbas+1)/2");
   if (c->nloaded != c->nchol)
     return set_error(c, MJK_ERR_SHORT_READ, "exchange: not all Cholesky vectors were uploaded");
-  switch (precision) {
-    case MJK_PREC_FP64: return exchange_t<F64>(c, nk, nocc, corb, klt, ldk);
-    case MJK_PREC_FP32: return exchange_t<F32>(c, nk, nocc, corb, klt, ldk);
-    case MJK_PREC_BF16X3: return exchange_t<B3>(c, nk, nocc, corb, klt, ldk);
-    default: return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
-  }
+  const int64_t M = c->nchol;
+  if (j_fp32 < 0 || j_fp32 > j_bf16 || j_bf16 > M)
+    return set_error(c, MJK_ERR_ARG, "exchange: need 0 <= j_fp32 <= j_bf16 <= nchol");
+  std::vector<Segment> segs = {{0, j_fp32, MJK_PREC_FP64},
+                               {j_fp32, j_bf16, MJK_PREC_FP32},
+                               {j_bf16, M, MJK_PREC_BF16X3}};
+  return exchange_segments(c, nk, nocc, corb, klt, ldk, segs);
 }
 
 }  // namespace mjk

@@ -33,9 +33,10 @@ struct DevBuf {
 // Device workspaces.  Contexts created with mjk_create_shared use the same
 // Workspace and must queue their work on the same stream.
 struct Workspace {
-  DevBuf d, vpart, v, jpart, cbank, x, kpart, stage, j, k, red, misc, sets;
+  DevBuf d, vpart, v, jpart, cbank, cb32, cb3, x, kpart, stage, j, k, red, misc, sets;
   ~Workspace() {
-    for (DevBuf *b : {&d, &vpart, &v, &jpart, &cbank, &x, &kpart, &stage, &j, &k, &red, &misc, &sets})
+    for (DevBuf *b : {&d, &vpart, &v, &jpart, &cbank, &cb32, &cb3, &x, &kpart, &stage, &j, &k, &red,
+                      &misc, &sets})
       if (b->ptr) (void)hipFree(b->ptr);
   }
 };
@@ -122,6 +123,14 @@ int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *
 // Half-transform of vectors j0..j0+jn-1 against the bank (16*ot columns):
 // X[(jr*16*ot + k)*n_pad + a] = sum_b Cb[b][k] L^{j0+jr}_{ab}.
 void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, double *X);
+
+// w[J] = squared Frobenius norm of the symmetric matrix L^J (device, M doubles).
+int vector_norms(Context *c, double *w);
+
+// Exchange with vectors [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32 and
+// [j_bf16, nchol) in split-BF16; all partial sums reduced in FP64.
+int exchange_mixed(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                   double *klt, int64_t ldk, int64_t j_fp32, int64_t j_bf16);
 
 // Fock assembly: flt(d) += fj[d]*jlt(jsrc[d]) + fk[d]*klt(d).
 int fock_update(Context *c, int32_t nd, double *flt, int64_t ldf, const double *jlt,

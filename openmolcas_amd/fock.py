@@ -209,6 +209,24 @@ class JKContext:
                                           precision_code(precision)))
         return k
 
+    def exchange_mixed(self, corbs, j_fp32: int, j_bf16: int) -> torch.Tensor:
+        """Exchange with vectors [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32
+        and [j_bf16, nchol) in split-BF16, reduced in FP64."""
+        cs = [_dev_f64(c, self.device) for c in corbs]
+        nk = len(cs)
+        nocc = np.array([c.shape[1] if c.dim() == 2 else 0 for c in cs], dtype=np.int32)
+        ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
+        k = torch.empty((nk, self.npair), dtype=torch.float64, device=self.device)
+        self._check(self.lib.mjk_exchange_mixed(self._h, nk, _ptr(nocc), ptrs, _ptr(k), _ld(k),
+                                                int(j_fp32), int(j_bf16)))
+        return k
+
+    def vector_norms(self) -> torch.Tensor:
+        """(nchol,) squared Frobenius norms of the symmetric vectors L^J."""
+        w = torch.empty(self.nchol, dtype=torch.float64, device=self.device)
+        self._check(self.lib.mjk_vector_norms(self._h, _ptr(w)))
+        return w
+
     def fock_update(self, flt: torch.Tensor, jlt=None, jsrc=None, fj=None, klt=None, fk=None):
         """flt[d] += fj[d] * jlt[jsrc[d]] + fk[d] * klt[d] (in place)."""
         nd = flt.shape[0]

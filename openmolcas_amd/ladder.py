@@ -1,0 +1,82 @@
+"""Precision ladder for the exchange build.
+
+The exchange energy is a sum over Cholesky vectors, E_K = -f sum_J
+||C^T L^J C||_F^2, and the contribution of vector J is bounded by its weight
+w_J = ||L^J||_F^2 (for orthonormal C).  Cholesky vectors come in pivot order,
+so the weights decay and a low precision applied to the tail [j, nchol) costs
+an energy error that scales with the tail sum T(j) = sum_{J >= j} w_J:
+
+    |dE_mode(j)| ~= kappa_mode * T(j)
+
+kappa_mode is measured once per system (``calibrate``: one exchange build in
+each precision against FP64); ``plan`` then places the split points so each
+reduced-precision segment stays below a share of the energy tolerance, and
+``mjk_exchange_mixed`` runs [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32 and
+[j_bf16, nchol) in split-BF16.
+
+The reference has no mixed-precision exchange: src/fock_util/cho_fscf.F90:
+250,276 runs every vector in FP64.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+
+@dataclass
+class Ladder:
+    j_fp32: int
+    j_bf16: int
+    predicted: float  # predicted |dE| of the two reduced-precision segments
+
+
+def tail_sums(w) -> np.ndarray:
+    """T[j] = sum_{J >= j} w_J, length nchol + 1 (T[nchol] = 0)."""
+    w = np.asarray(w.detach().cpu() if isinstance(w, torch.Tensor) else w, dtype=np.float64)
+    t = np.zeros(w.size + 1)
+    t[:-1] = np.cumsum(w[::-1])[::-1]
+    return t
+
+
+def plan(w, kappa_fp32: float, kappa_bf16: float, tol: float = 1e-6,
+         share: float = 0.25) -> Ladder:
+    """Smallest split points whose predicted energy error per reduced segment
+    is at most share * tol (FP32 segment [j_fp32, j_bf16), split-BF16 segment
+    [j_bf16, nchol))."""
+    t = tail_sums(w)
+    m = t.size - 1
+    budget = share * tol
+    jb = m
+    while jb > 0 and kappa_bf16 * t[jb - 1] <= budget:
+        jb -= 1
+    j3 = jb
+    while j3 > 0 and kappa_fp32 * (t[j3 - 1] - t[jb]) <= budget:
+        j3 -= 1
+    pred = kappa_fp32 * (t[j3] - t[jb]) + kappa_bf16 * t[jb]
+    return Ladder(int(j3), int(jb), float(pred))
+
+
+def exchange_energy_error(k_test: torch.Tensor, k_ref: torch.Tensor, dlt: torch.Tensor,
+                          factor: float = 0.5) -> float:
+    """|factor * sum_d sum_LT (K_test - K_ref)_d * dlt_d| for folded densities."""
+    return abs(factor * torch.sum((k_test - k_ref) * dlt).item())
+
+
+def calibrate(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, w=None):
+    """Measure kappa_fp32 and kappa_bf16 for this system: one exchange build
+    per precision against FP64.  dlt holds the folded densities that weight
+    K in the energy (one row per orbital set) and factor the exchange energy
+    factor (1/2 for the closed-shell E = h.D + 1/2 (J - K).D with D = 2 C C^T).
+    Returns (kappa_fp32, kappa_bf16, w, k_fp64)."""
+    if w is None:
+        w = jk.vector_norms()
+    total = float(torch.sum(w).item())
+    k64 = jk.exchange(corbs, precision="fp64")
+    kappas = []
+    for prec in ("fp32", "bf16x3"):
+        k = jk.exchange(corbs, precision=prec)
+        de = exchange_energy_error(k, k64, dlt, factor)
+        kappas.append(max(de, 1e-300) / max(total, 1e-300))
+    return kappas[0], kappas[1], w, k64
