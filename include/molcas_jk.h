@@ -130,6 +130,23 @@ int mjk_cho_fscf(mjk_context *ctx, int32_t nden, double *flt, const int32_t *nfo
                  const int32_t *niorb, const double *porb, const double *dlt,
                  double exfac, int32_t precision);
 
+/* CHO_LK_SCF (src/fock_util/cho_lk_scf.F90:14), nSym = 1:
+ *   FLT(d) += J[PLT(1)] - FactXI * sum_{J,k} L(a,J)[k] L(b,J)[k],  d < nden,
+ * nden in {1,2}; porb holds nden square matrices A2(nbas,nbas) back to back,
+ * AO index fastest, columns 0..nforb[d]+niorb[d]-1 used (the MSQ layout of
+ * src/scf/choscf_drv.F90:208).  klt (nden blocks of nbas(nbas+1)/2, or NULL)
+ * is accumulated with -FactXI*K in the reference's shell-pair-blocked
+ * storage (cho_lk_scf.F90:274-292) for the nshell shells of sizes nbassh
+ * (nshell = 0: one shell, i.e. plain LT order).  Every shell pair and orbital
+ * is kept (no LK screening), which is the reference's result in the limit
+ * dmpk -> 0; nscreen, dmpk >= 0 and dfmat are accepted for the signature.
+ * Called from src/scf/choscf_drv.F90:208,433. */
+int mjk_cho_lk_scf(mjk_context *ctx, int32_t nden, double *flt, double *klt,
+                   const int32_t *nforb, const int32_t *niorb, const double *porb,
+                   const double *plt, double factxi, int32_t nscreen, double dmpk,
+                   double dfmat, int32_t nshell, const int32_t *nbassh,
+                   int32_t precision);
+
 /* CHO_FMCSCF (src/fock_util/cho_fmcscf.F90:14), nSym = 1:
  *   FI += J[DI] + FactXI * sum X_I X_I^T      (porb1: nforb+niorb rows)
  *   FA += J[DA] - 1/2 * sum X_A X_A^T         (porb2: nchm rows, if do_active)
@@ -142,6 +159,18 @@ int mjk_cho_fmcscf(mjk_context *ctx, double *flt, int32_t nforb, int32_t niorb,
                    const double *porb1, int32_t nchm, const double *porb2,
                    const double *porb3, double *w_pwxy, const double *cmo,
                    int32_t nfro, int32_t norb, double exfac, int32_t precision);
+
+/* CD_InCore (src/cholesky_util/cd_incore.F90:35-60, cd_incore_1.F90:14-105)
+ * on the device: pivoted incomplete Cholesky of the n x n matrix x
+ * (column-major device array, destroyed) into vec (n x mxvec, column-major
+ * device array), *numcho vectors.  thr < 0 selects the default 1e-6.  *irc
+ * receives the reference's code: 0 success, 101 negative diagonal below
+ * -1e-8, 102 more than mxvec vectors needed, -1 mxvec <= 0.  The DECO
+ * density decomposition of src/scf/choscf_drv.F90:99-135 (thr = 1e-8 * max
+ * diagonal) and src/fock_util/cho_cas_drv.F90:129,227.  n <= 4096.
+ * Synchronous. */
+int mjk_cd_incore(mjk_context *ctx, int32_t n, double *x, double *vec, int32_t mxvec,
+                  int32_t *numcho, double thr, int32_t *irc);
 
 /* ---- device engine (device pointers, asynchronous on the context stream) */
 

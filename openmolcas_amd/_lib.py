@@ -44,12 +44,16 @@ SIGNATURES = {
     "mjk_set_reduced_set": (_c.c_int, [_vp, _i32, _i64, _vp]),
     "mjk_upload_vectors": (_c.c_int, [_vp, _i32, _i64, _i64, _vp, _i32]),
     "mjk_cho_fscf": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
+    "mjk_cho_lk_scf": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32,
+                                  _c.c_double, _c.c_double,
+                                  _i32, _vp, _i32]),
     "mjk_cho_fmcscf": (_c.c_int, [_vp, _vp, _i32, _i32, _i32, _c.c_double, _vp, _i32, _vp, _i32,
                                   _vp, _vp, _vp, _vp, _i32, _i32, _c.c_double, _i32]),
     "mjk_coulomb": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp]),
     "mjk_exchange": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32]),
     "mjk_exchange_mixed": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _i64]),
     "mjk_vector_norms": (_c.c_int, [_vp, _vp]),
+    "mjk_cd_incore": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _c.c_double, _vp]),
     "mjk_fock_update": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mjk_energy": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "mjk_puvx": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _i32, _vp]),

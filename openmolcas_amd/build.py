@@ -16,7 +16,8 @@ OBJ = os.path.join(HERE, "csrc", "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmolcas_jk.so")
 
-SOURCES = ["capi.hip", "coulomb.hip", "exchange.hip", "assembly.hip", "puvx.hip", "backward.hip", "peaks.hip", "tune.hip"]
+SOURCES = ["capi.hip", "coulomb.hip", "exchange.hip", "assembly.hip", "puvx.hip", "backward.hip", "peaks.hip", "tune.hip",
+           "deco.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -473,6 +473,104 @@ int mjk_cho_fscf(mjk_context *c, int32_t nden, double *flt, const int32_t *nforb
   return hip_check(c, hipStreamSynchronize(c->stream), "cho_fscf sync");
 }
 
+int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const int32_t *nforb,
+                   const int32_t *niorb, const double *porb, const double *plt, double factxi,
+                   int32_t nscreen, double dmpk, double dfmat, int32_t nshell,
+                   const int32_t *nbassh, int32_t precision) {
+  // Every shell pair and orbital is kept: the result is the reference's in the
+  // limit dmpk -> 0, so the screening controls only have to be valid.
+  (void)nscreen;
+  (void)dfmat;
+  if (!c) return MJK_ERR_ARG;
+  if (nden != 1 && nden != 2)
+    return set_error(c, MJK_ERR_ARG, "cho_lk_scf: invalid parameter nDen");  // cho_lk_scf.F90:93-96
+  if (!flt || !nforb || !niorb || !plt) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: null pointer");
+  if (dmpk < 0.0) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: dmpk < 0");
+  const int64_t np = c->np, n = c->n;
+  int32_t nocc[2] = {0, 0};
+  int64_t tot = 0;
+  for (int d = 0; d < nden; ++d) {
+    if (nforb[d] < 0 || niorb[d] < 0 || nforb[d] + niorb[d] > n)
+      return set_error(c, MJK_ERR_ARG, "cho_lk_scf: bad orbital count");
+    nocc[d] = nforb[d] + niorb[d];
+    tot += int64_t(nocc[d]) * n;
+  }
+  if (tot > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: null porb");
+  // KLT storage: shell-pair blocks (cho_lk_scf.F90:274-292); no shells = one shell
+  std::vector<int64_t> kpos;
+  if (klt) {
+    std::vector<int32_t> sh;
+    if (nshell > 0 && nbassh) sh.assign(nbassh, nbassh + nshell);
+    else sh.assign(1, int32_t(n));
+    int64_t sum = 0;
+    for (int32_t v : sh) {
+      if (v < 0) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: negative shell size");
+      sum += v;
+    }
+    if (sum != n) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: shell sizes do not add up to nbas");
+    kpos.resize(static_cast<size_t>(np));
+    int64_t off = 0, first_a = 0;
+    for (size_t sa = 0; sa < sh.size(); ++sa) {
+      int64_t first_b = 0;
+      for (size_t sb = 0; sb <= sa; ++sb) {
+        for (int64_t bl = 0; bl < sh[sb]; ++bl)
+          for (int64_t al = (sa == sb ? bl : 0); al < sh[sa]; ++al) {
+            const int64_t a = first_a + al, b = first_b + bl;
+            kpos[size_t(a * (a + 1) / 2 + b)] =
+                off + (sa == sb ? al * (al + 1) / 2 + bl : sh[sa] * bl + al);
+          }
+        off += sa == sb ? int64_t(sh[sa]) * (sh[sa] + 1) / 2 : int64_t(sh[sa]) * sh[sb];
+        first_b += sh[sb];
+      }
+      first_a += sh[sa];
+    }
+  }
+  // orbitals to the orbital-major layout the exchange engine reads
+  std::vector<double> corb(static_cast<size_t>(tot));
+  for (int d = 0, o0 = 0; d < nden; o0 += nocc[d] * int32_t(n), ++d)
+    for (int64_t k = 0; k < nocc[d]; ++k)
+      for (int64_t b = 0; b < n; ++b)
+        corb[size_t(o0 + k + nocc[d] * b)] = porb[size_t(d) * n * n + b + n * k];
+  // device scratch: D | J | K(nden) | F(nden) | orbitals
+  const size_t need = sizeof(double) * size_t(np * (2 + 2 * nden) + tot);
+  int rc;
+  if ((rc = ensure(c, c->w->misc, need))) return rc;
+  double *dD = static_cast<double *>(c->w->misc.ptr);
+  double *dJ = dD + np, *dK = dJ + np, *dF = dK + nden * np, *dC = dF + nden * np;
+  if ((rc = hip_check(c, hipMemcpyAsync(dD, plt, sizeof(double) * np, hipMemcpyHostToDevice,
+                                        c->stream), "copy PLT")) ||
+      (rc = hip_check(c, hipMemcpyAsync(dF, flt, sizeof(double) * np * nden,
+                                        hipMemcpyHostToDevice, c->stream), "copy FLT")))
+    return rc;
+  if (tot > 0 && (rc = hip_check(c, hipMemcpyAsync(dC, corb.data(), sizeof(double) * tot,
+                                                   hipMemcpyHostToDevice, c->stream),
+                                 "copy Porb")))
+    return rc;
+  if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
+  const double *cptr[2] = {dC, dC + int64_t(nocc[0]) * n};
+  if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
+  const int32_t jsrc[2] = {0, 0};
+  const double fj[2] = {1.0, 1.0};               // FactCI, cho_lk_scf.F90:102
+  const double fk[2] = {-factxi, -factxi};       // -FactXI, cho_lk_scf.F90:803,830
+  if ((rc = fock_update(c, nden, dF, np, dJ, np, jsrc, fj, dK, np, fk))) return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(flt, dF, sizeof(double) * np * nden,
+                                        hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
+    return rc;
+  std::vector<double> kh;
+  if (klt) {
+    kh.resize(static_cast<size_t>(np * nden));
+    if ((rc = hip_check(c, hipMemcpyAsync(kh.data(), dK, sizeof(double) * np * nden,
+                                          hipMemcpyDeviceToHost, c->stream), "copy K back")))
+      return rc;
+  }
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "cho_lk_scf sync"))) return rc;
+  if (klt)
+    for (int d = 0; d < nden; ++d)
+      for (int64_t p = 0; p < np; ++p)
+        klt[d * np + kpos[size_t(p)]] += -factxi * kh[size_t(d * np + p)];
+  return MJK_OK;
+}
+
 int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,
                    double factxi, const double *dlt, int32_t do_active, const double *porb1,
                    int32_t nchm, const double *porb2, const double *porb3, double *w_pwxy,

@@ -161,6 +161,38 @@ class JKContext:
         self._check(rc)
         return rc
 
+    def cho_lk_scf(self, nden: int, flt: np.ndarray, klt, nforb, niorb, porb, plt: np.ndarray,
+                   factxi: float, nscreen: int = 10, dmpk: float = 0.1, dfmat: float = 0.0,
+                   nbassh=None, precision="fp64") -> int:
+        """CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,
+        dFmat), nSym = 1, without LK screening (the dmpk -> 0 result).
+
+        flt: (nden, npair) updated in place.  klt: None or (nden, npair),
+        accumulated in the shell-pair-blocked storage of the shells nbassh.
+        porb: list of nden (nbas, >= nforb[d]+niorb[d]) AO x orbital matrices
+        C[b, k] (the MSQ columns).  plt: folded total density."""
+        for a in (flt,) + ((klt,) if klt is not None else ()):
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise TypeError("flt/klt must be C-contiguous float64 arrays (updated in place)")
+        nf = np.ascontiguousarray(nforb, dtype=np.int32)
+        ni = np.ascontiguousarray(niorb, dtype=np.int32)
+        n = self.nbas
+        pc = np.zeros(max(1, nden * n * n)) if nden in (1, 2) else np.zeros(1)
+        if nden in (1, 2):
+            for d in range(nden):
+                c = _host_f64(porb[d])
+                sq = np.zeros((n, n))
+                sq[:, :c.shape[1]] = c
+                pc[d * n * n:(d + 1) * n * n] = sq.T.reshape(-1)  # AO index fastest
+        p = _host_f64(plt).reshape(-1)[: self.npair].copy()
+        sh = np.ascontiguousarray(nbassh if nbassh is not None else [], dtype=np.int32)
+        rc = self.lib.mjk_cho_lk_scf(self._h, int(nden), _ptr(flt), _ptr(klt), _ptr(nf), _ptr(ni),
+                                     _ptr(pc), _ptr(p), float(factxi), int(nscreen), float(dmpk),
+                                     float(dfmat), len(sh), _ptr(sh) if len(sh) else None,
+                                     precision_code(precision))
+        self._check(rc)
+        return rc
+
     def cho_fmcscf(self, flt: np.ndarray, nforb: int, niorb: int, naorb: int, factxi: float,
                    dlt: np.ndarray, do_active: bool, porb1, nchm: int, porb2, porb3,
                    w_pwxy, cmo, nfro: int, norb: int, exfac: float = 1.0,
@@ -220,6 +252,20 @@ class JKContext:
         self._check(self.lib.mjk_exchange_mixed(self._h, nk, _ptr(nocc), ptrs, _ptr(k), _ld(k),
                                                 int(j_fp32), int(j_bf16)))
         return k
+
+    def cd_incore(self, x: torch.Tensor, thr: float, mxvec=None):
+        """CD_InCore on the device: returns (irc, V) with V (n, numcho) and
+        x @ x.T-style reconstruction V V^T ~ x to within thr on the diagonal.
+        x (n, n) symmetric (a copy is decomposed)."""
+        n = x.shape[0]
+        xc = _dev_f64(x, self.device).t().contiguous()  # column-major copy (symmetric anyway)
+        mx = n if mxvec is None else int(mxvec)
+        v = torch.zeros((max(mx, 1), n), dtype=torch.float64, device=self.device)  # column-major (n, mx)
+        num = ctypes.c_int32(0)
+        irc = ctypes.c_int32(0)
+        self._check(self.lib.mjk_cd_incore(self._h, int(n), _ptr(xc), _ptr(v), mx, ctypes.byref(num),
+                                           float(thr), ctypes.byref(irc)))
+        return int(irc.value), v[:num.value].t()
 
     def vector_norms(self) -> torch.Tensor:
         """(nchol,) squared Frobenius norms of the symmetric vectors L^J."""

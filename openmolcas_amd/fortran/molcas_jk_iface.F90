@@ -14,7 +14,7 @@ module molcas_jk_iface
   implicit none
   private
   public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_cho_fscf, &
-            mjk_cho_fmcscf, cho_fscf_gpu, cho_fmcscf_gpu, jk_gpu_init, jk_gpu_final
+            mjk_cho_fmcscf, mjk_cho_lk_scf, cho_fscf_gpu, cho_lk_scf_gpu, cho_fmcscf_gpu, jk_gpu_init, jk_gpu_final
 
   integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1
   type(c_ptr), save, public :: jk_ctx = c_null_ptr
@@ -55,6 +55,16 @@ module molcas_jk_iface
       real(c_double), intent(in) :: porb(*), dlt(*)
       real(c_double), value :: exfac
     end function mjk_cho_fscf
+    integer(c_int) function mjk_cho_lk_scf(ctx, nden, flt, klt, nforb, niorb, porb, plt, factxi, &
+        nscreen, dmpk, dfmat, nshell, nbassh, prec) bind(C, name='mjk_cho_lk_scf')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nden, nscreen, nshell, prec
+      real(c_double), intent(inout) :: flt(*), klt(*)
+      integer(c_int32_t), intent(in) :: nforb(*), niorb(*), nbassh(*)
+      real(c_double), intent(in) :: porb(*), plt(*)
+      real(c_double), value :: factxi, dmpk, dfmat
+    end function mjk_cho_lk_scf
     integer(c_int) function mjk_cho_fmcscf(ctx, flt, nforb, niorb, naorb, factxi, dlt, do_active, &
         porb1, nchm, porb2, porb3, w_pwxy, cmo, nfro, norb, exfac, prec) bind(C, name='mjk_cho_fmcscf')
       import :: c_ptr, c_int, c_int32_t, c_double
@@ -98,6 +108,30 @@ contains
     rc = int(mjk_cho_fscf(jk_ctx, int(nDen, c_int32_t), FLT, nf, ni, Porb, DLT, real(ExFac, c_double), &
                           MJK_PREC_FP64))
   end subroutine cho_fscf_gpu
+
+  ! CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,dFmat), nSym = 1,
+  ! unscreened: FLT(:,d) += J[PLT(:,1)] - FactXI * K[Porb(d)], Porb(d) = MSQ A2(nBas,nBas);
+  ! KLT(:,d) accumulates -FactXI*K in shell-pair blocks of the nShell shells nBasSh.
+  subroutine cho_lk_scf_gpu(rc, nDen, FLT, KLT, nForb, nIorb, Porb, PLT, FactXI, nScreen, dmpk, dFmat, &
+                            nShell, nBasSh)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nDen, nForb(nDen), nIorb(nDen), nScreen, nShell, nBasSh(*)
+    real(8), intent(inout) :: FLT(*), KLT(*)
+    real(8), intent(in) :: Porb(*), PLT(*), FactXI, dmpk, dFmat
+    integer(c_int32_t) :: nf(2), ni(2)
+    integer(c_int32_t), allocatable :: sh(:)
+    nf = 0
+    ni = 0
+    nf(1:min(nDen, 2)) = int(nForb(1:min(nDen, 2)), c_int32_t)
+    ni(1:min(nDen, 2)) = int(nIorb(1:min(nDen, 2)), c_int32_t)
+    allocate(sh(max(nShell, 1)))
+    sh = 0
+    if (nShell > 0) sh(1:nShell) = int(nBasSh(1:nShell), c_int32_t)
+    rc = int(mjk_cho_lk_scf(jk_ctx, int(nDen, c_int32_t), FLT, KLT, nf, ni, Porb, PLT, real(FactXI, c_double), &
+                            int(nScreen, c_int32_t), real(dmpk, c_double), real(dFmat, c_double), &
+                            int(nShell, c_int32_t), sh, MJK_PREC_FP64))
+    deallocate(sh)
+  end subroutine cho_lk_scf_gpu
 
   ! CHO_FMCSCF(rc,FLT,nForb,nIorb,nAorb,FactXI,DLT,DoActive,POrb,nChM,W_PWXY,CMO,ExFac), nSym = 1.
   subroutine cho_fmcscf_gpu(rc, FLT, nForb, nIorb, nAorb, FactXI, DLT, DoActive, POrb1, nChM, POrb2, &

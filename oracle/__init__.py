@@ -14,6 +14,9 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   cho_fscf              -- dense brute force J/K, batching invariance,
                            reduced-set invariance, 4FC gradient identity
   cho_fmcscf            -- dense brute force FI/FA/(pw|xy)
+  cho_lk_scf            -- dense brute force J[P] - FactXI K, shell-pair KLT
+                           layout against an index map built from the block
+                           rules, equality with cho_fscf (unscreened)
   cd_incore             -- ||D - V V^T||_max <= thr, exact rank, numpy cholesky
   energy                -- closed form tr(hP) + 1/2 tr(PJ) - 1/4 tr(PK)
 """
@@ -76,6 +79,10 @@ def lib():
         L.ora_cho_fscf.restype = ctypes.c_int32
         L.ora_cho_fscf.argtypes = [ctypes.POINTER(_Chol), ctypes.c_int32, _dp, _i32p, _i32p,
                                    ctypes.POINTER(_dp), _dp, ctypes.c_double, ctypes.c_int64]
+        L.ora_cho_lk_scf.restype = ctypes.c_int32
+        L.ora_cho_lk_scf.argtypes = [ctypes.POINTER(_Chol), ctypes.c_int32, _dp, _dp, _i32p, _i32p,
+                                     ctypes.POINTER(_dp), _dp, ctypes.c_double, ctypes.c_int32,
+                                     _i32p, ctypes.c_int64]
         L.ora_cho_fmcscf.restype = ctypes.c_int32
         L.ora_cho_fmcscf.argtypes = [ctypes.POINTER(_Chol), _dp, ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_int32, ctypes.c_double, _dp, ctypes.c_int32, _dp,
@@ -230,6 +237,32 @@ def cho_fscf(store: CholStore, flt: np.ndarray, nocc, porbs, dlt: np.ndarray, ex
     if rc != 0:
         raise RuntimeError(f"ora_cho_fscf rc={rc}")
     return f
+
+
+def cho_lk_scf(store: CholStore, flt: np.ndarray, nocc, porbs, plt: np.ndarray, factxi: float,
+               nbassh=None, batch: int = 1 << 30):
+    """CHO_LK_SCF without screening (nforb = 0, niorb = nocc).  porbs[d] is the
+    (n, nocc[d]) AO x orbital matrix C[b, k].  Returns (flt, klt), both
+    (nden, npair); klt in the shell-pair-blocked storage for shells nbassh
+    (default: one shell)."""
+    nden = len(nocc)
+    n = store.n
+    f = _f64(flt).reshape(nden, -1).copy()
+    k = np.zeros_like(f)
+    nf = np.zeros(nden, dtype=np.int32)
+    ni = np.array(nocc, dtype=np.int32)
+    cs = []
+    for p in porbs:
+        sq = np.zeros((n, n))
+        sq[:, :p.shape[1]] = p
+        cs.append(np.ascontiguousarray(sq.T))  # column-major A2(n,n): AO index fastest
+    arr = (_dp * nden)(*[_p(c) for c in cs])
+    sh = np.array([n] if nbassh is None else nbassh, dtype=np.int32)
+    rc = lib().ora_cho_lk_scf(ctypes.byref(store._c), nden, _p(f), _p(k), _p(nf, _i32p), _p(ni, _i32p),
+                              arr, _p(_f64(plt)), float(factxi), len(sh), _p(sh, _i32p), int(batch))
+    if rc != 0:
+        raise RuntimeError(f"ora_cho_lk_scf rc={rc}")
+    return f, k
 
 
 def cho_fmcscf(store: CholStore, flt: np.ndarray, nk: int, naorb: int, factxi: float,

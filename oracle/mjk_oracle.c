@@ -282,6 +282,134 @@ int32_t ora_cho_fscf(const ora_chol *ch, int32_t nden, double *flt,
   return 0;
 }
 
+/* src/fock_util/cho_lk_scf.F90:14-1103 with nSym = 1 and every shell pair and
+ * orbital kept (no LK screening: the limit dmpk -> 0 of the reference).
+ *   FLT(d) += J[PLT(1)] + KLT(d),  KLT(d) += -FactXI * sum_{J,k} L(a,J)[k] L(b,J)[k]
+ * with L(a,J)[k] = sum_b L^J_ab C(b,k) over the full square L^J
+ * (CHO_GetShFull + DGEMV, :507,:675-689) and KLT(d) in the reference's
+ * shell-pair-blocked storage (:274-292): pairs iaSh >= ibSh in order, an
+ * off-diagonal block nBasSh(iaSh) x nBasSh(ibSh) column-major, a diagonal
+ * block lower-packed; folded into FLT at the end (:963-1028).
+ * porb[d] is Porb(d)%SB(1)%A2(n,n), AO index fastest, columns 0..nforb+niorb-1
+ * used.  klt (nden blocks of n(n+1)/2, may be NULL) is accumulated.  nbassh
+ * lists the shell sizes (sum = n).  Returns rc. */
+int32_t ora_cho_lk_scf(const ora_chol *ch, int32_t nden, double *flt, double *klt,
+                       const int32_t *nforb, const int32_t *niorb,
+                       const double *const *porb, const double *plt, double factxi,
+                       int32_t nshell, const int32_t *nbassh, int64_t nvec_batch) {
+  if (nden != 1 && nden != 2) return 99; /* :93-96 abend */
+  const int32_t n = ch->n;
+  const int64_t np = (int64_t)n * (n + 1) / 2;
+  int64_t tot = 0;
+  for (int32_t s = 0; s < nshell; ++s) tot += nbassh[s];
+  if (nshell < 1 || tot != n) return 99;
+  /* shell offsets kOffSh and shell-pair offsets nnBfShp (:255-292) */
+  int64_t *koff = malloc(sizeof(int64_t) * (size_t)nshell);
+  int64_t *poff = malloc(sizeof(int64_t) * (size_t)nshell * (nshell + 1) / 2);
+  int64_t lk = 0, ao = 0;
+  for (int32_t a = 0; a < nshell; ++a) {
+    koff[a] = ao;
+    ao += nbassh[a];
+    for (int32_t b = 0; b <= a; ++b) {
+      poff[(int64_t)a * (a + 1) / 2 + b] = lk;
+      lk += (int64_t)nbassh[a] * nbassh[b] -
+            (a == b ? (int64_t)nbassh[a] * (nbassh[a] - 1) / 2 : 0);
+    }
+  }
+  double *kbuf = klt ? klt : calloc((size_t)(nden * np), sizeof(double));
+  for (int32_t jred = 0; jred < ch->nred; ++jred) {
+    int64_t ivrs, nvrs;
+    nvec_rs(ch, jred, &ivrs, &nvrs);
+    if (nvrs == 0) continue;
+    const int64_t nrs = ch->nrs[jred];
+    const int64_t *map = ch->rs2lt + ch->rs_off[jred];
+    int64_t nvec = nvec_batch < nvrs ? nvec_batch : nvrs;
+    if (nvec < 1) { free(koff); free(poff); if (!klt) free(kbuf); return 33; }
+    double *drs = calloc((size_t)nrs, sizeof(double));
+    double *frs = calloc((size_t)nrs, sizeof(double));
+    double *lrs = malloc(sizeof(double) * (size_t)nrs * nvec);
+    double *vj = malloc(sizeof(double) * (size_t)nvec);
+    double *lfull = malloc(sizeof(double) * (size_t)nvec * n * n);
+    double *lab = malloc(sizeof(double) * (size_t)nvec * n);
+    ora_swap_full2rs(nrs, map, 1, np, plt, drs); /* :397-403 */
+    int64_t nbatch = (nvrs - 1) / nvec + 1;
+    for (int64_t ib = 0; ib < nbatch; ++ib) {
+      int64_t jnum = (ib == nbatch - 1) ? nvrs - nvec * (nbatch - 1) : nvec;
+      int64_t jvec = nvec * ib + ivrs;
+      int64_t numv = vec_read(ch, jvec, jnum, nrs, lrs);
+      if (numv <= 0 || numv != jnum) { /* :419-424 */
+        free(drs); free(frs); free(lrs); free(vj); free(lfull); free(lab);
+        free(koff); free(poff); if (!klt) free(kbuf);
+        return 77;
+      }
+      /* Coulomb of the total density, :428-449 */
+      ora_dgemv('T', nrs, jnum, 1.0, lrs, nrs, drs, 0.0, vj);
+      double fact = (jvec - ivrs) < 1 ? (double)(jvec - ivrs) : 1.0;
+      ora_dgemv('N', nrs, jnum, 1.0, lrs, nrs, vj, fact, frs);
+      /* full square L(a,b) of each vector (CHO_GetShFull, :507) */
+      memset(lfull, 0, sizeof(double) * (size_t)jnum * n * n);
+      for (int64_t j = 0; j < jnum; ++j) {
+        double *lf = lfull + j * n * n;
+        for (int64_t r = 0; r < nrs; ++r) {
+          int64_t a, b;
+          lt_pair(map[r], &a, &b);
+          lf[a * n + b] = lrs[r + j * nrs];
+          lf[b * n + a] = lrs[r + j * nrs];
+        }
+      }
+      for (int32_t d = 0; d < nden; ++d) {
+        const int32_t nk = nforb[d] + niorb[d];
+        double *kd = kbuf + d * np;
+        for (int32_t k = 0; k < nk; ++k) {
+          const double *ck = porb[d] + (int64_t)n * k;
+          /* L(a,J)[k] = sum_b L(a,b) C(b,k), :675-689 */
+          for (int64_t j = 0; j < jnum; ++j)
+            for (int64_t a = 0; a < n; ++a) {
+              const double *row = lfull + j * n * n + a * n;
+              double s = 0.0;
+              for (int64_t b = 0; b < n; ++b) s += row[b] * ck[b];
+              lab[j * n + a] = s;
+            }
+          /* shell-pair blocks, DGEMM_Tri on diagonal pairs (:803), DGEMM off it (:830) */
+          for (int32_t sa = 0; sa < nshell; ++sa)
+            for (int32_t sb = 0; sb <= sa; ++sb) {
+              const int64_t off = poff[(int64_t)sa * (sa + 1) / 2 + sb];
+              for (int64_t bl = 0; bl < nbassh[sb]; ++bl)
+                for (int64_t al = (sa == sb ? bl : 0); al < nbassh[sa]; ++al) {
+                  const int64_t ag = koff[sa] + al, bg = koff[sb] + bl;
+                  double s = 0.0;
+                  for (int64_t j = 0; j < jnum; ++j) s += lab[j * n + ag] * lab[j * n + bg];
+                  const int64_t idx = sa == sb ? al * (al + 1) / 2 + bl : nbassh[sa] * bl + al;
+                  kd[off + idx] += -factxi * s;
+                }
+            }
+        }
+      }
+    }
+    for (int32_t d = 0; d < nden; ++d) /* :912-918 */
+      ora_swap_rs2full(nrs, map, 1, np, frs, flt + d * np, 1);
+    free(drs); free(frs); free(lrs); free(vj); free(lfull); free(lab);
+  }
+  /* KLT -> FLT (:963-1028) */
+  for (int32_t d = 0; d < nden; ++d) {
+    const double *kd = kbuf + d * np;
+    double *fd = flt + d * np;
+    for (int32_t sa = 0; sa < nshell; ++sa)
+      for (int32_t sb = 0; sb <= sa; ++sb) {
+        const int64_t off = poff[(int64_t)sa * (sa + 1) / 2 + sb];
+        for (int64_t bl = 0; bl < nbassh[sb]; ++bl)
+          for (int64_t al = (sa == sb ? bl : 0); al < nbassh[sa]; ++al) {
+            const int64_t ag = koff[sa] + al, bg = koff[sb] + bl;
+            const int64_t idx = sa == sb ? al * (al + 1) / 2 + bl : nbassh[sa] * bl + al;
+            fd[ag * (ag + 1) / 2 + bg] += kd[off + idx];
+          }
+      }
+  }
+  free(koff); free(poff);
+  if (!klt) free(kbuf);
+  return 0;
+}
+
 /* src/fock_util/cho_fmcscf.F90:14-585 with nSym = 1, followed by
  * src/fock_util/cho_eval_waxy.F90:12-150 for the (pw|xy) integrals.
  *   FI += J[DI] + FactXI * sum X_I X_I^T     (POrb(1), nforb+niorb rows)

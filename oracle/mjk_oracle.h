@@ -72,6 +72,11 @@ int32_t ora_cho_fscf(const ora_chol *ch, int32_t nden, double *flt,
                      const double *const *porb, const double *dlt,
                      double exfac, int64_t nvec_batch);
 
+int32_t ora_cho_lk_scf(const ora_chol *ch, int32_t nden, double *flt, double *klt,
+                       const int32_t *nforb, const int32_t *niorb,
+                       const double *const *porb, const double *plt, double factxi,
+                       int32_t nshell, const int32_t *nbassh, int64_t nvec_batch);
+
 int32_t ora_cho_fmcscf(const ora_chol *ch, double *flt, int32_t nforb,
                        int32_t niorb, int32_t naorb, double factxi,
                        const double *dlt, int32_t do_active,

@@ -163,6 +163,41 @@ def test_cho_fscf_dropin_multiple_reduced_sets(gpu):
     assert rel(flt, want) < RTOL
 
 
+@pytest.mark.parametrize("nden,shells", [(1, None), (2, [5, 9, 3, 20, 13]), (2, [1] * 50)])
+def test_cho_lk_scf_dropin(gpu, nden, shells):
+    """CHO_LK_SCF without screening: FLT and the shell-pair-blocked KLT."""
+    rng = np.random.default_rng(40 + nden)
+    n, M1, M2 = 50, 24, 9
+    npr = npair(n)
+    s0 = np.arange(npr)
+    s1 = np.sort(rng.choice(s0, size=500, replace=False))
+    L0 = chol_vectors_np(n, M1, seed=41)
+    L1 = rng.standard_normal((M2, len(s1))) * 0.05
+    st = oracle.CholStore(n, [s0, s1], [(0, v) for v in L0] + [(1, v) for v in L1])
+    C = [orbitals_np(n, 9 + 4 * d, seed=42 + d) for d in range(nden)]
+    plt = fold(sum(c @ c.T for c in C))
+    flt0 = rng.standard_normal((nden, npr))
+    want_f, want_k = oracle.cho_lk_scf(st, flt0, [c.shape[1] for c in C], C, plt, 0.5,
+                                       nbassh=shells, batch=7)
+    ctx = JKContext(n, M1 + M2, device=gpu)
+    ctx.set_reduced_set(0, s0)
+    ctx.set_reduced_set(1, s1)
+    ctx.upload_vectors(0, 0, L0)
+    ctx.upload_vectors(1, M1, L1)
+    flt = flt0.copy()
+    klt = np.zeros((nden, npr))
+    ctx.cho_lk_scf(nden, flt, klt, [0] * nden, [c.shape[1] for c in C], C, plt, 0.5,
+                   nbassh=shells)
+    assert rel(flt, want_f) < RTOL
+    assert rel(klt, want_k) < RTOL
+    with pytest.raises(MjkError):
+        ctx.cho_lk_scf(3, flt, klt, [0] * 3, [1] * 3, C * 3, plt, 0.5)
+    with pytest.raises(MjkError):
+        ctx.cho_lk_scf(1, flt[:1].copy(), klt[:1].copy(), [0], [3], C[:1], plt, 0.5,
+                       nbassh=[10, 10])  # shells do not cover nbas
+    ctx.close()
+
+
 def test_cho_fmcscf_dropin_with_puvx(gpu):
     n, M = 60, 35
     nk, na = 7, 5

@@ -172,6 +172,68 @@ def test_cho_fscf_rejects_bad_nden():
                         np.zeros(npair(n)), 1.0)
 
 
+def _shell_pair_map(nbassh):
+    """LT index (a >= b) -> position in the shell-pair-blocked KLT of
+    src/fock_util/cho_lk_scf.F90:274-292 (pairs iaSh >= ibSh in order; an
+    off-diagonal block is nBasSh(iaSh) x nBasSh(ibSh) column-major, a diagonal
+    block is lower-packed by rows)."""
+    first = np.concatenate([[0], np.cumsum(nbassh)])
+    n = int(first[-1])
+    pos = np.empty(npair(n), dtype=np.int64)
+    off = 0
+    for sa, na in enumerate(nbassh):
+        for sb in range(sa + 1):
+            nb = nbassh[sb]
+            for bl in range(nb):
+                for al in range(na):
+                    if sa == sb and al < bl:
+                        continue
+                    a, b = first[sa] + al, first[sb] + bl
+                    loc = al * (al + 1) // 2 + bl if sa == sb else na * bl + al
+                    pos[a * (a + 1) // 2 + b] = off + loc
+            off += na * nb if sa != sb else na * (na + 1) // 2
+    assert off == npair(n)
+    return pos
+
+
+@pytest.mark.parametrize("nden,shells", [(1, [13]), (2, [3, 1, 5, 4]), (2, [1] * 13)])
+def test_cho_lk_scf_matches_dense(nden, shells):
+    n, M = 13, 25
+    L = chol_vectors_np(n, M, seed=60)
+    C = [orbitals_np(n, 3 + 2 * d, seed=61 + d) for d in range(nden)]
+    P = 2.0 * C[0] @ C[0].T
+    plt = fold(P)
+    flt = np.random.default_rng(62).standard_normal((nden, npair(n)))
+    factxi = 0.7
+    got, klt = oracle.cho_lk_scf(oracle.CholStore.full(L), flt, [c.shape[1] for c in C], C, plt,
+                                 factxi, nbassh=shells, batch=6)
+    Lf = lfull(L, n)
+    J, _ = coulomb(Lf, unpack(plt, n) * np.where(np.eye(n) > 0, 1.0, 0.5))
+    pos = _shell_pair_map(shells)
+    for d in range(nden):
+        K = exchange(Lf, C[d])
+        want = flt[d] + pack(J) - factxi * pack(K)
+        assert rel(got[d], want) < TOL
+        assert rel(klt[d][pos], -factxi * pack(K)) < TOL
+
+
+def test_cho_lk_scf_equals_cho_fscf_unscreened():
+    rng = np.random.default_rng(70)
+    n, M1, M2 = 11, 8, 7
+    npr = npair(n)
+    s0 = np.arange(npr)
+    s1 = np.sort(rng.choice(s0, size=40, replace=False))
+    L0 = chol_vectors_np(n, M1, seed=71)
+    L1 = rng.standard_normal((M2, len(s1))) * 0.1
+    st = oracle.CholStore(n, [s0, s1], [(0, L0[j]) for j in range(M1)] + [(1, L1[j]) for j in range(M2)])
+    C = [orbitals_np(n, 4, seed=72), orbitals_np(n, 2, seed=73)]
+    plt = fold(C[0] @ C[0].T + C[1] @ C[1].T)
+    flt = rng.standard_normal((2, npr))
+    lk, _ = oracle.cho_lk_scf(st, flt, [4, 2], C, plt, 0.5, nbassh=[4, 4, 3], batch=5)
+    fs = oracle.cho_fscf(st, flt, [4, 2], C, plt, 0.5, batch=3)
+    assert rel(lk, fs) < TOL
+
+
 # ------------------------------------------------------------- energy and gradient
 def test_energy_and_orbital_gradient_identity():
     """E(C) with P = 2CC^T built through cho_fscf (DECO convention: V = sqrt2 C,
