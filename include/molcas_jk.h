@@ -147,6 +147,12 @@ int mjk_cho_lk_scf(mjk_context *ctx, int32_t nden, double *flt, double *klt,
                    double dfmat, int32_t nshell, const int32_t *nbassh,
                    int32_t precision);
 
+/* CHO_FOCK_DFT_RED (src/fock_util/cho_fock_dft_red.F90:14), the J-only build
+ * of pure-DFT SCF: FLT = J[DLT], written, not accumulated (the first reduced
+ * set's swap_rs2full runs with add = .false., which zeroes FLT,
+ * swap_rs2full.F90:37-41).  Called from src/scf/choscf_drv.F90:95,302. */
+int mjk_cho_fock_dft_red(mjk_context *ctx, const double *dlt, double *flt);
+
 /* CHO_FMCSCF (src/fock_util/cho_fmcscf.F90:14), nSym = 1:
  *   FI += J[DI] + FactXI * sum X_I X_I^T      (porb1: nforb+niorb rows)
  *   FA += J[DA] - 1/2 * sum X_A X_A^T         (porb2: nchm rows, if do_active)

@@ -158,7 +158,7 @@ int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
   c->ldl = round_up(c->np, kRowAlign);
   c->n_pad = int32_t(round_up(nbas, 128));
   c->nchol = nchol;
-  c->loaded.assign(size_t(nchol), 0);
+  c->vset.assign(size_t(nchol), 0);
   c->wsp = std::make_shared<Workspace>();
   c->w = c->wsp.get();
   int rc;
@@ -240,7 +240,7 @@ int mjk_device_vectors(mjk_context *c, double **L, int64_t *ldl) {
   if (!c || !L || !ldl) return MJK_ERR_ARG;
   *L = c->L;
   *ldl = c->ldl;
-  std::fill(c->loaded.begin(), c->loaded.end(), uint8_t(1));
+  std::fill(c->vset.begin(), c->vset.end(), -1);
   c->nloaded = c->nchol;
   c->l32_valid = false;
   return MJK_OK;
@@ -345,10 +345,8 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "upload sync"))) return rc;
   c->l32_valid = false;
   for (int64_t j = jvec; j < jvec + jnum; ++j) {
-    if (!c->loaded[size_t(j)]) {
-      c->loaded[size_t(j)] = 1;
-      ++c->nloaded;
-    }
+    if (!c->vset[size_t(j)]) ++c->nloaded;
+    c->vset[size_t(j)] = jred + 1;
   }
   return MJK_OK;
 }
@@ -569,6 +567,27 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
       for (int64_t p = 0; p < np; ++p)
         klt[d * np + kpos[size_t(p)]] += -factxi * kh[size_t(d * np + p)];
   return MJK_OK;
+}
+
+int mjk_cho_fock_dft_red(mjk_context *c, const double *dlt, double *flt) {
+  if (!c) return MJK_ERR_ARG;
+  if (!dlt || !flt) return set_error(c, MJK_ERR_ARG, "cho_fock_dft_red: null pointer");
+  const int64_t np = c->np;
+  int rc;
+  if ((rc = ensure(c, c->w->misc, sizeof(double) * size_t(2 * np)))) return rc;
+  double *dD = static_cast<double *>(c->w->misc.ptr);
+  double *dJ = dD + np;
+  if ((rc = hip_check(c, hipMemcpyAsync(dD, dlt, sizeof(double) * np, hipMemcpyHostToDevice,
+                                        c->stream), "copy DLT")))
+    return rc;
+  if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
+  // FLT is written, not accumulated: the first reduced set's swap_rs2full
+  // (add = .false., cho_fock_dft_red.F90:190-191) zeroes all of FLT
+  // (swap_rs2full.F90:37-41) and later sets add to it.
+  if ((rc = hip_check(c, hipMemcpyAsync(flt, dJ, sizeof(double) * np, hipMemcpyDeviceToHost,
+                                        c->stream), "copy FLT back")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "cho_fock_dft_red sync");
 }
 
 int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,

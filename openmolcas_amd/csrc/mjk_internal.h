@@ -54,7 +54,9 @@ struct Context {
   int32_t n_pad = 0;      // n rounded up to kTileA
   int64_t nchol = 0;      // vector capacity
   double *L = nullptr;    // nchol x ldl, packed LT per vector, zero padded
-  std::vector<uint8_t> loaded;  // per vector: uploaded?
+  // per vector: 0 = not uploaded, r + 1 = uploaded on reduced set r, -1 = written in
+  // place on the full LT (mjk_device_vectors)
+  std::vector<int32_t> vset;
   int64_t nloaded = 0;
   std::vector<ReducedSet> rsets;
 

@@ -61,8 +61,10 @@ def choscf_fock(jk: JKContext, dsq, exfac: float = 1.0, algo: int = 3, deco: boo
     out = np.ascontiguousarray(out)
     dens = [np.asarray(dsq, dtype=np.float64)] + ([np.asarray(dsq_beta, dtype=np.float64)] if uhf else [])
     dlt = _fold(dens[0] + dens[1]) if uhf else _fold(dens[0])
-    if exfac == 0.0:  # no exchange (pure DFT): Coulomb only, the role of CHO_FOCK_DFT_RED
-        jk.cho_fscf(nden, out, [0] * nden, [0] * nden, [np.zeros((n, 0))] * nden, dlt, 0.0, precision)
+    if exfac == 0.0:  # pure DFT: CHO_FOCK_DFT_RED writes J into FLT(1) (:95, :302)
+        f1 = np.ascontiguousarray(out[0])
+        jk.cho_fock_dft_red(dlt, f1)
+        out[0] = f1
         return out
     if deco:
         orbs = [_deco(jk, d, 1e-8 * float(np.max(np.diag(d)))) for d in dens]

@@ -161,6 +161,16 @@ class JKContext:
         self._check(rc)
         return rc
 
+    def cho_fock_dft_red(self, dlt: np.ndarray, flt: np.ndarray) -> int:
+        """CHO_FOCK_DFT_RED(irc,DLT,FLT): Coulomb-only Fock matrix, flt (npair,)
+        overwritten with J[DLT] (the reference zeroes FLT first)."""
+        if flt.dtype != np.float64 or not flt.flags.c_contiguous or flt.size < self.npair:
+            raise TypeError("flt must be a C-contiguous float64 array of npair values")
+        d = _host_f64(dlt).reshape(-1)[: self.npair].copy()
+        rc = self.lib.mjk_cho_fock_dft_red(self._h, _ptr(d), _ptr(flt))
+        self._check(rc)
+        return rc
+
     def cho_lk_scf(self, nden: int, flt: np.ndarray, klt, nforb, niorb, porb, plt: np.ndarray,
                    factxi: float, nscreen: int = 10, dmpk: float = 0.1, dfmat: float = 0.0,
                    nbassh=None, precision="fp64") -> int:

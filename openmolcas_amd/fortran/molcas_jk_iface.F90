@@ -14,7 +14,8 @@ module molcas_jk_iface
   implicit none
   private
   public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_cho_fscf, &
-            mjk_cho_fmcscf, mjk_cho_lk_scf, cho_fscf_gpu, cho_lk_scf_gpu, cho_fmcscf_gpu, jk_gpu_init, jk_gpu_final
+            mjk_cho_fmcscf, mjk_cho_lk_scf, mjk_cho_fock_dft_red, cho_fscf_gpu, cho_lk_scf_gpu, &
+            cho_fock_dft_red_gpu, cho_fmcscf_gpu, jk_gpu_init, jk_gpu_final
 
   integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1
   type(c_ptr), save, public :: jk_ctx = c_null_ptr
@@ -55,6 +56,12 @@ module molcas_jk_iface
       real(c_double), intent(in) :: porb(*), dlt(*)
       real(c_double), value :: exfac
     end function mjk_cho_fscf
+    integer(c_int) function mjk_cho_fock_dft_red(ctx, dlt, flt) bind(C, name='mjk_cho_fock_dft_red')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: dlt(*)
+      real(c_double), intent(inout) :: flt(*)
+    end function mjk_cho_fock_dft_red
     integer(c_int) function mjk_cho_lk_scf(ctx, nden, flt, klt, nforb, niorb, porb, plt, factxi, &
         nscreen, dmpk, dfmat, nshell, nbassh, prec) bind(C, name='mjk_cho_lk_scf')
       import :: c_ptr, c_int, c_int32_t, c_double
@@ -108,6 +115,14 @@ contains
     rc = int(mjk_cho_fscf(jk_ctx, int(nDen, c_int32_t), FLT, nf, ni, Porb, DLT, real(ExFac, c_double), &
                           MJK_PREC_FP64))
   end subroutine cho_fscf_gpu
+
+  ! CHO_FOCK_DFT_RED(irc,DLT,FLT), nSym = 1: FLT = J[DLT] (pure DFT, no exchange).
+  subroutine cho_fock_dft_red_gpu(irc, DLT, FLT)
+    integer, intent(out) :: irc
+    real(8), intent(in) :: DLT(*)
+    real(8), intent(inout) :: FLT(*)
+    irc = int(mjk_cho_fock_dft_red(jk_ctx, DLT, FLT))
+  end subroutine cho_fock_dft_red_gpu
 
   ! CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,dFmat), nSym = 1,
   ! unscreened: FLT(:,d) += J[PLT(:,1)] - FactXI * K[Porb(d)], Porb(d) = MSQ A2(nBas,nBas);

@@ -14,6 +14,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   cho_fscf              -- dense brute force J/K, batching invariance,
                            reduced-set invariance, 4FC gradient identity
   cho_fmcscf            -- dense brute force FI/FA/(pw|xy)
+  cho_fock_dft_red      -- dense brute force J over several reduced sets
+                           (FLT overwritten)
   cho_lk_scf            -- dense brute force J[P] - FactXI K, shell-pair KLT
                            layout against an index map built from the block
                            rules, equality with cho_fscf (unscreened)
@@ -79,6 +81,8 @@ def lib():
         L.ora_cho_fscf.restype = ctypes.c_int32
         L.ora_cho_fscf.argtypes = [ctypes.POINTER(_Chol), ctypes.c_int32, _dp, _i32p, _i32p,
                                    ctypes.POINTER(_dp), _dp, ctypes.c_double, ctypes.c_int64]
+        L.ora_cho_fock_dft_red.restype = ctypes.c_int32
+        L.ora_cho_fock_dft_red.argtypes = [ctypes.POINTER(_Chol), _dp, _dp, ctypes.c_int64]
         L.ora_cho_lk_scf.restype = ctypes.c_int32
         L.ora_cho_lk_scf.argtypes = [ctypes.POINTER(_Chol), ctypes.c_int32, _dp, _dp, _i32p, _i32p,
                                      ctypes.POINTER(_dp), _dp, ctypes.c_double, ctypes.c_int32,
@@ -236,6 +240,15 @@ def cho_fscf(store: CholStore, flt: np.ndarray, nocc, porbs, dlt: np.ndarray, ex
                             _p(d), float(exfac), int(batch))
     if rc != 0:
         raise RuntimeError(f"ora_cho_fscf rc={rc}")
+    return f
+
+
+def cho_fock_dft_red(store: CholStore, dlt: np.ndarray, flt: np.ndarray, batch: int = 1 << 30):
+    """CHO_FOCK_DFT_RED; returns the updated copy of flt (npair,)."""
+    f = _f64(flt).reshape(-1).copy()
+    rc = lib().ora_cho_fock_dft_red(ctypes.byref(store._c), _p(_f64(dlt)), _p(f), int(batch))
+    if rc != 0:
+        raise RuntimeError(f"ora_cho_fock_dft_red rc={rc}")
     return f
 
 

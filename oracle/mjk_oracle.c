@@ -282,6 +282,48 @@ int32_t ora_cho_fscf(const ora_chol *ch, int32_t nden, double *flt,
   return 0;
 }
 
+/* src/fock_util/cho_fock_dft_red.F90:14-235 (JSYM = 1): Coulomb only.
+ *   Frs = sum_J L_rs,J V_J per reduced set, V_J = sum_rs L_rs,J Drs
+ * written into FLT for the first reduced set (add = JRED > JRED1, :190-191)
+ * and added for the later ones.  Returns irc. */
+int32_t ora_cho_fock_dft_red(const ora_chol *ch, const double *dlt, double *flt,
+                             int64_t nvec_batch) {
+  const int32_t n = ch->n;
+  const int64_t np = (int64_t)n * (n + 1) / 2;
+  if (ch->nvec < 1) return 0; /* :60 */
+  const int32_t jred1 = ch->vred[0], jred2 = ch->vred[ch->nvec - 1]; /* :70-71 */
+  for (int32_t jred = jred1; jred <= jred2; ++jred) {
+    int64_t ivrs, nvrs;
+    nvec_rs(ch, jred, &ivrs, &nvrs);
+    if (nvrs == 0) continue;
+    const int64_t nrs = ch->nrs[jred];
+    const int64_t *map = ch->rs2lt + ch->rs_off[jred];
+    int64_t nvec = nvec_batch < nvrs ? nvec_batch : nvrs;
+    if (nvec < 1) return 33; /* :104-111 */
+    double *drs = calloc((size_t)nrs, sizeof(double));
+    double *frs = calloc((size_t)nrs, sizeof(double));
+    double *lrs = malloc(sizeof(double) * (size_t)nrs * nvec);
+    double *vj = malloc(sizeof(double) * (size_t)nvec);
+    ora_swap_full2rs(nrs, map, 1, np, dlt, drs); /* :122 */
+    int64_t nbatch = (nvrs - 1) / nvec + 1;
+    for (int64_t ib = 0; ib < nbatch; ++ib) {
+      int64_t jnum = (ib == nbatch - 1) ? nvrs - nvec * (nbatch - 1) : nvec;
+      int64_t jvec = nvec * ib + ivrs;
+      int64_t numv = vec_read(ch, jvec, jnum, nrs, lrs);
+      if (numv <= 0 || numv != jnum) {
+        free(drs); free(frs); free(lrs); free(vj);
+        return 77;
+      }
+      ora_dgemv('T', nrs, jnum, 1.0, lrs, nrs, drs, 0.0, vj); /* :167 */
+      double xfac = (jvec - ivrs) < 1 ? (double)(jvec - ivrs) : 1.0;
+      ora_dgemv('N', nrs, jnum, 1.0, lrs, nrs, vj, xfac, frs); /* :174 */
+    }
+    ora_swap_rs2full(nrs, map, 1, np, frs, flt, jred > jred1); /* :188-192 */
+    free(drs); free(frs); free(lrs); free(vj);
+  }
+  return 0;
+}
+
 /* src/fock_util/cho_lk_scf.F90:14-1103 with nSym = 1 and every shell pair and
  * orbital kept (no LK screening: the limit dmpk -> 0 of the reference).
  *   FLT(d) += J[PLT(1)] + KLT(d),  KLT(d) += -FactXI * sum_{J,k} L(a,J)[k] L(b,J)[k]

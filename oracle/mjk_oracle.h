@@ -72,6 +72,9 @@ int32_t ora_cho_fscf(const ora_chol *ch, int32_t nden, double *flt,
                      const double *const *porb, const double *dlt,
                      double exfac, int64_t nvec_batch);
 
+int32_t ora_cho_fock_dft_red(const ora_chol *ch, const double *dlt, double *flt,
+                             int64_t nvec_batch);
+
 int32_t ora_cho_lk_scf(const ora_chol *ch, int32_t nden, double *flt, double *klt,
                        const int32_t *nforb, const int32_t *niorb,
                        const double *const *porb, const double *plt, double factxi,

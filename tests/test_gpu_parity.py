@@ -198,6 +198,31 @@ def test_cho_lk_scf_dropin(gpu, nden, shells):
     ctx.close()
 
 
+def test_cho_fock_dft_red_dropin(gpu):
+    """Coulomb-only drop-in over two reduced sets; FLT is overwritten."""
+    rng = np.random.default_rng(60)
+    n, M1, M2 = 45, 14, 11
+    npr = npair(n)
+    s0 = np.sort(rng.choice(np.arange(npr), size=400, replace=False))
+    s1 = np.arange(npr)
+    L0 = rng.standard_normal((M1, len(s0))) * 0.2
+    L1 = chol_vectors_np(n, M2, seed=61)
+    st = oracle.CholStore(n, [s0, s1], [(0, v) for v in L0] + [(1, v) for v in L1])
+    C = orbitals_np(n, 6, seed=62)
+    dlt = fold(2.0 * C @ C.T)
+    flt0 = rng.standard_normal(npr)
+    want = oracle.cho_fock_dft_red(st, dlt, flt0, batch=5)
+    ctx = JKContext(n, M1 + M2, device=gpu)
+    ctx.set_reduced_set(0, s0)
+    ctx.set_reduced_set(1, s1)
+    ctx.upload_vectors(0, 0, L0)
+    ctx.upload_vectors(1, M1, L1)
+    flt = flt0.copy()
+    ctx.cho_fock_dft_red(dlt, flt)
+    assert rel(flt, want) < RTOL
+    ctx.close()
+
+
 def test_cho_fmcscf_dropin_with_puvx(gpu):
     n, M = 60, 35
     nk, na = 7, 5

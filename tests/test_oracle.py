@@ -172,6 +172,24 @@ def test_cho_fscf_rejects_bad_nden():
                         np.zeros(npair(n)), 1.0)
 
 
+def test_cho_fock_dft_red_overwrites_flt():
+    rng = np.random.default_rng(90)
+    n, M1, M2 = 12, 10, 6
+    npr = npair(n)
+    s0 = np.sort(rng.choice(np.arange(npr), size=50, replace=False))   # first set: a subset
+    s1 = np.arange(npr)
+    L0 = rng.standard_normal((M1, len(s0))) * 0.3
+    L1 = chol_vectors_np(n, M2, seed=91)
+    st = oracle.CholStore(n, [s0, s1], [(0, v) for v in L0] + [(1, v) for v in L1])
+    C = orbitals_np(n, 4, seed=92)
+    dlt = fold(2.0 * C @ C.T)
+    flt = rng.standard_normal(npr)
+    got = oracle.cho_fock_dft_red(st, dlt, flt, batch=4)
+    J, _ = coulomb(lfull(st.packed(), n), unpack(dlt, n) * np.where(np.eye(n) > 0, 1.0, 0.5))
+    # swap_rs2full with add = .false. zeroes all of FLT before the first set
+    assert rel(got, pack(J)) < TOL
+
+
 def _shell_pair_map(nbassh):
     """LT index (a >= b) -> position in the shell-pair-blocked KLT of
     src/fock_util/cho_lk_scf.F90:274-292 (pairs iaSh >= ibSh in order; an
