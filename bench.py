@@ -123,18 +123,15 @@ def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
 
 
 def ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, t64, res, steps):
-    """Precision ladder: kappa per mode from the full-precision-mode energy
-    errors just measured, split points planned for |dE| <= 1e-6 Eh, then the
-    ladder build timed and its energy checked against FP64."""
+    """Precision ladder: split points calibrated, planned and verified for
+    |dE| <= 1e-6 Eh (openmolcas_amd.ladder.tune), then the ladder build timed
+    and its energy checked against FP64 with the same energy expression."""
     import torch
 
     from openmolcas_amd import ladder
 
-    w = ctx.vector_norms()
-    total = float(torch.sum(w).item())
-    k32 = max(res["fp32"]["dE"], 1e-300) / total
-    kb3 = max(res["bf16x3"]["dE"], 1e-300) / total
-    plan = ladder.plan(w, k32, kb3, tol=1e-6)
+    # energy_of uses G = J - 1/2 K and E = h.D + 1/2 G.D: dE = 1/4 dK.D
+    plan = ladder.tune(ctx, [C], dlt, factor=0.25, tol=1e-6)
     K = ctx.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -143,9 +140,9 @@ def ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, t64, res, steps):
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / steps
     e = energy_of(ctx, dlt, h, J, K)
-    return {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "nchol": int(w.numel()),
-            "predicted_dE": plan.predicted, "dE": abs(e - e64),
-            "exchange_ms": round(1e3 * t, 4), "exchange_speedup_vs_fp64": round(t64 / t, 3)}
+    return {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "nchol": int(ctx.nchol),
+            "dE": abs(e - e64), "exchange_ms": round(1e3 * t, 4),
+            "exchange_speedup_vs_fp64": round(t64 / t, 3)}
 
 
 def profiled_traffic(kernel_prefix, n, M, o, precision):
@@ -224,9 +221,7 @@ def extra_workloads(dev):
             res[f"dEdC_maxabs_err_{prec}"] = float(torch.max(torch.abs(dC_mixed - dC64)).item())
     # precision ladder planned from the measured full-mode errors (|dE| <= 1e-6 Eh)
     from openmolcas_amd import ladder
-    wv = jk.vector_norms()
-    tot = float(torch.sum(wv).item())
-    plan = ladder.plan(wv, max(res["dE_fp32"], 1e-300) / tot, max(res["dE_bf16x3"], 1e-300) / tot)
+    plan = ladder.tune(jk, [C], dlt, factor=0.5, tol=1e-6)  # G = J - K[C]: dE = 1/2 dK.D
     t = _timeit(lambda: jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16), 2)
     K = jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
     Gp = torch.zeros_like(J)

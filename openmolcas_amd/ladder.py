@@ -12,7 +12,11 @@ kappa_mode is measured once per system (``calibrate``: one exchange build in
 each precision against FP64); ``plan`` then places the split points so each
 reduced-precision segment stays below a share of the energy tolerance, and
 ``mjk_exchange_mixed`` runs [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32 and
-[j_bf16, nchol) in split-BF16.
+[j_bf16, nchol) in split-BF16.  Rounding errors of a full-precision-mode build
+partly cancel, so the linear model can under-predict a tail's error;
+``tune`` therefore builds the planned ladder, measures its energy error
+against FP64 and tightens the budget until the measured error is within the
+acceptance share of the tolerance.
 
 The reference has no mixed-precision exchange: src/fock_util/cho_fscf.F90:
 250,276 runs every vector in FP64.
@@ -30,6 +34,7 @@ class Ladder:
     j_fp32: int
     j_bf16: int
     predicted: float  # predicted |dE| of the two reduced-precision segments
+    measured: float = float("nan")  # |dE| of this ladder against FP64, when verified
 
 
 def tail_sums(w) -> np.ndarray:
@@ -80,3 +85,24 @@ def calibrate(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, w=None):
         de = exchange_energy_error(k, k64, dlt, factor)
         kappas.append(max(de, 1e-300) / max(total, 1e-300))
     return kappas[0], kappas[1], w, k64
+
+
+def tune(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, tol: float = 1e-6,
+         share: float = 0.25, accept: float = 0.5, rounds: int = 6) -> Ladder:
+    """Calibrate, plan and verify: returns a ladder whose measured |dE|
+    against the FP64 build is at most accept * tol (FP64 throughout if no
+    ladder within ``rounds`` tightening steps qualifies)."""
+    k32, kb3, w, k64 = calibrate(jk, corbs, dlt, factor)
+    m = int(w.numel())
+    s = share
+    for _ in range(rounds):
+        p = plan(w, k32, kb3, tol, s)
+        if p.j_fp32 >= m:
+            break
+        k = jk.exchange_mixed(corbs, p.j_fp32, p.j_bf16)
+        de = exchange_energy_error(k, k64, dlt, factor)
+        p.measured = de
+        if de <= accept * tol:
+            return p
+        s *= 0.5 * accept * tol / max(de, 1e-300)
+    return Ladder(m, m, 0.0, 0.0)
