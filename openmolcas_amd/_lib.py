@@ -82,7 +82,7 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(
-                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                f"{LIB_PATH} is missing: build it with `python -m openmolcas_amd.build`"
                 " (the J/K build has no CPU fallback)")
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():

@@ -1,6 +1,6 @@
 """Builds libmolcas_jk.so in-tree with hipcc for gfx950.
 
-Usage: ``python -m openmolcas_amd.build`` (also called by __graft_entry__.build()).
+Usage: ``python -m openmolcas_amd.build``.
 """
 from __future__ import annotations
 

@@ -127,8 +127,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
   constexpr int CN = BK * LDC / kThreads;
   constexpr int LN = BK * TA / kThreads;
   static_assert(BK * LDC % kThreads == 0 && BK * TA % kThreads == 0, "tile shape");
-  __shared__ T sC[BK * LDC];
-  __shared__ T sL[BK * SLD];
+  __shared__ T sC[2][BK * LDC];
+  __shared__ T sL[2][BK * SLD];
   const int64_t lb = xcd_logical(nblocks);
   if (lb < 0) return;
   const int64_t jr = lb / ntile_a;
@@ -147,7 +147,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     // Element i of a thread: "row-major" chunks (b > a and diagonal blocks)
     // walk e -> (bb = e / TA, aa = e % TA); "column" chunks (b < a, where the
     // packed row a holds b contiguous) walk e -> (bb = e % BK, aa = e / BK).
-    auto fetch = [&](int b0, bool colmajor) {
+    auto fetch = [&](int b0) {
+      const bool colmajor = b0 + BK <= a0;
       const T *__restrict__ cs = Cb + int64_t(b0) * LDC;
 #pragma unroll
       for (int i = 0; i < CN; ++i) rc[i] = cs[tid + kThreads * i];
@@ -163,34 +164,51 @@ __global__ __launch_bounds__(kThreads) void k_ht(
         rl[i] = ok ? v : T(0);
       }
     };
-    auto is_col = [&](int b0) { return b0 + BK <= a0; };
-    fetch(0, is_col(0));
-    for (int b0 = 0; b0 < n; b0 += BK) {
-      const bool colmajor = is_col(b0);
-      if (b0) __syncthreads();  // every wave is done with the previous chunk
+    auto stash = [&](int b0, int buf) {
+      const bool colmajor = b0 + BK <= a0;
 #pragma unroll
-      for (int i = 0; i < CN; ++i) sC[tid + kThreads * i] = rc[i];
+      for (int i = 0; i < CN; ++i) sC[buf][tid + kThreads * i] = rc[i];
 #pragma unroll
       for (int i = 0; i < LN; ++i) {
         const int e = tid + kThreads * i;
         const int bb = colmajor ? e % BK : e / TA, aa = colmajor ? e / BK : e % TA;
-        sL[bb * SLD + aa] = rl[i];
+        sL[buf][bb * SLD + aa] = rl[i];
       }
-      __syncthreads();
-      if (b0 + BK < n) fetch(b0 + BK, is_col(b0 + BK));
+    };
+    // Double-buffered LDS: chunk c is multiplied out of buffer c&1 while
+    // chunk c+1 is stored into the other buffer and chunk c+2 is fetched to
+    // registers; one barrier per chunk.  The operand fragments of the next
+    // 4-row step are read while the current step's MFMAs run.
+    const int nch = (n + BK - 1) / BK;
+    fetch(0);
+    stash(0, 0);
+    if (nch > 1) fetch(BK);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+      const int cur = c & 1;
+      const T *__restrict__ sc = sC[cur];
+      const T *__restrict__ sl = sL[cur];
+      T af[2][OT], bf[2][NB];
+      auto ld = [&](int ks, int q) {
+        const int kr = ks * 4 + (lane >> 4);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) bf[q][j] = sl[kr * SLD + w * (TA / 4) + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int t = 0; t < OT; ++t) af[q][t] = sc[kr * LDC + 16 * t + (lane & 15)];
+      };
+      ld(0, 0);
 #pragma unroll
       for (int ks = 0; ks < BK / 4; ++ks) {
-        const int kr = ks * 4 + (lane >> 4);
-        T bf[NB];
+        const int q = ks & 1;
+        if (ks + 1 < BK / 4) ld(ks + 1, q ^ 1);
 #pragma unroll
-        for (int j = 0; j < NB; ++j) bf[j] = sL[kr * SLD + w * (TA / 4) + j * 16 + (lane & 15)];
+        for (int t = 0; t < OT; ++t)
 #pragma unroll
-        for (int t = 0; t < OT; ++t) {
-          const T af = sC[kr * LDC + 16 * t + (lane & 15)];
-#pragma unroll
-          for (int j = 0; j < NB; ++j) acc[t][j] = P::mma(af, bf[j], acc[t][j]);
-        }
+          for (int j = 0; j < NB; ++j) acc[t][j] = P::mma(af[q][t], bf[q][j], acc[t][j]);
       }
+      if (c + 1 < nch) stash((c + 1) * BK, cur ^ 1);
+      if (c + 2 < nch) fetch((c + 2) * BK);
+      __syncthreads();
     }
   }
   T *__restrict__ xj = X + jr * LDC * int64_t(n_pad);
@@ -222,8 +240,8 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   constexpr int MT = WT / 16;
   constexpr int PN = BKF * TM / kThreads;
   static_assert(BKF * TM % kThreads == 0, "tile shape");
-  __shared__ T sA[BKF * TM];
-  __shared__ T sB[BKF * TM];
+  __shared__ T sA[2][BKF * TM];
+  __shared__ T sB[2][BKF * TM];
   const int set = find_set(sets, blockIdx.x);
   const int local = blockIdx.x - sets.item0[set];
   const int tile = local % ntiles;
@@ -284,37 +302,56 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
       rb[i] = ok ? vb : T(0);
     }
   };
-  if (nf > 0) fetch(0);
-  for (int f0 = 0; f0 < nf; f0 += BKF) {
-    if (f0) __syncthreads();
+  auto stash = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < PN; ++i) sA[tid + kThreads * i] = ra[i];
+    for (int i = 0; i < PN; ++i) sA[buf][tid + kThreads * i] = ra[i];
     if (!diag) {
 #pragma unroll
-      for (int i = 0; i < PN; ++i) sB[tid + kThreads * i] = rb[i];
+      for (int i = 0; i < PN; ++i) sB[buf][tid + kThreads * i] = rb[i];
     }
-    __syncthreads();
-    if (f0 + BKF < nf) fetch(f0 + BKF);
+  };
+  // Double-buffered LDS: chunk c is multiplied out of buffer c&1 while chunk
+  // c+1 is stored into the other buffer and chunk c+2 is fetched to
+  // registers; one barrier per chunk.  Operand fragments of the next
+  // 4-row step are read from LDS while the current step's MFMAs run.
+  const int nch = (nf + BKF - 1) / BKF;
+  if (nch > 0) {
+    fetch(0);
+    stash(0);
+  }
+  if (nch > 1) fetch(BKF);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    const int cur = c & 1;
     if (!idle) {
-      const T *__restrict__ sb = diag ? sA : sB;
+      const T *__restrict__ sa = sA[cur];
+      const T *__restrict__ sb = diag ? sA[cur] : sB[cur];
+      T af[2][MT], bf[2][MT];
+      auto ld = [&](int ks, int q) {
+        const int kr = ks * 4 + (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < MT; ++i) af[q][i] = sa[kr * TM + wm * WT + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < MT; ++j) bf[q][j] = sb[kr * TM + wn * WT + j * 16 + (lane & 15)];
+      };
+      ld(0, 0);
 #pragma unroll
       for (int ks = 0; ks < BKF / 4; ++ks) {
-        const int kr = ks * 4 + (lane >> 4);
-        T af[MT], bf[MT];
-#pragma unroll
-        for (int i = 0; i < MT; ++i) af[i] = sA[kr * TM + wm * WT + i * 16 + (lane & 15)];
-#pragma unroll
-        for (int j = 0; j < MT; ++j) bf[j] = sb[kr * TM + wn * WT + j * 16 + (lane & 15)];
+        const int q = ks & 1;
+        if (ks + 1 < BKF / 4) ld(ks + 1, q ^ 1);
 #pragma unroll
         for (int i = 0; i < MT; ++i)
 #pragma unroll
           for (int j = 0; j < MT; ++j)
-            if (!diag || wm > wn || i >= j) acc[i][j] = P::mma(af[i], bf[j], acc[i][j]);
+            if (!diag || wm > wn || i >= j) acc[i][j] = P::mma(af[q][i], bf[q][j], acc[i][j]);
       }
     }
     if constexpr (P::kFlush) {
-      if ((f0 + BKF) % kFlushRows == 0) flush();
+      if (((c + 1) * BKF) % kFlushRows == 0) flush();
     }
+    if (c + 1 < nch) stash(cur ^ 1);
+    if (c + 2 < nch) fetch((c + 2) * BKF);
+    __syncthreads();
   }
   flush();
   // slot (chunk, slice): every chunk of vectors writes its own partial tiles
@@ -641,28 +678,32 @@ void launch_ht(Context *c, const typename P::T *L, const typename P::T *cb, int6
                      c->ldl, c->n, c->n_pad, cb, j0, ntile_a, nblocks, X);
 }
 
-// AO tile width: 128 while the accumulators fit (ot <= 6), 64 beyond; the
-// context's ht_ta overrides (autotuner).
+// AO tile width: FP64 defaults to 64 (more workgroups per CU); FP32 uses 128
+// while the accumulators fit (ot <= 6).  The context's ht_ta overrides
+// (autotuner).
+template <class P>
 inline int ht_tile(const Context *c, int ot) {
   if (c->ht_ta == 64) return 64;
   if (c->ht_ta == 128 && ot <= 6) return 128;
+  if (std::is_same<P, F64>::value) return 64;
   return ot <= 6 ? 128 : 64;
 }
 
 template <class P>
 void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T *cb, int64_t j0,
                  int64_t jn, typename P::T *X) {
-  const bool wide = ht_tile(c, ot) == 128;
+  // BK keeps the two LDS buffers within 64 KiB, so two or more workgroups fit a CU
+  const bool wide = ht_tile<P>(c, ot) == 128;
   switch (ot) {
-    case 1: wide ? launch_ht<P, 1, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 1, 64, 32>(c, L, cb, j0, jn, X); break;
-    case 2: wide ? launch_ht<P, 2, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 2, 64, 32>(c, L, cb, j0, jn, X); break;
-    case 3: wide ? launch_ht<P, 3, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 3, 64, 32>(c, L, cb, j0, jn, X); break;
-    case 4: wide ? launch_ht<P, 4, 128, 32>(c, L, cb, j0, jn, X) : launch_ht<P, 4, 64, 32>(c, L, cb, j0, jn, X); break;
-    case 6: wide ? launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 6, 64, 32>(c, L, cb, j0, jn, X); break;
-    case 8: launch_ht<P, 8, 64, 32>(c, L, cb, j0, jn, X); break;
-    case 10: launch_ht<P, 10, 64, 32>(c, L, cb, j0, jn, X); break;
-    case 12: launch_ht<P, 12, 64, 16>(c, L, cb, j0, jn, X); break;
-    default: launch_ht<P, 16, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 1: wide ? launch_ht<P, 1, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 1, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 2: wide ? launch_ht<P, 2, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 2, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 3: wide ? launch_ht<P, 3, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 3, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 4: wide ? launch_ht<P, 4, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 4, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 6: wide ? launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 6, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 8: launch_ht<P, 8, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 10: launch_ht<P, 10, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 12: launch_ht<P, 12, 64, 8>(c, L, cb, j0, jn, X); break;
+    default: launch_ht<P, 16, 64, 8>(c, L, cb, j0, jn, X); break;
   }
 }
 
@@ -688,7 +729,7 @@ void launch_ht_b3(Context *c, const float *L, const __bf16 *ch, const __bf16 *cl
 
 void ht_dispatch_b3(Context *c, int ot, const float *L, const __bf16 *ch, const __bf16 *cl,
                     int64_t j0, int64_t jn, float *X) {
-  const bool wide = ht_tile(c, ot) == 128;
+  const bool wide = ht_tile<F32>(c, ot) == 128;
   switch (ot) {
     case 1: wide ? launch_ht_b3<1, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<1, 64>(c, L, ch, cl, j0, jn, X); break;
     case 2: wide ? launch_ht_b3<2, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<2, 64>(c, L, ch, cl, j0, jn, X); break;

@@ -38,4 +38,4 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_is_built():
     # build() runs before the CPU suite; a missing library is a build failure
-    assert os.path.exists(_lib.LIB_PATH), "run __graft_entry__.build() first"
+    assert os.path.exists(_lib.LIB_PATH), "build the library first: python -m openmolcas_amd.build"
