@@ -178,6 +178,23 @@ int mjk_cho_fmcscf(mjk_context *ctx, double *flt, int32_t nforb, int32_t niorb,
 int mjk_cd_incore(mjk_context *ctx, int32_t n, double *x, double *vec, int32_t mxvec,
                   int32_t *numcho, double thr, int32_t *irc);
 
+/* CHO_LK_CASSCF (src/fock_util/cho_lk_casscf.F90:14), nSym = 1: the LK form
+ * of CHO_FMCSCF,
+ *   FI += J[DI] + FactXI*ExFac * K[MSQ],  FA += J[DA] - 1/2*ExFac * K[Ash(2)]
+ * (FactX of cho_lk_casscf.F90:113; FA only if do_active), W_PWXY from the
+ * active MOs ash1 (orbital-major A2(naorb,nBas)) and cmo (CMO(n,n),
+ * column-major, p over norb MOs after nfro).  msq and ash2 are square
+ * A2(nBas,nBas) with the AO index fastest (first nfiorb / nchm columns used).
+ * dlt = [DI, DA], flt = [FI, FA].  Every shell pair and orbital is kept (the
+ * reference's result for dmpk -> 0).  Called from
+ * src/fock_util/cho_cas_drv.F90:285. */
+int mjk_cho_lk_casscf(mjk_context *ctx, const double *dlt, double *flt, const double *msq,
+                      double *w_pwxy, double factxi, int32_t nfiorb, int32_t naorb,
+                      int32_t nchm, const double *ash1, const double *ash2,
+                      int32_t do_active, int32_t nscreen, double dmpk, double dfmat,
+                      const double *cmo, int32_t nfro, int32_t norb, double exfac,
+                      int32_t precision);
+
 /* ---- device engine (device pointers, asynchronous on the context stream) */
 
 /* Coulomb for nd folded LT densities dlt(d*ldd + p): jlt(d*ldj + p) =

@@ -590,13 +590,16 @@ int mjk_cho_fock_dft_red(mjk_context *c, const double *dlt, double *flt) {
   return hip_check(c, hipStreamSynchronize(c->stream), "cho_fock_dft_red sync");
 }
 
-int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,
-                   double factxi, const double *dlt, int32_t do_active, const double *porb1,
-                   int32_t nchm, const double *porb2, const double *porb3, double *w_pwxy,
-                   const double *cmo, int32_t nfro, int32_t norb, double exfac,
-                   int32_t precision) {
-  (void)exfac;  // the reference only prints a warning for ExFac /= 1 (cho_fmcscf.F90:88-94)
-  if (!c) return MJK_ERR_ARG;
+}  // extern "C"
+
+// FI += J[DI] + fk_i * K[porb1], FA += J[DA] + fk_a * K[porb2] (if do_active),
+// W_PWXY from porb3 and cmo: the shared body of the CASSCF drop-ins.
+// porb1/porb2/porb3 are host arrays in the orbital-major layout A2(nk,nBas).
+static int fmcscf_core(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,
+                       double fk_i, double fk_a, const double *dlt, int32_t do_active,
+                       const double *porb1, int32_t nchm, const double *porb2,
+                       const double *porb3, double *w_pwxy, const double *cmo, int32_t nfro,
+                       int32_t norb, int32_t precision) {
   if (!flt || !dlt) return set_error(c, MJK_ERR_ARG, "cho_fmcscf: null pointer");
   if (nforb < 0 || niorb < 0 || naorb < 0 || nchm < 0)
     return set_error(c, MJK_ERR_ARG, "cho_fmcscf: negative orbital count");
@@ -634,7 +637,7 @@ int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, in
   if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
   const int32_t jsrc[2] = {0, 1};
   const double fj[2] = {1.0, 1.0};        // FactCI, FactCA (cho_fmcscf.F90:71)
-  const double fk[2] = {factxi, -0.5};    // FactXI, FactXA
+  const double fk[2] = {fk_i, fk_a};      // FactXI, FactXA
   if ((rc = fock_update(c, nden, dF, np, dJ, np, jsrc, fj, dK, np, fk))) return rc;
   if (do_puvx) {
     if ((rc = h2d(dC3, porb3, size_t(naorb * n), "copy POrb(3)")) ||
@@ -649,6 +652,47 @@ int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, in
                                         hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
     return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "cho_fmcscf sync");
+}
+
+extern "C" {
+
+int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,
+                   double factxi, const double *dlt, int32_t do_active, const double *porb1,
+                   int32_t nchm, const double *porb2, const double *porb3, double *w_pwxy,
+                   const double *cmo, int32_t nfro, int32_t norb, double exfac,
+                   int32_t precision) {
+  (void)exfac;  // the reference only prints a warning for ExFac /= 1 (cho_fmcscf.F90:88-94)
+  if (!c) return MJK_ERR_ARG;
+  return fmcscf_core(c, flt, nforb, niorb, naorb, factxi, -0.5, dlt, do_active, porb1, nchm, porb2,
+                     porb3, w_pwxy, cmo, nfro, norb, precision);  // FactXA = -1/2, cho_fmcscf.F90:74
+}
+
+int mjk_cho_lk_casscf(mjk_context *c, const double *dlt, double *flt, const double *msq,
+                      double *w_pwxy, double factxi, int32_t nfiorb, int32_t naorb, int32_t nchm,
+                      const double *ash1, const double *ash2, int32_t do_active, int32_t nscreen,
+                      double dmpk, double dfmat, const double *cmo, int32_t nfro, int32_t norb,
+                      double exfac, int32_t precision) {
+  // Every shell pair and orbital is kept (the reference's result for dmpk -> 0).
+  (void)nscreen;
+  (void)dfmat;
+  if (!c) return MJK_ERR_ARG;
+  if (dmpk < 0.0) return set_error(c, MJK_ERR_ARG, "cho_lk_casscf: dmpk < 0");
+  const int64_t n = c->n;
+  if (nfiorb < 0 || nfiorb > n || nchm < 0 || nchm > n)
+    return set_error(c, MJK_ERR_ARG, "cho_lk_casscf: bad orbital count");
+  const int32_t nk2 = do_active ? nchm : 0;
+  if ((nfiorb > 0 && !msq) || (nk2 > 0 && !ash2))
+    return set_error(c, MJK_ERR_ARG, "cho_lk_casscf: null orbital matrix");
+  // MSQ and Ash(2) are square A2(nBas,nBas) with the AO index fastest; the
+  // engine reads the orbital-major A2(nk,nBas) layout
+  std::vector<double> p1(static_cast<size_t>(nfiorb) * n), p2(static_cast<size_t>(nk2) * n);
+  for (int64_t k = 0; k < nfiorb; ++k)
+    for (int64_t b = 0; b < n; ++b) p1[size_t(k + nfiorb * b)] = msq[b + n * k];
+  for (int64_t k = 0; k < nk2; ++k)
+    for (int64_t b = 0; b < n; ++b) p2[size_t(k + nk2 * b)] = ash2[b + n * k];
+  // FactX = [FactXI*ExFac, -1/2*ExFac], cho_lk_casscf.F90:113
+  return fmcscf_core(c, flt, 0, nfiorb, naorb, factxi * exfac, -0.5 * exfac, dlt, do_active,
+                     p1.data(), nchm, p2.data(), ash1, w_pwxy, cmo, nfro, norb, precision);
 }
 
 }  // extern "C"

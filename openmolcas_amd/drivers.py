@@ -9,10 +9,10 @@ on top of the drop-ins, with the density decomposition (DECO) on the device.
   MO columns are used with the full ExFac.  UHF sends the total density to
   Coulomb and the alpha/beta densities to exchange with FactX = ExFac
   (:276-298).
-* ``cho_cas_drv`` follows src/fock_util/cho_cas_drv.F90:14-330 (ALGO 1, no
-  LK): folds DI and DA (:104-105), FactXI = -1, or -1/2 with DECO of DI at
-  Thr = 1e-12 (:21,107-129); the active density is always decomposed
-  (:202-227); then CHO_FMCSCF on zeroed FI/FA (:268-277).
+* ``cho_cas_drv`` follows src/fock_util/cho_cas_drv.F90:14-330: folds DI and
+  DA (:104-105), FactXI = -1, or -1/2 with DECO of DI at Thr = 1e-12
+  (:21,107-129); the active density is always decomposed (:202-227); then
+  CHO_FMCSCF (ALGO 1) or CHO_LK_CASSCF (LK) on zeroed FI/FA (:268-286).
 
 All arrays are host numpy arrays (the reference's calling convention);
 densities are square (n, n), CMO is (n, n) with CMO[a, p] = CMO(a,p).
@@ -87,7 +87,8 @@ def choscf_fock(jk: JKContext, dsq, exfac: float = 1.0, algo: int = 3, deco: boo
 
 
 def cho_cas_drv(jk: JKContext, cmo, di_sq, da_sq, nfro: int, nish: int, nash: int,
-                deco: bool = True, do_active: bool = True, exfac: float = 1.0, precision="fp64"):
+                deco: bool = True, do_active: bool = True, exfac: float = 1.0, lk: bool = False,
+                precision="fp64"):
     """Returns (FI, FA, W_PWXY): FI/FA in LT storage (npair,), W_PWXY for the
     active orbitals nfro+nish .. nfro+nish+nash-1 over all norb = nmo - nfro
     MOs after the frozen ones."""
@@ -113,7 +114,11 @@ def cho_cas_drv(jk: JKContext, cmo, di_sq, da_sq, nfro: int, nish: int, nash: in
     norb = cmo.shape[1] - nfro
     nxy = nash * (nash + 1) // 2
     w = np.zeros(max(1, norb * nash * nxy))
-    jk.cho_fmcscf(flt, 0, porb1.shape[1], nash, factxi, dlt, do_active,
-                  np.ascontiguousarray(porb1), porb2.shape[1], np.ascontiguousarray(porb2),
-                  np.ascontiguousarray(porb3), w, cmo, nfro, norb, exfac, precision)
+    if lk:  # DoLocK: CHO_LK_CASSCF with MSQ and CVa (cho_cas_drv.F90:281-286)
+        jk.cho_lk_casscf(dlt, flt, porb1, w, factxi, porb1.shape[1], nash, porb2.shape[1], porb3,
+                         porb2, do_active, cmo, nfro, norb, exfac=exfac, precision=precision)
+    else:   # ALGO 1: CHO_FMCSCF (cho_cas_drv.F90:273-277)
+        jk.cho_fmcscf(flt, 0, porb1.shape[1], nash, factxi, dlt, do_active,
+                      np.ascontiguousarray(porb1), porb2.shape[1], np.ascontiguousarray(porb2),
+                      np.ascontiguousarray(porb3), w, cmo, nfro, norb, exfac, precision)
     return flt[0], flt[1], w[:norb * nash * nxy]

@@ -203,6 +203,37 @@ class JKContext:
         self._check(rc)
         return rc
 
+    def cho_lk_casscf(self, dlt: np.ndarray, flt: np.ndarray, msq, w_pwxy, factxi: float,
+                      nfiorb: int, naorb: int, nchm: int, ash1, ash2, do_active: bool, cmo,
+                      nfro: int, norb: int, nscreen: int = 10, dmpk: float = 0.1,
+                      dfmat: float = 0.0, exfac: float = 1.0, precision="fp64") -> int:
+        """CHO_LK_CASSCF(DLT,FLT,MSQ,W_PWXY,FactXI,nFIorb,nAorb,nChM,Ash,DoActive,
+        nScreen,dmpk,dFmat,CMO,ExFac), nSym = 1, without screening.  flt (2, npair)
+        updated in place; msq (nbas, >= nfiorb) and ash2 (nbas, >= nchm) are AO x
+        vector matrices; ash1 (nbas, naorb) the active MOs; cmo (nbas, nbas)."""
+        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
+            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        n = self.nbas
+
+        def square(a, k):  # A2(nBas,nBas), AO index fastest
+            sq = np.zeros((n, n))
+            if a is not None and k > 0:
+                sq[:, :k] = _host_f64(a)[:, :k]
+            return np.ascontiguousarray(sq.T)
+
+        m = square(msq, nfiorb)
+        a2 = square(ash2, nchm)
+        a1 = _host_f64(ash1) if ash1 is not None and naorb > 0 else np.zeros(1)
+        cm = _host_f64(np.asarray(cmo).T) if cmo is not None else np.zeros(1)
+        d = _host_f64(dlt)
+        rc = self.lib.mjk_cho_lk_casscf(self._h, _ptr(d), _ptr(flt), _ptr(m), _ptr(w_pwxy),
+                                        float(factxi), int(nfiorb), int(naorb), int(nchm),
+                                        _ptr(a1), _ptr(a2), int(bool(do_active)), int(nscreen),
+                                        float(dmpk), float(dfmat), _ptr(cm), int(nfro), int(norb),
+                                        float(exfac), precision_code(precision))
+        self._check(rc)
+        return rc
+
     def cho_fmcscf(self, flt: np.ndarray, nforb: int, niorb: int, naorb: int, factxi: float,
                    dlt: np.ndarray, do_active: bool, porb1, nchm: int, porb2, porb3,
                    w_pwxy, cmo, nfro: int, norb: int, exfac: float = 1.0,

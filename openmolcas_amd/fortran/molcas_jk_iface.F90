@@ -15,7 +15,7 @@ module molcas_jk_iface
   private
   public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_cho_fscf, &
             mjk_cho_fmcscf, mjk_cho_lk_scf, mjk_cho_fock_dft_red, cho_fscf_gpu, cho_lk_scf_gpu, &
-            cho_fock_dft_red_gpu, cho_fmcscf_gpu, jk_gpu_init, jk_gpu_final
+            cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_lk_casscf, cho_lk_casscf_gpu, jk_gpu_init, jk_gpu_final
 
   integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1
   type(c_ptr), save, public :: jk_ctx = c_null_ptr
@@ -56,6 +56,16 @@ module molcas_jk_iface
       real(c_double), intent(in) :: porb(*), dlt(*)
       real(c_double), value :: exfac
     end function mjk_cho_fscf
+    integer(c_int) function mjk_cho_lk_casscf(ctx, dlt, flt, msq, w_pwxy, factxi, nfiorb, naorb, nchm, &
+        ash1, ash2, do_active, nscreen, dmpk, dfmat, cmo, nfro, norb, exfac, prec) &
+        bind(C, name='mjk_cho_lk_casscf')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: dlt(*), msq(*), ash1(*), ash2(*), cmo(*)
+      real(c_double), intent(inout) :: flt(*), w_pwxy(*)
+      real(c_double), value :: factxi, dmpk, dfmat, exfac
+      integer(c_int32_t), value :: nfiorb, naorb, nchm, do_active, nscreen, nfro, norb, prec
+    end function mjk_cho_lk_casscf
     integer(c_int) function mjk_cho_fock_dft_red(ctx, dlt, flt) bind(C, name='mjk_cho_fock_dft_red')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
@@ -147,6 +157,24 @@ contains
                             int(nShell, c_int32_t), sh, MJK_PREC_FP64))
     deallocate(sh)
   end subroutine cho_lk_scf_gpu
+
+  ! CHO_LK_CASSCF(DLT,FLT,MSQ,W_PWXY,FactXI,nFIorb,nAorb,nChM,Ash,DoActive,nScreen,dmpk,dFmat,CMO,ExFac),
+  ! nSym = 1, unscreened; Ash1 = active MOs A2(nAorb,nBas), Ash2 = DA vectors A2(nBas,nBas).
+  subroutine cho_lk_casscf_gpu(rc, DLT, FLT, MSQ, W_PWXY, FactXI, nFIorb, nAorb, nChM, Ash1, Ash2, DoActive, &
+                               nScreen, dmpk, dFmat, CMO, nFro, nOrb, ExFac)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nFIorb, nAorb, nChM, nScreen, nFro, nOrb
+    logical, intent(in) :: DoActive
+    real(8), intent(in) :: DLT(*), MSQ(*), Ash1(*), Ash2(*), CMO(*), FactXI, dmpk, dFmat, ExFac
+    real(8), intent(inout) :: FLT(*), W_PWXY(*)
+    integer(c_int32_t) :: act
+    act = 0
+    if (DoActive) act = 1
+    rc = int(mjk_cho_lk_casscf(jk_ctx, DLT, FLT, MSQ, W_PWXY, real(FactXI, c_double), int(nFIorb, c_int32_t), &
+                               int(nAorb, c_int32_t), int(nChM, c_int32_t), Ash1, Ash2, act, int(nScreen, c_int32_t), &
+                               real(dmpk, c_double), real(dFmat, c_double), CMO, int(nFro, c_int32_t), &
+                               int(nOrb, c_int32_t), real(ExFac, c_double), MJK_PREC_FP64))
+  end subroutine cho_lk_casscf_gpu
 
   ! CHO_FMCSCF(rc,FLT,nForb,nIorb,nAorb,FactXI,DLT,DoActive,POrb,nChM,W_PWXY,CMO,ExFac), nSym = 1.
   subroutine cho_fmcscf_gpu(rc, FLT, nForb, nIorb, nAorb, FactXI, DLT, DoActive, POrb1, nChM, POrb2, &

@@ -19,6 +19,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   cho_lk_scf            -- dense brute force J[P] - FactXI K, shell-pair KLT
                            layout against an index map built from the block
                            rules, equality with cho_fscf (unscreened)
+  cho_lk_casscf         -- dense brute force FI/FA with ExFac-scaled factors,
+                           (pw|xy), equality with cho_fmcscf at ExFac = 1
   cd_incore             -- ||D - V V^T||_max <= thr, exact rank, numpy cholesky
   energy                -- closed form tr(hP) + 1/2 tr(PJ) - 1/4 tr(PK)
 """
@@ -92,6 +94,11 @@ def lib():
                                      ctypes.c_int32, ctypes.c_double, _dp, ctypes.c_int32, _dp,
                                      ctypes.c_int32, _dp, _dp, _dp, _dp, ctypes.c_int32,
                                      ctypes.c_int32, ctypes.c_double, ctypes.c_int64]
+        L.ora_cho_lk_casscf.restype = ctypes.c_int32
+        L.ora_cho_lk_casscf.argtypes = [ctypes.POINTER(_Chol), _dp, _dp, _dp, _dp, ctypes.c_double,
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
+                                        ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_double, ctypes.c_int64]
         L.ora_cd_incore.restype = ctypes.c_int32
         L.ora_cd_incore.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32, _i32p, ctypes.c_double]
         L.ora_energy.restype = ctypes.c_double
@@ -296,6 +303,33 @@ def cho_fmcscf(store: CholStore, flt: np.ndarray, nk: int, naorb: int, factxi: f
                               norb, 1.0, int(batch))
     if rc != 0:
         raise RuntimeError(f"ora_cho_fmcscf rc={rc}")
+    return f, w[:norb * naorb * nxy]
+
+
+def cho_lk_casscf(store: CholStore, dlt: np.ndarray, flt: np.ndarray, msq, factxi: float,
+                  nfiorb: int, naorb: int, nchm: int, ash1, ash2, do_active: bool, cmo,
+                  nfro: int, norb: int, exfac: float, batch: int = 1 << 30):
+    """CHO_LK_CASSCF without screening.  msq (n, >= nfiorb) and ash2 (n, >= nchm)
+    are AO x vector matrices, ash1 (n, naorb) the active MOs.  Returns (flt, w)."""
+    n = store.n
+
+    def square(a, k):
+        sq = np.zeros((n, n))
+        if a is not None and k > 0:
+            sq[:, :k] = np.asarray(a)[:, :k]
+        return np.ascontiguousarray(sq.T)  # column-major A2(n,n)
+
+    f = _f64(flt).reshape(2, -1).copy()
+    d = _f64(dlt).reshape(2, -1)
+    a1 = _f64(ash1) if naorb else np.zeros(1)
+    cm = _f64(np.asarray(cmo).T) if cmo is not None else np.zeros(1)
+    nxy = naorb * (naorb + 1) // 2
+    w = np.zeros(max(norb * naorb * nxy, 1))
+    rc = lib().ora_cho_lk_casscf(ctypes.byref(store._c), _p(d), _p(f), _p(square(msq, nfiorb)), _p(w),
+                                 float(factxi), nfiorb, naorb, nchm, _p(a1), _p(square(ash2, nchm)),
+                                 int(do_active), _p(cm), nfro, norb, float(exfac), int(batch))
+    if rc != 0:
+        raise RuntimeError(f"ora_cho_lk_casscf rc={rc}")
     return f, w[:norb * naorb * nxy]
 
 

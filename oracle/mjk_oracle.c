@@ -459,6 +459,14 @@ int32_t ora_cho_lk_scf(const ora_chol *ch, int32_t nden, double *flt, double *kl
  *   W_PWXY(p,w,xy) = sum_J (pw|J)(xy|J), p over norb MOs after nfro frozen.
  * dlt is DLT(1:2), flt is FLT(1:2), each np long.  cmo is CMO(n,n)
  * column-major.  w_pwxy receives norb*naorb*naorb(naorb+1)/2 values. */
+static int32_t fmcscf_body(const ora_chol *ch, double *flt, int32_t nforb,
+                           int32_t niorb, int32_t naorb, double factxi,
+                           double fact_xa, const double *dlt, int32_t do_active,
+                           const double *porb1, int32_t nchm,
+                           const double *porb2, const double *porb3,
+                           double *w_pwxy, const double *cmo, int32_t nfro,
+                           int32_t norb, int64_t nvec_batch);
+
 int32_t ora_cho_fmcscf(const ora_chol *ch, double *flt, int32_t nforb,
                        int32_t niorb, int32_t naorb, double factxi,
                        const double *dlt, int32_t do_active,
@@ -467,9 +475,48 @@ int32_t ora_cho_fmcscf(const ora_chol *ch, double *flt, int32_t nforb,
                        double *w_pwxy, const double *cmo, int32_t nfro,
                        int32_t norb, double exfac, int64_t nvec_batch) {
   (void)exfac; /* only triggers a warning message in the reference, :88-94 */
+  return fmcscf_body(ch, flt, nforb, niorb, naorb, factxi, -0.5 /* FactXA, :74 */, dlt,
+                     do_active, porb1, nchm, porb2, porb3, w_pwxy, cmo, nfro, norb,
+                     nvec_batch);
+}
+
+/* src/fock_util/cho_lk_casscf.F90:14-1268 with nSym = 1 and no screening:
+ * the same FI, FA and (tw|xy) as CHO_FMCSCF with the exchange factors
+ * FactX = [FactXI*ExFac, -1/2*ExFac] (:113).  msq (the inactive orbitals or
+ * DI vectors) and ash2 (the DA vectors) are square A2(n,n), AO index fastest,
+ * first nfiorb / nchm columns used; ash1 holds the active MOs orbital-major
+ * A2(naorb,n). */
+int32_t ora_cho_lk_casscf(const ora_chol *ch, const double *dlt, double *flt,
+                          const double *msq, double *w_pwxy, double factxi,
+                          int32_t nfiorb, int32_t naorb, int32_t nchm,
+                          const double *ash1, const double *ash2, int32_t do_active,
+                          const double *cmo, int32_t nfro, int32_t norb, double exfac,
+                          int64_t nvec_batch) {
+  const int32_t n = ch->n;
+  double *p1 = calloc((size_t)(nfiorb > 0 ? nfiorb : 1) * n, sizeof(double));
+  double *p2 = calloc((size_t)(nchm > 0 ? nchm : 1) * n, sizeof(double));
+  for (int64_t k = 0; k < nfiorb; ++k)
+    for (int64_t b = 0; b < n; ++b) p1[k + nfiorb * b] = msq[b + (int64_t)n * k];
+  for (int64_t k = 0; k < nchm; ++k)
+    for (int64_t b = 0; b < n; ++b) p2[k + nchm * b] = ash2[b + (int64_t)n * k];
+  int32_t rc = fmcscf_body(ch, flt, 0, nfiorb, naorb, factxi * exfac, -0.5 * exfac, dlt,
+                           do_active, p1, nchm, p2, ash1, w_pwxy, cmo, nfro, norb,
+                           nvec_batch);
+  free(p1);
+  free(p2);
+  return rc;
+}
+
+static int32_t fmcscf_body(const ora_chol *ch, double *flt, int32_t nforb,
+                           int32_t niorb, int32_t naorb, double factxi,
+                           double fact_xa, const double *dlt, int32_t do_active,
+                           const double *porb1, int32_t nchm,
+                           const double *porb2, const double *porb3,
+                           double *w_pwxy, const double *cmo, int32_t nfro,
+                           int32_t norb, int64_t nvec_batch) {
   const int32_t n = ch->n;
   const int64_t np = (int64_t)n * (n + 1) / 2;
-  const double fact_ci = 1.0, fact_ca = 1.0, fact_xa = -0.5; /* :71 */
+  const double fact_ci = 1.0, fact_ca = 1.0; /* :71 */
   const int32_t nden = do_active ? 2 : 1;
   const int32_t nk = nforb + niorb;
   const int64_t nxy = (int64_t)naorb * (naorb + 1) / 2;

@@ -88,6 +88,13 @@ int32_t ora_cho_fmcscf(const ora_chol *ch, double *flt, int32_t nforb,
                        double *w_pwxy, const double *cmo, int32_t nfro,
                        int32_t norb, double exfac, int64_t nvec_batch);
 
+int32_t ora_cho_lk_casscf(const ora_chol *ch, const double *dlt, double *flt,
+                          const double *msq, double *w_pwxy, double factxi,
+                          int32_t nfiorb, int32_t naorb, int32_t nchm,
+                          const double *ash1, const double *ash2, int32_t do_active,
+                          const double *cmo, int32_t nfro, int32_t norb, double exfac,
+                          int64_t nvec_batch);
+
 int32_t ora_cd_incore(double *x, int32_t n, double *vec, int32_t mxvec,
                       int32_t *numcho, double thr);
 

@@ -67,8 +67,8 @@ def test_choscf_uhf(system, algo, deco):
     assert rel(F[1], pack(J - kmat(Lf, Db))) < RTOL
 
 
-@pytest.mark.parametrize("deco", [True, False])
-def test_cho_cas_drv(system, deco):
+@pytest.mark.parametrize("deco,lk", [(True, False), (False, False), (True, True), (False, True)])
+def test_cho_cas_drv(system, deco, lk):
     jk, Lf, cmo = system
     n = cmo.shape[0]
     nfro, nish, nash = 2, 5, 4
@@ -80,7 +80,7 @@ def test_cho_cas_drv(system, deco):
     rdm = (q * rng.uniform(0.1, 1.9, nash)) @ q.T  # active 1-RDM, eigenvalues in (0, 2)
     DI = 2.0 * Ci @ Ci.T
     DA = Ca @ rdm @ Ca.T
-    FI, FA, W = cho_cas_drv(jk, cmo, DI, DA, nfro, nish, nash, deco=deco)
+    FI, FA, W = cho_cas_drv(jk, cmo, DI, DA, nfro, nish, nash, deco=deco, lk=lk)
     JI, _ = coulomb(Lf, DI)
     JA, _ = coulomb(Lf, DA)
     assert rel(FI, pack(JI - 0.5 * kmat(Lf, DI))) < RTOL

@@ -223,6 +223,33 @@ def test_cho_fock_dft_red_dropin(gpu):
     ctx.close()
 
 
+@pytest.mark.parametrize("exfac,do_active", [(1.0, True), (0.7, True), (0.7, False)])
+def test_cho_lk_casscf_dropin(gpu, exfac, do_active):
+    n, M = 56, 40
+    nfi, na = 9, 5
+    L = chol_vectors_np(n, M, seed=70)
+    rng = np.random.default_rng(71)
+    cmo, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Ci, Ca = cmo[:, :nfi], cmo[:, nfi:nfi + na]
+    occ = rng.uniform(0.1, 1.9, na)
+    DI, DA = 2.0 * Ci @ Ci.T, (Ca * occ) @ Ca.T
+    msq = np.sqrt(2.0) * Ci                    # DECO vectors of DI
+    ash2 = Ca * np.sqrt(occ)                   # DECO vectors of DA
+    dlt = np.stack([fold(DI), fold(DA)])
+    flt0 = rng.standard_normal((2, npair(n)))
+    nfro, norb = 1, n - 1
+    want_f, want_w = oracle.cho_lk_casscf(oracle.CholStore.full(L), dlt, flt0, msq, -0.5, nfi, na, na,
+                                          Ca, ash2, do_active, cmo, nfro, norb, exfac, batch=9)
+    ctx = make_ctx(L, gpu)
+    flt = flt0.copy()
+    w = np.zeros(norb * na * (na * (na + 1) // 2))
+    ctx.cho_lk_casscf(dlt, flt, msq, w, -0.5, nfi, na, na, Ca, ash2, do_active, cmo, nfro, norb,
+                      exfac=exfac)
+    assert rel(flt, want_f) < RTOL
+    assert rel(w, want_w) < RTOL
+    ctx.close()
+
+
 def test_cho_fmcscf_dropin_with_puvx(gpu):
     n, M = 60, 35
     nk, na = 7, 5

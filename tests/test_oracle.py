@@ -324,6 +324,35 @@ def test_cho_fmcscf_matches_dense_including_puvx():
     assert rel(w, want.reshape(-1)) < TOL
 
 
+@pytest.mark.parametrize("exfac,do_active", [(1.0, True), (0.6, True), (0.6, False)])
+def test_cho_lk_casscf_matches_dense(exfac, do_active):
+    rng = np.random.default_rng(95)
+    n, M = 12, 22
+    L = chol_vectors_np(n, M, seed=96)
+    Lf = lfull(L, n)
+    cmo, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    nfro, nk, na = 1, 4, 3
+    Ci, Ca = cmo[:, :nk], cmo[:, nk:nk + na]
+    occ = rng.uniform(0.1, 1.9, na)
+    DI, DA = 2.0 * Ci @ Ci.T, (Ca * occ) @ Ca.T
+    Cm = Ca * np.sqrt(occ)                    # DA = Cm Cm^T
+    dlt = np.stack([fold(DI), fold(DA)])
+    flt = rng.standard_normal((2, npair(n)))
+    norb = n - nfro
+    st = oracle.CholStore.full(L)
+    got, w = oracle.cho_lk_casscf(st, dlt, flt, np.sqrt(2.0) * Ci, -0.5, nk, na, na, Ca, Cm,
+                                  do_active, cmo, nfro, norb, exfac, batch=5)
+    JI, _ = coulomb(Lf, DI)
+    JA, _ = coulomb(Lf, DA)
+    assert rel(got[0], flt[0] + pack(JI) - 0.5 * exfac * pack(exchange(Lf, np.sqrt(2.0) * Ci))) < TOL
+    want_a = flt[1] + pack(JA) - 0.5 * exfac * pack(exchange(Lf, Cm)) if do_active else flt[1]
+    assert rel(got[1], want_a) < TOL
+    if exfac == 1.0:  # the unscreened LK form is CHO_FMCSCF
+        f2, w2 = oracle.cho_fmcscf(st, flt, nk, na, -0.5, dlt, do_active, np.sqrt(2.0) * Ci, na, Cm,
+                                   Ca, cmo, nfro, norb, batch=3)
+        assert rel(got, f2) < TOL and rel(w, w2) < TOL
+
+
 # ------------------------------------------------------------- CD_InCore
 def test_cd_incore_reconstructs_psd_density():
     rng = np.random.default_rng(60)
