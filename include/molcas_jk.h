@@ -224,6 +224,32 @@ int mjk_exchange_mixed(mjk_context *ctx, int32_t nk, const int32_t *nocc,
                        const double *const *corb, double *klt, int64_t ldk,
                        int64_t j_fp32, int64_t j_bf16);
 
+/* Exchange as mjk_exchange with tile screening, the MI355X form of LK
+ * screening (src/fock_util/cho_lk_scf.F90:547-627,713-781 skip shell pairs
+ * and orbitals below thresholds): a K tile (TM x TM AOs) of a vector slice
+ * is skipped when sum_J ||X^J_A||_F ||X^J_B||_F < tau (an elementwise bound
+ * of its contribution), a half-transform tile (one vector, 64 AOs) when
+ * ||L^J[:, tile]||_F max_d ||C_d||_F < thr_x.  tau = thr_x = 0 computes
+ * everything; precision MJK_PREC_BF16X3 cannot be screened. */
+int mjk_exchange_screened(mjk_context *ctx, int32_t nk, const int32_t *nocc,
+                          const double *const *corb, double *klt, int64_t ldk,
+                          int32_t precision, double tau, double thr_x);
+
+/* Work items of the last exchange call: K tiles x slices launched and
+ * skipped, half-transform tiles launched and skipped (zero when the call was
+ * not screened).  Synchronises the context stream.  No counterpart in the
+ * reference. */
+int mjk_screening_stats(mjk_context *ctx, int64_t *k_items, int64_t *k_skipped,
+                        int64_t *x_tiles, int64_t *x_skipped);
+
+/* Turns LK screening of mjk_cho_lk_scf / mjk_cho_lk_casscf on (enable != 0)
+ * or off (default).  thrcom is the decomposition threshold ThrCom that
+ * Cho_LK_ScreeningThreshold reads from the Cholesky module
+ * (src/fock_util/cho_lk_screeningthreshold.F90:24-39); the drop-ins then use
+ * tau = LKThr/nOcc*dmpk and thrv = sqrt(LKThr/(nOcc*NumCho))*dmpk
+ * (src/fock_util/cho_lk_scf.F90:143-171) as the two tile thresholds. */
+int mjk_set_lk_screening(mjk_context *ctx, int32_t enable, double thrcom);
+
 /* w_dev[J] = ||L^J||_F^2 of the full symmetric matrix (device, nchol
  * doubles): the weights a caller uses to place the ladder split points.
  * No counterpart in the reference. */

@@ -57,6 +57,10 @@ SIGNATURES = {
     "mjk_exchange": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32]),
     "mjk_exchange_mixed": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _i64]),
     "mjk_vector_norms": (_c.c_int, [_vp, _vp]),
+    "mjk_exchange_screened": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32, _c.c_double,
+                                         _c.c_double]),
+    "mjk_screening_stats": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "mjk_set_lk_screening": (_c.c_int, [_vp, _i32, _c.c_double]),
     "mjk_cd_incore": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _c.c_double, _vp]),
     "mjk_fock_update": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mjk_energy": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),

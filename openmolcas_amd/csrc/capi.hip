@@ -1,5 +1,6 @@
 // C ABI: context lifecycle, resident vector store, reference-shaped drop-ins
 // and the device engine entry points.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -145,6 +146,27 @@ void free_buf(DevBuf &b) {
 
 using namespace mjk;
 
+// LK thresholds of the drop-ins: the basic threshold of
+// Cho_LK_ScreeningThreshold (cho_lk_screeningthreshold.F90:34-39) and
+//   tau  = LKThr / max(1,nOcc) * dmpk               (cho_lk_scf.F90:155-156)
+//   thrv = sqrt(LKThr / (max(1,nOcc) * NumVT)) * dmpk (cho_lk_scf.F90:170-171)
+// with the smallest threshold over the densities.  tau bounds a skipped K
+// tile of one vector slice, thrv a skipped half-transform tile.
+static void lk_thresholds(const mjk_context *c, int nden, const int32_t *nocc, double dmpk,
+                          double dfmat, double *tau, double *thrv) {
+  double thr = std::min(c->lk_thrcom, 1.0e-6);
+  if (dfmat >= 0.0 && dfmat <= 1.0) thr *= dfmat;
+  const double lkthr = std::max(thr, 1.0e-15);
+  *tau = *thrv = 0.0;
+  for (int d = 0; d < nden; ++d) {
+    const double no = std::max(1, nocc[d]);
+    const double t = lkthr / no * dmpk;
+    const double v = std::sqrt(lkthr / (no * double(c->nchol))) * dmpk;
+    *tau = d ? std::min(*tau, t) : t;
+    *thrv = d ? std::min(*thrv, v) : v;
+  }
+}
+
 extern "C" {
 
 int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
@@ -193,6 +215,8 @@ int mjk_destroy(mjk_context *c) {
   if (!c) return MJK_OK;
   hipStreamSynchronize(c->stream);
   free_buf(c->ws_l32);  // the shared workspace goes with its last owner
+  free_buf(c->lnorm);
+  free_buf(c->scount);
   for (auto &r : c->rsets)
     if (r.d_map) hipFree(r.d_map);
   if (c->L) hipFree(c->L);
@@ -241,6 +265,7 @@ int mjk_device_vectors(mjk_context *c, double **L, int64_t *ldl) {
   *L = c->L;
   *ldl = c->ldl;
   std::fill(c->vset.begin(), c->vset.end(), -1);
+  c->lnorm_valid = false;
   c->nloaded = c->nchol;
   c->l32_valid = false;
   return MJK_OK;
@@ -344,6 +369,7 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
   }
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "upload sync"))) return rc;
   c->l32_valid = false;
+  c->lnorm_valid = false;
   for (int64_t j = jvec; j < jvec + jnum; ++j) {
     if (!c->vset[size_t(j)]) ++c->nloaded;
     c->vset[size_t(j)] = jred + 1;
@@ -379,6 +405,42 @@ int mjk_exchange_mixed(mjk_context *c, int32_t nk, const int32_t *nocc, const do
 int mjk_vector_norms(mjk_context *c, double *w_dev) {
   if (!c) return MJK_ERR_ARG;
   return vector_norms(c, w_dev);
+}
+
+int mjk_exchange_screened(mjk_context *c, int32_t nk, const int32_t *nocc,
+                          const double *const *corb, double *klt, int64_t ldk, int32_t precision,
+                          double tau, double thr_x) {
+  if (!c) return MJK_ERR_ARG;
+  if (!nocc || !corb || !klt) return set_error(c, MJK_ERR_ARG, "exchange: null pointer");
+  for (int d = 0; d < nk; ++d)
+    if (nocc[d] > 0 && !corb[d]) return set_error(c, MJK_ERR_ARG, "exchange: null orbital matrix");
+  return exchange_screened(c, nk, nocc, corb, klt, ldk, precision, tau, thr_x);
+}
+
+int mjk_screening_stats(mjk_context *c, int64_t *k_items, int64_t *k_skipped, int64_t *x_tiles,
+                        int64_t *x_skipped) {
+  if (!c) return MJK_ERR_ARG;
+  unsigned long long h[2] = {0, 0};
+  if (c->scount.ptr) {
+    int rc;
+    if ((rc = hip_check(c, hipMemcpyAsync(h, c->scount.ptr, sizeof(h), hipMemcpyDeviceToHost,
+                                          c->stream), "screening counters")) ||
+        (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))
+      return rc;
+  }
+  if (k_items) *k_items = c->k_items;
+  if (k_skipped) *k_skipped = int64_t(h[0]);
+  if (x_tiles) *x_tiles = c->x_tiles;
+  if (x_skipped) *x_skipped = int64_t(h[1]);
+  return MJK_OK;
+}
+
+int mjk_set_lk_screening(mjk_context *c, int32_t enable, double thrcom) {
+  if (!c) return MJK_ERR_ARG;
+  if (enable && !(thrcom > 0.0)) return set_error(c, MJK_ERR_ARG, "set_lk_screening: thrcom <= 0");
+  c->lk_on = enable != 0;
+  if (enable) c->lk_thrcom = thrcom;
+  return MJK_OK;
 }
 
 int mjk_fock_update(mjk_context *c, int32_t nd, double *flt, int64_t ldf, const double *jlt,
@@ -475,10 +537,9 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
                    const int32_t *niorb, const double *porb, const double *plt, double factxi,
                    int32_t nscreen, double dmpk, double dfmat, int32_t nshell,
                    const int32_t *nbassh, int32_t precision) {
-  // Every shell pair and orbital is kept: the result is the reference's in the
-  // limit dmpk -> 0, so the screening controls only have to be valid.
+  // Without mjk_set_lk_screening every tile is computed (the reference's result
+  // for dmpk -> 0); with it, tiles are skipped below the LK thresholds.
   (void)nscreen;
-  (void)dfmat;
   if (!c) return MJK_ERR_ARG;
   if (nden != 1 && nden != 2)
     return set_error(c, MJK_ERR_ARG, "cho_lk_scf: invalid parameter nDen");  // cho_lk_scf.F90:93-96
@@ -546,7 +607,9 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
     return rc;
   if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
   const double *cptr[2] = {dC, dC + int64_t(nocc[0]) * n};
-  if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
+  double tau = 0.0, thrv = 0.0;
+  if (c->lk_on) lk_thresholds(c, nden, nocc, dmpk, dfmat, &tau, &thrv);
+  if ((rc = exchange_screened(c, nden, nocc, cptr, dK, np, precision, tau, thrv))) return rc;
   const int32_t jsrc[2] = {0, 0};
   const double fj[2] = {1.0, 1.0};               // FactCI, cho_lk_scf.F90:102
   const double fk[2] = {-factxi, -factxi};       // -FactXI, cho_lk_scf.F90:803,830
@@ -599,7 +662,7 @@ static int fmcscf_core(mjk_context *c, double *flt, int32_t nforb, int32_t niorb
                        double fk_i, double fk_a, const double *dlt, int32_t do_active,
                        const double *porb1, int32_t nchm, const double *porb2,
                        const double *porb3, double *w_pwxy, const double *cmo, int32_t nfro,
-                       int32_t norb, int32_t precision) {
+                       int32_t norb, int32_t precision, double tau = 0.0, double thrv = 0.0) {
   if (!flt || !dlt) return set_error(c, MJK_ERR_ARG, "cho_fmcscf: null pointer");
   if (nforb < 0 || niorb < 0 || naorb < 0 || nchm < 0)
     return set_error(c, MJK_ERR_ARG, "cho_fmcscf: negative orbital count");
@@ -634,7 +697,7 @@ static int fmcscf_core(mjk_context *c, double *flt, int32_t nforb, int32_t niorb
   if ((rc = coulomb(c, nden, dD, np, dJ, np, nullptr))) return rc;
   const int32_t nocc[2] = {nk1, nk2};
   const double *cptr[2] = {dC1, dC2};
-  if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
+  if ((rc = exchange_screened(c, nden, nocc, cptr, dK, np, precision, tau, thrv))) return rc;
   const int32_t jsrc[2] = {0, 1};
   const double fj[2] = {1.0, 1.0};        // FactCI, FactCA (cho_fmcscf.F90:71)
   const double fk[2] = {fk_i, fk_a};      // FactXI, FactXA
@@ -672,9 +735,9 @@ int mjk_cho_lk_casscf(mjk_context *c, const double *dlt, double *flt, const doub
                       const double *ash1, const double *ash2, int32_t do_active, int32_t nscreen,
                       double dmpk, double dfmat, const double *cmo, int32_t nfro, int32_t norb,
                       double exfac, int32_t precision) {
-  // Every shell pair and orbital is kept (the reference's result for dmpk -> 0).
+  // Without mjk_set_lk_screening every tile is computed (the reference's result
+  // for dmpk -> 0); with it, tiles are skipped below the LK thresholds.
   (void)nscreen;
-  (void)dfmat;
   if (!c) return MJK_ERR_ARG;
   if (dmpk < 0.0) return set_error(c, MJK_ERR_ARG, "cho_lk_casscf: dmpk < 0");
   const int64_t n = c->n;
@@ -690,9 +753,15 @@ int mjk_cho_lk_casscf(mjk_context *c, const double *dlt, double *flt, const doub
     for (int64_t b = 0; b < n; ++b) p1[size_t(k + nfiorb * b)] = msq[b + n * k];
   for (int64_t k = 0; k < nk2; ++k)
     for (int64_t b = 0; b < n; ++b) p2[size_t(k + nk2 * b)] = ash2[b + n * k];
+  double tau = 0.0, thrv = 0.0;
+  if (c->lk_on) {
+    const int32_t no[2] = {nfiorb, nk2};
+    lk_thresholds(c, do_active ? 2 : 1, no, dmpk, dfmat, &tau, &thrv);
+  }
   // FactX = [FactXI*ExFac, -1/2*ExFac], cho_lk_casscf.F90:113
   return fmcscf_core(c, flt, 0, nfiorb, naorb, factxi * exfac, -0.5 * exfac, dlt, do_active,
-                     p1.data(), nchm, p2.data(), ash1, w_pwxy, cmo, nfro, norb, precision);
+                     p1.data(), nchm, p2.data(), ash1, w_pwxy, cmo, nfro, norb, precision, tau,
+                     thrv);
 }
 
 }  // extern "C"

@@ -118,7 +118,7 @@ template <class P, int OT, int TA, int BK>
 __global__ __launch_bounds__(kThreads) void k_ht(
     const typename P::T *__restrict__ L, int64_t ldl, int n, int n_pad,
     const typename P::T *__restrict__ Cb, int64_t j0, int ntile_a, int64_t nblocks,
-    typename P::T *__restrict__ X) {
+    typename P::T *__restrict__ X, const Screen sc) {
   typedef typename P::T T;
   typedef typename P::Acc Acc;
   constexpr int LDC = 16 * OT;
@@ -135,6 +135,19 @@ __global__ __launch_bounds__(kThreads) void k_ht(
   const int a0 = int(lb % ntile_a) * TA;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const T *__restrict__ Lj = L + (j0 + jr) * ldl;
+  if (sc.lnorm) {  // screened: ||X tile||_F <= ||L^J[:, tile]||_F max_d ||C_d||_F
+    double s2 = 0.0;
+    for (int st = a0 / 64; st < (a0 + TA) / 64; ++st) {
+      const double v = sc.lnorm[(j0 + jr) * sc.nstrip + st];
+      s2 += v * v;
+    }
+    if (sqrt(s2) * sc.cmax[0] < sc.thr_x) {
+      T *__restrict__ xj = X + jr * LDC * int64_t(n_pad);
+      for (int e = tid; e < LDC * TA; e += kThreads) xj[int64_t(e / TA) * n_pad + a0 + e % TA] = T(0);
+      if (tid == 0) atomicAdd(sc.cnt + 1, 1ull);
+      return;
+    }
+  }
 
   Acc acc[OT][NB];
 #pragma unroll
@@ -233,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 template <class P, int TM, int BKF>
 __global__ __launch_bounds__(kThreads) void k_syrk(
     const typename P::T *__restrict__ X, int n_pad, int ldc, int64_t jc, int ntiles,
-    const SyrkSets sets, double *__restrict__ kpart_all, int chunk) {
+    const SyrkSets sets, double *__restrict__ kpart_all, int chunk, const Screen sc) {
   typedef typename P::T T;
   typedef typename P::Acc Acc;
   constexpr int WT = TM / 2;
@@ -260,6 +273,27 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const bool idle = diag && wm < wn;  // wave tile entirely above the diagonal
+  // slot (chunk, slice): every chunk of vectors writes its own partial tiles
+  double *__restrict__ kp =
+      kpart + ((int64_t(chunk) * sets.nsplit[set] + s) * ntiles + tile) * TM * TM;
+  if (sc.xn2 && nf > 0) {  // screened: |K_ab| of this slice <= sum_J ||X^J_A||_F ||X^J_B||_F
+    double bound = 0.0;
+    for (int64_t J = ja; J < jz; ++J) {
+      const double *__restrict__ r = sc.xn2 + (int64_t(set) * jc + J) * sc.nstrip;
+      double sa2 = 0.0, sb2 = 0.0;
+#pragma unroll
+      for (int st = 0; st < TM / 64; ++st) {
+        sa2 += r[A * (TM / 64) + st];
+        sb2 += r[B * (TM / 64) + st];
+      }
+      bound += sqrt(sa2 * sb2);
+    }
+    if (bound < sc.tau) {
+      for (int e = tid; e < TM * TM; e += kThreads) kp[e] = 0.0;
+      if (tid == 0) atomicAdd(sc.cnt, 1ull);
+      return;
+    }
+  }
 
   Acc acc[MT][MT];
   double accd[P::kFlush ? MT : 1][P::kFlush ? MT : 1][4];
@@ -354,9 +388,6 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
     __syncthreads();
   }
   flush();
-  // slot (chunk, slice): every chunk of vectors writes its own partial tiles
-  double *__restrict__ kp =
-      kpart + ((int64_t(chunk) * sets.nsplit[set] + s) * ntiles + tile) * TM * TM;
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -370,6 +401,80 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
         else v = acc[i][j][r];
         kp[al * TM + bl] = v;
       }
+}
+
+// ---- screening norms -------------------------------------------------------
+__device__ inline double block_sum256(double x, double *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  const double t = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();  // red is reused by the next call
+  return t;
+}
+
+// out[J*nstrip + st] = ||L^J[:, 64*st .. 64*st+63]||_F of the symmetric matrix.
+// grid = (M, nstrip), 256 threads.
+__global__ __launch_bounds__(kThreads) void k_strip_norms(const double *__restrict__ L,
+                                                          int64_t ldl, int n, int nstrip,
+                                                          double *__restrict__ out) {
+  __shared__ double red[4];
+  const int64_t J = blockIdx.x;
+  const int st = blockIdx.y;
+  const double *__restrict__ Lj = L + J * ldl;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < 64 * n; e += kThreads) {
+    const int a = st * 64 + (e & 63), b = e >> 6;
+    if (a < n) {
+      const int hi = a > b ? a : b, lo = a > b ? b : a;
+      const double v = Lj[tri0(hi) + lo];
+      s += v * v;
+    }
+  }
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) out[J * nstrip + st] = sqrt(s);
+}
+
+// xn2[(d*jn + J)*nstrip + st] = sum over the rows of set d and the 64 AOs of
+// strip st of X[J]^2.  grid = (jn, nstrip, nk), 256 threads.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_xnorms(const T *__restrict__ X, int ldc, int n_pad,
+                                                     const SyrkSets sets, int nstrip,
+                                                     double *__restrict__ xn2) {
+  __shared__ double red[4];
+  const int64_t J = blockIdx.x;
+  const int st = blockIdx.y, d = blockIdx.z;
+  const int off = sets.off[d], opad = sets.opad[d];
+  double s = 0.0;
+  for (int e = threadIdx.x; e < opad * 64; e += kThreads) {
+    const double v = double(X[(J * ldc + off + (e >> 6)) * int64_t(n_pad) + st * 64 + (e & 63)]);
+    s += v * v;
+  }
+  s = block_sum256(s, red);
+  if (threadIdx.x == 0) xn2[(int64_t(d) * gridDim.x + J) * nstrip + st] = s;
+}
+
+// out[d] = ||C_d||_F from the bank (rows a < n, columns off_d .. off_d+opad_d-1),
+// out[nk] = max_d out[d].  One workgroup.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_cnorm(const T *__restrict__ Cb, int ldc, int n,
+                                                    const SyrkSets sets, double *__restrict__ out) {
+  __shared__ double red[4];
+  double mx = 0.0;
+  for (int d = 0; d < sets.n; ++d) {
+    const int off = sets.off[d], opad = sets.opad[d];
+    double s = 0.0;
+    for (int e = threadIdx.x; e < n * opad; e += kThreads) {
+      const double v = double(Cb[int64_t(e / opad) * ldc + off + e % opad]);
+      s += v * v;
+    }
+    s = block_sum256(s, red);
+    mx = fmax(mx, sqrt(s));
+    if (threadIdx.x == 0) out[d] = sqrt(s);
+  }
+  if (threadIdx.x == 0) out[sets.n] = mx;
 }
 
 // ---- split-BF16, three products ---------------------------------------------
@@ -675,7 +780,8 @@ void launch_ht(Context *c, const typename P::T *L, const typename P::T *cb, int6
   const int64_t nblocks = jc * ntile_a;
   const int64_t grid = round_up(nblocks, 8);
   hipLaunchKernelGGL((k_ht<P, OT, TA, BK>), dim3(unsigned(grid)), dim3(kThreads), 0, c->stream, L,
-                     c->ldl, c->n, c->n_pad, cb, j0, ntile_a, nblocks, X);
+                     c->ldl, c->n, c->n_pad, cb, j0, ntile_a, nblocks, X, c->scr);
+  if (c->scr.lnorm) c->x_tiles += nblocks;
 }
 
 // AO tile width: FP64 defaults to 64 (more workgroups per CU); FP32 uses 128
@@ -712,10 +818,11 @@ void syrk_launch(Context *c, int tm, int items, const typename P::T *X, int ldc,
                  int ntiles, const SyrkSets &sets, double *kpart, int chunk) {
   if (tm == 128)
     hipLaunchKernelGGL((k_syrk<P, 128, 16>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
-                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk);
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk, c->scr);
   else
     hipLaunchKernelGGL((k_syrk<P, 64, 32>), dim3(unsigned(items)), dim3(kThreads), 0, c->stream, X,
-                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk);
+                       c->n_pad, ldc, jn, ntiles, sets, kpart, chunk, c->scr);
+  if (c->scr.xn2) c->k_items += items;
 }
 
 template <int OT, int TA>
@@ -839,7 +946,8 @@ struct Segment {
 };
 
 int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
-                      double *klt, int64_t ldk, const std::vector<Segment> &segs) {
+                      double *klt, int64_t ldk, const std::vector<Segment> &segs,
+                      double tau = 0.0, double thr_x = 0.0) {
   const int n = c->n, n_pad = c->n_pad;
   int rc;
   std::vector<int> off, opad;
@@ -904,6 +1012,45 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
   if ((rc = ensure(c, c->w->kpart, sizeof(double) * size_t(std::max<int64_t>(kb, 1))))) return rc;
   double *kpart = static_cast<double *>(c->w->kpart.ptr);
 
+  // ---- screening set-up (norms of L strips, of the orbital sets, counters)
+  const bool screen = tau > 0.0 || thr_x > 0.0;
+  const int nstrip = n_pad / 64;
+  c->k_items = c->x_tiles = 0;
+  c->scr = Screen{};
+  if (screen) {
+    if (use[MJK_PREC_BF16X3])
+      return set_error(c, MJK_ERR_ARG, "exchange: screening is not available for split-BF16");
+    if ((rc = ensure(c, c->scount, 2 * sizeof(unsigned long long))) ||
+        (rc = fill_zero(c, c->scount.ptr, 2 * sizeof(unsigned long long))))
+      return rc;
+    c->scr.cnt = static_cast<unsigned long long *>(c->scount.ptr);
+    c->scr.nstrip = nstrip;
+    if (thr_x > 0.0) {
+      if (!c->lnorm_valid) {
+        if ((rc = ensure(c, c->lnorm, sizeof(double) * size_t(c->nchol) * nstrip))) return rc;
+        hipLaunchKernelGGL(k_strip_norms, dim3(unsigned(c->nchol), unsigned(nstrip)), dim3(kThreads),
+                           0, c->stream, c->L, c->ldl, n, nstrip,
+                           static_cast<double *>(c->lnorm.ptr));
+        c->lnorm_valid = true;
+      }
+      if ((rc = ensure(c, c->w->cnorm, sizeof(double) * (kMaxSets + 1)))) return rc;
+      double *cn = static_cast<double *>(c->w->cnorm.ptr);
+      if (use[MJK_PREC_FP64])
+        hipLaunchKernelGGL(k_cnorm<double>, dim3(1), dim3(kThreads), 0, c->stream,
+                           static_cast<const double *>(c->w->cbank.ptr), ldc, n, sets, cn);
+      else
+        hipLaunchKernelGGL(k_cnorm<float>, dim3(1), dim3(kThreads), 0, c->stream,
+                           static_cast<const float *>(c->w->cb32.ptr), ldc, n, sets, cn);
+      c->scr.lnorm = static_cast<const double *>(c->lnorm.ptr);
+      c->scr.cmax = cn + nk;
+      c->scr.thr_x = thr_x;
+    }
+    if (tau > 0.0) {
+      if ((rc = ensure(c, c->w->xn2, sizeof(double) * size_t(nk) * jc * nstrip))) return rc;
+      c->scr.tau = tau;
+    }
+  }
+
   int64_t slot = 0;  // running chunk index over all segments
   for (const Segment &s : segs) {
     for (int64_t j0 = s.j0; j0 < s.j1; j0 += jc, ++slot) {
@@ -925,6 +1072,18 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
         }
       }
       phase_end(c, MJK_PHASE_HALF_TRANSFORM);
+      if (screen && tau > 0.0) {
+        double *xn2 = static_cast<double *>(c->w->xn2.ptr);
+        if (s.prec == MJK_PREC_FP64)
+          hipLaunchKernelGGL(k_xnorms<double>, dim3(unsigned(jn), unsigned(nstrip), unsigned(nk)),
+                             dim3(kThreads), 0, c->stream, static_cast<const double *>(c->w->x.ptr),
+                             ldc, n_pad, sets, nstrip, xn2);
+        else
+          hipLaunchKernelGGL(k_xnorms<float>, dim3(unsigned(jn), unsigned(nstrip), unsigned(nk)),
+                             dim3(kThreads), 0, c->stream, static_cast<const float *>(c->w->x.ptr),
+                             ldc, n_pad, sets, nstrip, xn2);
+        c->scr.xn2 = xn2;
+      }
       phase_begin(c);
       switch (s.prec) {
         case MJK_PREC_FP64:
@@ -942,6 +1101,7 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
       phase_end(c, MJK_PHASE_EXCHANGE);
     }
   }
+  c->scr = Screen{};  // later launches (puvx, backward) are not screened
   phase_begin(c);
   hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n), unsigned(nk)), dim3(256), 0, c->stream, kpart, n,
                      tm, ntiles, sets, int(nchunk), klt, ldk);
@@ -969,6 +1129,20 @@ int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *c
   const int64_t M = c->nchol;
   return exchange_mixed(c, nk, nocc, corb, klt, ldk, precision == MJK_PREC_FP64 ? M : 0,
                         precision == MJK_PREC_BF16X3 ? 0 : M);
+}
+
+int exchange_screened(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                      double *klt, int64_t ldk, int32_t precision, double tau, double thr_x) {
+  if (nk < 1) return set_error(c, MJK_ERR_ARG, "exchange: nk < 1");
+  if (ldk < c->np) return set_error(c, MJK_ERR_ARG, "exchange: ldk < nbas(nbas+1)/2");
+  if (c->nloaded != c->nchol)
+    return set_error(c, MJK_ERR_SHORT_READ, "exchange: not all Cholesky vectors were uploaded");
+  if (precision < MJK_PREC_FP64 || precision > MJK_PREC_BF16X3)
+    return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
+  if (!(tau >= 0.0) || !(thr_x >= 0.0))
+    return set_error(c, MJK_ERR_ARG, "exchange: screening thresholds must be >= 0");
+  std::vector<Segment> segs = {{0, c->nchol, precision}};
+  return exchange_segments(c, nk, nocc, corb, klt, ldk, segs, tau, thr_x);
 }
 
 int exchange_mixed(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,

@@ -33,12 +33,23 @@ struct DevBuf {
 // Device workspaces.  Contexts created with mjk_create_shared use the same
 // Workspace and must queue their work on the same stream.
 struct Workspace {
-  DevBuf d, vpart, v, jpart, cbank, cb32, cb3, x, kpart, stage, j, k, red, misc, sets;
+  DevBuf d, vpart, v, jpart, cbank, cb32, cb3, x, kpart, stage, j, k, red, misc, sets, xn2, cnorm;
   ~Workspace() {
     for (DevBuf *b : {&d, &vpart, &v, &jpart, &cbank, &cb32, &cb3, &x, &kpart, &stage, &j, &k, &red,
-                      &misc, &sets})
+                      &misc, &sets, &xn2, &cnorm})
       if (b->ptr) (void)hipFree(b->ptr);
   }
+};
+
+// Tile screening of the exchange kernels (all pointers null = off).
+struct Screen {
+  const double *lnorm = nullptr;      // [J][strip] ||L^J[:, 64-AO strip]||_F
+  const double *cmax = nullptr;       // device scalar: max over sets of ||C_d||_F
+  double thr_x = 0.0;                 // half-transform tile threshold
+  const double *xn2 = nullptr;        // [d][J][strip] squared norms of the chunk's X blocks
+  double tau = 0.0;                   // K tile threshold
+  int nstrip = 0;                     // n_pad / 64
+  unsigned long long *cnt = nullptr;  // [K items skipped, X tiles skipped]
 };
 
 struct ReducedSet {
@@ -69,6 +80,15 @@ struct Context {
   // workspaces
   DevBuf ws_l32;          // FP32 shadow of L (mixed-precision exchange)
   bool l32_valid = false;
+  DevBuf lnorm;           // ||L^J[:, 64-AO strip]||_F per vector and strip (screening)
+  bool lnorm_valid = false;
+
+  // LK screening of the LK drop-ins (mjk_set_lk_screening) and its counters
+  bool lk_on = false;
+  double lk_thrcom = 1.0e-6;
+  DevBuf scount;                  // device: [K items skipped, X tiles skipped]
+  Screen scr;                     // screening of the exchange launch in progress
+  int64_t k_items = 0, x_tiles = 0;  // work items of the last screened exchange
   std::shared_ptr<Workspace> wsp;  // workspaces, possibly shared with other contexts
   Workspace *w = nullptr;
 
@@ -125,6 +145,12 @@ int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *
 // Half-transform of vectors j0..j0+jn-1 against the bank (16*ot columns):
 // X[(jr*16*ot + k)*n_pad + a] = sum_b Cb[b][k] L^{j0+jr}_{ab}.
 void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, double *X);
+
+// Exchange with tile screening (thresholds 0 = off): a K tile of a vector
+// slice is skipped when sum_J ||X^J_A||_F ||X^J_B||_F < tau, a half-transform
+// tile when ||L^J[:, tile]||_F max_d ||C_d||_F < thr_x.
+int exchange_screened(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
+                      double *klt, int64_t ldk, int32_t precision, double tau, double thr_x);
 
 // w[J] = squared Frobenius norm of the symmetric matrix L^J (device, M doubles).
 int vector_norms(Context *c, double *w);

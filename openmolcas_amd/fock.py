@@ -308,6 +308,29 @@ class JKContext:
                                            float(thr), ctypes.byref(irc)))
         return int(irc.value), v[:num.value].t()
 
+    def exchange_screened(self, corbs, tau: float, thr_x: float = 0.0, precision="fp64") -> torch.Tensor:
+        """Exchange with tile screening (see mjk_exchange_screened)."""
+        cs = [_dev_f64(c, self.device) for c in corbs]
+        nk = len(cs)
+        nocc = np.array([c.shape[1] if c.dim() == 2 else 0 for c in cs], dtype=np.int32)
+        ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
+        k = torch.empty((nk, self.npair), dtype=torch.float64, device=self.device)
+        self._check(self.lib.mjk_exchange_screened(self._h, nk, _ptr(nocc), ptrs, _ptr(k), _ld(k),
+                                                   precision_code(precision), float(tau),
+                                                   float(thr_x)))
+        return k
+
+    def screening_stats(self) -> dict:
+        """Work items of the last exchange: K tiles x slices and half-transform tiles."""
+        v = [ctypes.c_int64(0) for _ in range(4)]
+        self._check(self.lib.mjk_screening_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return {"k_items": v[0].value, "k_skipped": v[1].value, "x_tiles": v[2].value,
+                "x_skipped": v[3].value}
+
+    def set_lk_screening(self, enable: bool, thrcom: float = 1e-6) -> None:
+        """LK screening of the LK drop-ins; thrcom is the decomposition threshold."""
+        self._check(self.lib.mjk_set_lk_screening(self._h, int(bool(enable)), float(thrcom)))
+
     def vector_norms(self) -> torch.Tensor:
         """(nchol,) squared Frobenius norms of the symmetric vectors L^J."""
         w = torch.empty(self.nchol, dtype=torch.float64, device=self.device)
