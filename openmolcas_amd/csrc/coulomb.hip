@@ -94,13 +94,14 @@ __global__ __launch_bounds__(kThreads) void k_coulomb_j(
 #pragma unroll
   for (int d = 0; d < ND; ++d) acc[d] = v2d{0.0, 0.0};
   int64_t j = ja;
-  for (; j + 4 <= jz; j += 4) {
-    v2d l[4];
+  constexpr int U = ND <= 2 ? 8 : 4;  // rows in flight per thread
+  for (; j + U <= jz; j += U) {
+    v2d l[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
       l[u] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(L + (j + u) * ldl + p));
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
         if (d < nd) {
