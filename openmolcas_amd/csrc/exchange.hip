@@ -950,6 +950,7 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
                       double tau = 0.0, double thr_x = 0.0) {
   const int n = c->n, n_pad = c->n_pad;
   int rc;
+  if (nk > kMaxSets) return set_error(c, MJK_ERR_ARG, "exchange: more than 16 orbital sets");
   std::vector<int> off, opad;
   int ot = 0, tot = 0;
   if ((rc = bank_layout(c, nk, nocc, off, opad, &ot, &tot))) return rc;

@@ -1,0 +1,113 @@
+"""Edge cases of the drop-ins and the engine against the oracle (FP64, relative
+1e-11): the smallest basis (n = 1, one vector), a density without orbitals
+next to one with, a reduced set without pairs, a basis just past a tile
+boundary (n = 129), one orbital, the 16-set and 16-density maxima and their
+error codes."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from openmolcas_amd import MjkError
+from openmolcas_amd.fock import JKContext
+from openmolcas_amd.synthetic import chol_vectors_np, fold, npair, orbitals_np
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-11
+
+
+def rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(1.0, np.max(np.abs(b)))
+
+
+def test_smallest_basis(gpu):
+    L = np.array([[0.7]])
+    jk = JKContext(1, 1, device=gpu)
+    jk.upload_packed(L)
+    C = np.array([[0.9]])
+    dlt = fold(2.0 * C @ C.T)
+    flt = np.array([[0.25]])
+    want = oracle.cho_fscf(oracle.CholStore.full(L), flt, [1], [C], dlt, 0.5)
+    got = flt.copy()
+    jk.cho_fscf(1, got, [0], [1], [C], dlt, 0.5)
+    assert rel(got, want) < RTOL
+    # J = L (L D), K = L C C^T L for a 1 x 1 problem
+    assert got[0, 0] == pytest.approx(0.25 + 0.7 * 0.7 * dlt[0] - 0.5 * (0.7 * 0.9) ** 2, rel=1e-14)
+    jk.close()
+
+
+def test_density_without_orbitals(gpu):
+    n, M = 37, 21
+    L = chol_vectors_np(n, M, seed=1)
+    C = orbitals_np(n, 5, seed=2)
+    dlt = fold(2.0 * C @ C.T)
+    flt0 = np.random.default_rng(3).standard_normal((2, npair(n)))
+    want = oracle.cho_fscf(oracle.CholStore.full(L), flt0, [5, 0], [C, np.zeros((n, 0))], dlt, 1.0)
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(L)
+    got = flt0.copy()
+    jk.cho_fscf(2, got, [0, 0], [5, 0], [C, np.zeros((n, 0))], dlt, 1.0)
+    assert rel(got, want) < RTOL
+    K = jk.exchange([torch.from_numpy(C), torch.zeros((n, 0), dtype=torch.float64)])
+    assert torch.count_nonzero(K[1]).item() == 0
+    jk.close()
+
+
+def test_reduced_set_without_pairs(gpu):
+    n, M1 = 20, 6
+    npr = npair(n)
+    L0 = chol_vectors_np(n, M1, seed=4)
+    st = oracle.CholStore(n, [np.arange(npr), np.zeros(0, dtype=np.int64)],
+                          [(0, v) for v in L0] + [(1, np.zeros(0)) for _ in range(2)])
+    C = orbitals_np(n, 4, seed=5)
+    dlt = fold(2.0 * C @ C.T)
+    want = oracle.cho_fscf(st, np.zeros((1, npr)), [4], [C], dlt, 1.0)
+    jk = JKContext(n, M1 + 2, device=gpu)
+    jk.set_reduced_set(0, np.arange(npr))
+    jk.set_reduced_set(1, np.zeros(0, dtype=np.int64))
+    jk.upload_vectors(0, 0, L0)
+    jk.upload_vectors(1, M1, np.zeros((2, 0)))
+    got = np.zeros((1, npr))
+    jk.cho_fscf(1, got, [0], [4], [C], dlt, 1.0)
+    assert rel(got, want) < RTOL
+    jk.close()
+
+
+@pytest.mark.parametrize("n,o", [(129, 1), (128, 128), (200, 200)])
+def test_tile_boundaries(gpu, n, o):
+    M = 17
+    L = chol_vectors_np(n, M, seed=n)
+    C = orbitals_np(n, o, seed=o)
+    dlt = fold(2.0 * C @ C.T)
+    want = oracle.cho_fscf(oracle.CholStore.full(L), np.zeros((1, npair(n))), [o], [C], dlt, 0.5)
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(L)
+    got = np.zeros((1, npair(n)))
+    jk.cho_fscf(1, got, [0], [o], [C], dlt, 0.5)
+    assert rel(got, want) < RTOL
+    jk.close()
+
+
+def test_maximum_sets_and_densities(gpu):
+    n, M = 40, 12
+    L = chol_vectors_np(n, M, seed=9)
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(L)
+    Cs = [orbitals_np(n, 1 + (d % 5), seed=20 + d) for d in range(16)]
+    K = jk.exchange([torch.from_numpy(c) for c in Cs]).cpu().numpy()
+    st = oracle.CholStore.full(L)
+    for d in (0, 7, 15):
+        ref = oracle.cho_fscf(st, np.zeros((1, npair(n))), [Cs[d].shape[1]], [Cs[d]],
+                              np.zeros(npair(n)), -1.0)[0]
+        assert rel(K[d], ref) < RTOL
+    with pytest.raises(MjkError):
+        jk.exchange([torch.from_numpy(Cs[0])] * 17)
+    D = torch.from_numpy(np.stack([fold(c @ c.T) for c in Cs]))
+    J = jk.coulomb(D).cpu().numpy()
+    for d in (0, 15):
+        ref = oracle.cho_fscf(st, np.zeros((1, npair(n))), [0], [np.zeros((n, 0))], D[d].numpy(), 0.0)[0]
+        assert rel(J[d], ref) < RTOL
+    with pytest.raises(MjkError):
+        jk.coulomb(torch.cat([D, D[:1]]))
+    jk.close()
