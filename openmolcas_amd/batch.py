@@ -1,14 +1,17 @@
 """Batched J/K builds for many independent systems, captured as one HIP graph.
 
 Each system (molecule) has its own context -- its Cholesky vectors live in that
-context's HBM store, written in place -- while all contexts share one set of
-device workspaces and one stream (mjk_create_shared).  ``step()`` runs, for
-every system, the closed-shell energy build (density, Coulomb, exchange, Fock
-and energy steps) followed by the hand-written backward (dE/dC = 4FC and the
-packed dE/dL).  ``capture()`` records one step into a torch CUDAGraph (a
-hipGraph on ROCm) after a warm-up step has sized every workspace, so a replay
-is a single graph launch with no allocation, no host synchronisation and no
-memset nodes.
+context's HBM store, written in place.  The systems are dealt to ``lanes``
+streams (largest first, each to the lane with the least work so far); the
+contexts of one lane share one set of device workspaces and that lane's stream
+(mjk_create_shared), so the lanes run concurrently and small systems fill the
+GPU together.  ``step()`` runs, for every system, the closed-shell energy build
+(density, Coulomb, exchange, Fock and energy steps) followed by the
+hand-written backward (dE/dC = 4FC and the packed dE/dL).  ``capture()``
+records one step -- the fork to the lane streams and the join back included --
+into a torch CUDAGraph (a hipGraph on ROCm) after a warm-up step has sized
+every workspace, so a replay is a single graph launch with no allocation, no
+host synchronisation and no memset nodes.
 """
 from __future__ import annotations
 
@@ -20,58 +23,82 @@ from .synthetic import chol_vectors_torch, npair, orbitals_torch
 
 
 class MoleculeBatch:
-    def __init__(self, shapes, device=None, seed: int = 0, with_grad_l: bool = True):
+    def __init__(self, shapes, device=None, seed: int = 0, with_grad_l: bool = True,
+                 lanes: int = 4):
         """shapes: list of (nbas, nchol, nocc)."""
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         self.stream = torch.cuda.Stream(device=self.device)
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
-        self.ctx = []
-        self.L, self.C, self.h = [], [], []
+        nl = max(1, min(int(lanes), len(self.shapes)))
+        self.lane_streams = [torch.cuda.Stream(device=self.device) for _ in range(nl)]
+        # deal systems to lanes: largest exchange work first, to the least loaded lane
+        work = [M * o * n * n for n, M, o in self.shapes]
+        load = [0.0] * nl
+        self.lane_of = [0] * len(self.shapes)
+        for i in sorted(range(len(self.shapes)), key=lambda i: -work[i]):
+            k = min(range(nl), key=lambda k: load[k])
+            self.lane_of[i] = k
+            load[k] += work[i]
+        self.ctx = [None] * len(self.shapes)
+        self.L, self.C, self.h = [None] * len(shapes), [None] * len(shapes), [None] * len(shapes)
         self.E = torch.zeros(len(shapes), dtype=torch.float64, device=self.device)
-        self.dC, self.dL = [], []
-        with torch.cuda.stream(self.stream):
-            first = None
-            for i, (n, M, o) in enumerate(self.shapes):
-                c = JKContext(n, M, device=self.device, share=first)
-                if first is None:
-                    first = c
-                    c.bind_stream(self.stream)
-                self.ctx.append(c)
+        self.dC, self.dL = [None] * len(shapes), [None] * len(shapes)
+        first = [None] * nl
+        for i, (n, M, o) in enumerate(self.shapes):
+            k = self.lane_of[i]
+            with torch.cuda.stream(self.lane_streams[k]):
+                c = JKContext(n, M, device=self.device, share=first[k])
+                if first[k] is None:
+                    first[k] = c
+                    c.bind_stream(self.lane_streams[k])
+                self.ctx[i] = c
                 L = c.device_vectors()
                 chol_vectors_torch(n, M, seed + 7 * i, device=self.device, out=L)
-                self.L.append(L)
-                self.C.append(orbitals_torch(n, o, seed + 7 * i + 1, device=self.device))
+                self.L[i] = L
+                self.C[i] = orbitals_torch(n, o, seed + 7 * i + 1, device=self.device)
                 g = torch.Generator(device=self.device)
                 g.manual_seed(seed + 7 * i + 2)
                 hs = torch.randn((n, n), generator=g, dtype=torch.float64, device=self.device)
                 ii, jj = torch.tril_indices(n, n, device=self.device)
-                self.h.append(((hs + hs.T) / (2.0 * n))[ii, jj].reshape(1, -1).contiguous())
-                self.dC.append(torch.zeros((n, o), dtype=torch.float64, device=self.device))
-                self.dL.append(torch.zeros((M, npair(n)), dtype=torch.float64, device=self.device)
-                               if with_grad_l else None)
-        self.stream.synchronize()
+                self.h[i] = ((hs + hs.T) / (2.0 * n))[ii, jj].reshape(1, -1).contiguous()
+                self.dC[i] = torch.zeros((n, o), dtype=torch.float64, device=self.device)
+                self.dL[i] = (torch.zeros((M, npair(n)), dtype=torch.float64, device=self.device)
+                              if with_grad_l else None)
+        for st in self.lane_streams:
+            st.synchronize()
         self.graph = None
 
     def autotune(self, precision="fp64"):
-        with torch.cuda.stream(self.stream):
-            for c, C in zip(self.ctx, self.C):
+        for i, (c, C) in enumerate(zip(self.ctx, self.C)):
+            with torch.cuda.stream(self.lane_streams[self.lane_of[i]]):
                 c.autotune([C], precision)
-        self.stream.synchronize()
+        for st in self.lane_streams:
+            st.synchronize()
+
+    def _system(self, i, precision):
+        c, C, h = self.ctx[i], self.C[i], self.h[i]
+        dlt = c.density(C, 2.0)                      # fold(P), P = 2 C C^T
+        J, V = c.coulomb(dlt, want_v=True)
+        K = c.exchange([C], precision=precision)     # K[C C^T]
+        G = torch.zeros_like(J)
+        c.fock_update(G, J, [0], [1.0], K, [-1.0])   # G = J - 1/2 K[P]
+        self.E[i:i + 1].copy_(c.energy(h, dlt, G))
+        F = h + G
+        self.dC[i].copy_(c.grad_orbitals(C, F, 4.0))
+        if self.dL[i] is not None:
+            c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i])
 
     def step(self, precision="fp64"):
-        """Forward + backward of every system (queued on the batch stream)."""
+        """Forward + backward of every system: forks from the batch stream to
+        the lane streams and joins back (all queued, nothing synchronised)."""
         with torch.cuda.stream(self.stream):
-            for i, (c, C, h) in enumerate(zip(self.ctx, self.C, self.h)):
-                dlt = c.density(C, 2.0)                      # fold(P), P = 2 C C^T
-                J, V = c.coulomb(dlt, want_v=True)
-                K = c.exchange([C], precision=precision)     # K[C C^T]
-                G = torch.zeros_like(J)
-                c.fock_update(G, J, [0], [1.0], K, [-1.0])   # G = J - 1/2 K[P]
-                self.E[i:i + 1].copy_(c.energy(h, dlt, G))
-                F = h + G
-                self.dC[i].copy_(c.grad_orbitals(C, F, 4.0))
-                if self.dL[i] is not None:
-                    c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i])
+            for st in self.lane_streams:
+                st.wait_stream(self.stream)
+            for i in range(len(self.shapes)):
+                with torch.cuda.stream(self.lane_streams[self.lane_of[i]]):
+                    self._system(i, precision)
+            for st in self.lane_streams:
+                self.stream.wait_stream(st)
 
     def capture(self, precision="fp64"):
         """Warm-up step (sizes the shared workspaces), then one captured step."""
