@@ -49,10 +49,13 @@ struct GemmBatch {
 };
 
 // grid = (tiles_i * tiles_j, batch).  O = alpha * A B (NN) or A B^T (NT).
+// Double-buffered LDS with one barrier per kBK chunk: chunk c is multiplied
+// out of buffer c&1 while chunk c+1 is stored from registers into the other
+// buffer and chunk c+2 is fetched.
 template <bool BT, int EPI>
 __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
-  __shared__ double sA[kTM][kBK + 1];
-  __shared__ double sB[kBK][kTM + 1];
+  __shared__ double sA[2][kTM][kBK + 1];
+  __shared__ double sB[2][kBK][kTM + 1];
   const int ntj = (g.n + kTM - 1) / kTM;
   const int ti = blockIdx.x / ntj, tj = blockIdx.x % ntj;
   if (EPI == 1 && ti < tj) return;  // strictly upper tile of a lower-triangle output
@@ -68,36 +71,56 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  for (int k0 = 0; k0 < g.k; k0 += kBK) {
+  constexpr int PN = kTM * kBK / kThreads;
+  double ra[PN], rb[PN];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int it = 0; it < kTM * kBK / kThreads; ++it) {
+    for (int it = 0; it < PN; ++it) {
       const int e = tid + kThreads * it;
       const int r = e / kBK, kk = e % kBK;  // A: row r, column kk
       const int gi = i0 + r, gk = k0 + kk;
-      sA[r][kk] = (gi < g.m && gk < g.k) ? A[int64_t(gi) * g.lda + gk] : 0.0;
+      ra[it] = (gi < g.m && gk < g.k) ? A[int64_t(gi) * g.lda + gk] : 0.0;
       if (BT) {  // B stored [j][k]: row r of the j tile
         const int gj = j0 + r;
-        sB[kk][r] = (gj < g.n && gk < g.k) ? B[int64_t(gj) * g.ldb + gk] : 0.0;
+        rb[it] = (gj < g.n && gk < g.k) ? B[int64_t(gj) * g.ldb + gk] : 0.0;
       } else {  // B stored [k][j]
         const int kr = e / kTM, jj = e % kTM;
         const int gk2 = k0 + kr, gj = j0 + jj;
-        sB[kr][jj] = (gk2 < g.k && gj < g.n) ? B[int64_t(gk2) * g.ldb + gj] : 0.0;
+        rb[it] = (gk2 < g.k && gj < g.n) ? B[int64_t(gk2) * g.ldb + gj] : 0.0;
       }
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int it = 0; it < PN; ++it) {
+      const int e = tid + kThreads * it;
+      sA[buf][e / kBK][e % kBK] = ra[it];
+      if (BT) sB[buf][e % kBK][e / kBK] = rb[it];
+      else sB[buf][e / kTM][e % kTM] = rb[it];
+    }
+  };
+  const int nch = (g.k + kBK - 1) / kBK;
+  fetch(0);
+  stash(0);
+  if (nch > 1) fetch(kBK);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    const int cur = c & 1;
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
       double af[2], bf[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = sA[wm * 32 + i * 16 + (lane & 15)][kr];
+      for (int i = 0; i < 2; ++i) af[i] = sA[cur][wm * 32 + i * 16 + (lane & 15)][kr];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = sB[kr][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 2; ++j) bf[j] = sB[cur][kr][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = mfma64(af[i], bf[j], acc[i][j]);
     }
+    if (c + 1 < nch) stash(cur ^ 1);
+    if (c + 2 < nch) fetch((c + 2) * kBK);
     __syncthreads();
   }
   // result layout of v_mfma_f64_16x16x4f64: row 4*r + lane/16, column lane%16
