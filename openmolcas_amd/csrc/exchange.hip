@@ -768,7 +768,15 @@ __global__ void k_reduce_k(const double *__restrict__ kpart_all, int n, int tm, 
     const double *__restrict__ src = kpart + t * tm * tm + al * tm + bl;
     const int64_t stride = int64_t(ntiles) * tm * tm;
     double s = 0.0;
-    for (int k = 0; k < nsplit; ++k) s += src[k * stride];
+    int k = 0;
+    for (; k + 8 <= nsplit; k += 8) {  // eight loads in flight, summed in slot order
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = src[(k + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; k < nsplit; ++k) s += src[k * stride];
     out[b] = s;
   }
 }
@@ -1018,6 +1026,12 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
   const int nstrip = n_pad / 64;
   c->k_items = c->x_tiles = 0;
   c->scr = Screen{};
+  // the screening state must not outlive this call on any return path: other
+  // users of the half-transform (puvx, backward) launch unscreened
+  struct ScreenReset {
+    Context *c;
+    ~ScreenReset() { c->scr = Screen{}; }
+  } screen_reset{c};
   if (screen) {
     if (use[MJK_PREC_BF16X3])
       return set_error(c, MJK_ERR_ARG, "exchange: screening is not available for split-BF16");
@@ -1102,7 +1116,6 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
       phase_end(c, MJK_PHASE_EXCHANGE);
     }
   }
-  c->scr = Screen{};  // later launches (puvx, backward) are not screened
   phase_begin(c);
   hipLaunchKernelGGL(k_reduce_k, dim3(unsigned(n), unsigned(nk)), dim3(256), 0, c->stream, kpart, n,
                      tm, ntiles, sets, int(nchunk), klt, ldk);
