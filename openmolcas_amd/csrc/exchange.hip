@@ -323,17 +323,28 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   };
 
   T ra[PN], rb[PN];
+  // Element i of a thread is row f0 + ff0 + STEP*i, column cc; the row's
+  // (vector, bank column) pair is stepped incrementally (one division per chunk).
+  constexpr int STEP = kThreads / TM;
+  const int cc = tid % TM, ff0 = tid / TM;
+  const int64_t row0 = (ja * ldc + off) * int64_t(n_pad);
   auto fetch = [&](int f0) {
+    int f = f0 + ff0;
+    int J = f / opad, kk = f - J * opad;
 #pragma unroll
     for (int i = 0; i < PN; ++i) {
-      const int e = tid + kThreads * i, ff = e / TM, cc = e % TM, f = f0 + ff;
       const bool ok = f < nf;
-      const int fc = ok ? f : 0;  // branch-free masked load
-      const int64_t ro = ((ja + fc / opad) * ldc + off + fc % opad) * int64_t(n_pad);
+      const int64_t ro = ok ? ((ja + J) * ldc + off + kk) * int64_t(n_pad) : row0;  // masked load
       const T va = X[ro + A * TM + cc];
       const T vb = X[ro + B * TM + cc];
       ra[i] = ok ? va : T(0);
       rb[i] = ok ? vb : T(0);
+      f += STEP;
+      kk += STEP;
+      if (kk >= opad) {  // STEP <= 4 <= opad: at most one wrap
+        kk -= opad;
+        ++J;
+      }
     }
   };
   auto stash = [&](int buf) {
