@@ -684,18 +684,28 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
       for (int r = 0; r < 4; ++r) accd[i][j][r] = 0.0;
     }
   float ra[PN], rb[PN];
+  // element i: row f0 + ff0 + STEP*i, column cc (rows stepped incrementally)
+  constexpr int STEP = kThreads / TM;
+  const int cc0 = tid % TM, ff0 = tid / TM;
   auto fetch = [&](int f0) {
+    int f = f0 + ff0;
+    int J = f / opad, kk = f - J * opad;
 #pragma unroll
     for (int i = 0; i < PN; ++i) {
-      const int e = tid + kThreads * i, ff = e / TM, cc = e % TM, f = f0 + ff;
       float va = 0.f, vb = 0.f;
       if (f < nf) {
-        const int64_t ro = ((ja + f / opad) * ldc + off + f % opad) * int64_t(n_pad);
-        va = X[ro + A * TM + cc];
-        if (!diag) vb = X[ro + B * TM + cc];
+        const int64_t ro = ((ja + J) * ldc + off + kk) * int64_t(n_pad);
+        va = X[ro + A * TM + cc0];
+        if (!diag) vb = X[ro + B * TM + cc0];
       }
       ra[i] = va;
       rb[i] = vb;
+      f += STEP;
+      kk += STEP;
+      if (kk >= opad) {  // STEP <= 4 <= opad: at most one wrap
+        kk -= opad;
+        ++J;
+      }
     }
   };
   if (nf > 0) fetch(0);
