@@ -80,53 +80,72 @@ __global__ void k_reduce_v(const double *__restrict__ vpart, int64_t M, int nd, 
   v[i] = s;
 }
 
-// Pass 2.  grid = (ldl/kSeg, nsplit).  Workgroup (c, s) owns pairs
-// [c*kSeg, (c+1)*kSeg) and sums over the vectors of split s:
-// jpart[(s*nd + d)*ldl + p] = sum_J L(J,p) V(J,d).
-template <int ND>
+// Pass 2.  grid = (ceil(ldl/(W*kSeg)), nsplit).  Workgroup (c, s) owns pairs
+// [c*W*kSeg, (c+1)*W*kSeg) (W row segments, so each vector row is read in
+// W*4 KiB runs) and sums over the vectors of split s:
+// jpart[(s*nd + d)*ldl + p] = sum_J L(J,p) V(J,d).  U rows in flight per thread.
+template <int ND, int W, int U>
 __global__ __launch_bounds__(kThreads) void k_coulomb_j(
     const double *__restrict__ L, int64_t ldl, int64_t M, const double *__restrict__ V,
     int nd, int64_t jchunk, double *__restrict__ jpart) {
-  const int64_t p = int64_t(blockIdx.x) * kSeg + 2 * threadIdx.x;
+  const int64_t pb = int64_t(blockIdx.x) * (W * kSeg) + 2 * threadIdx.x;
   const int64_t ja = int64_t(blockIdx.y) * jchunk;
   const int64_t jz = min(ja + jchunk, M);
-  v2d acc[ND];
+  v2d acc[W][ND];
+  bool ok[W];
 #pragma unroll
-  for (int d = 0; d < ND; ++d) acc[d] = v2d{0.0, 0.0};
+  for (int w = 0; w < W; ++w) {
+    ok[w] = pb + w * kSeg < ldl;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) acc[w][d] = v2d{0.0, 0.0};
+  }
   int64_t j = ja;
-  constexpr int U = ND <= 2 ? 8 : 4;  // rows in flight per thread
   for (; j + U <= jz; j += U) {
-    v2d l[U];
+    v2d l[U][W];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      l[u] = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(L + (j + u) * ldl + p));
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        l[u][w] = ok[w] ? __builtin_nontemporal_load(
+                              reinterpret_cast<const v2d *>(L + (j + u) * ldl + pb + w * kSeg))
+                        : v2d{0.0, 0.0};
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
         if (d < nd) {
           const double v = V[(j + u) * nd + d];
-          acc[d].x += l[u].x * v;
-          acc[d].y += l[u].y * v;
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            acc[w][d].x += l[u][w].x * v;
+            acc[w][d].y += l[u][w].y * v;
+          }
         }
       }
     }
   }
   for (; j < jz; ++j) {
-    const v2d l = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(L + j * ldl + p));
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      if (d < nd) {
-        const double v = V[j * nd + d];
-        acc[d].x += l.x * v;
-        acc[d].y += l.y * v;
+    for (int w = 0; w < W; ++w) {
+      if (!ok[w]) continue;
+      const v2d l = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(L + j * ldl + pb + w * kSeg));
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        if (d < nd) {
+          const double v = V[j * nd + d];
+          acc[w][d].x += l.x * v;
+          acc[w][d].y += l.y * v;
+        }
       }
     }
   }
 #pragma unroll
-  for (int d = 0; d < ND; ++d)
-    if (d < nd)
-      *reinterpret_cast<v2d *>(jpart + (int64_t(blockIdx.y) * nd + d) * ldl + p) = acc[d];
+  for (int w = 0; w < W; ++w)
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+      if (ok[w] && d < nd)
+        *reinterpret_cast<v2d *>(jpart + (int64_t(blockIdx.y) * nd + d) * ldl + pb + w * kSeg) =
+            acc[w][d];
 }
 
 // jlt(d*ldj + p) = sum_s jpart[(s*nd + d)*ldl + p], p < np.
@@ -291,10 +310,15 @@ void launch_v(Context *c, const double *D, int nd, double *vpart, int nsplit, in
                      c->nchol, D, nd, pchunk, vpart);
 }
 
+// Row segments per workgroup of the J pass: two for one or two densities
+// (8 KiB runs per vector row, measured faster), one otherwise (register budget).
+constexpr int j_width(int ndt) { return ndt <= 2 ? 2 : 1; }
+
 template <int ND>
 void launch_j(Context *c, const double *V, int nd, double *jpart, int nsplit, int64_t jchunk) {
-  dim3 grid(unsigned(c->ldl / kSeg), unsigned(nsplit));
-  hipLaunchKernelGGL((k_coulomb_j<ND>), grid, dim3(kThreads), 0, c->stream, c->L, c->ldl,
+  constexpr int W = j_width(ND), U = 4;
+  dim3 grid(unsigned(ceil_div(c->ldl / kSeg, W)), unsigned(nsplit));
+  hipLaunchKernelGGL((k_coulomb_j<ND, W, U>), grid, dim3(kThreads), 0, c->stream, c->L, c->ldl,
                      c->nchol, V, nd, jchunk, jpart);
 }
 
@@ -406,8 +430,9 @@ int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
                      vpart, M, nd, nsplit1, V);
   phase_end(c, MJK_PHASE_REDUCE);
 
-  // ---- pass 2: J = L V
-  int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nseg), 1), M));
+  // ---- pass 2: J = L V (about 1024 workgroups of two segments, 2048 of one)
+  const int64_t nxj = ceil_div(nseg, j_width(ndt));
+  int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048 / j_width(ndt), nxj), 1), M));
   const int64_t jchunk = ceil_div(M, nsplit2);
   nsplit2 = int(ceil_div(M, jchunk));
   if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
