@@ -838,6 +838,7 @@ void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T
     case 8: launch_ht<P, 8, 64, 16>(c, L, cb, j0, jn, X); break;
     case 10: launch_ht<P, 10, 64, 16>(c, L, cb, j0, jn, X); break;
     case 12: launch_ht<P, 12, 64, 8>(c, L, cb, j0, jn, X); break;
+    case 14: launch_ht<P, 14, 64, 8>(c, L, cb, j0, jn, X); break;
     default: launch_ht<P, 16, 64, 8>(c, L, cb, j0, jn, X); break;
   }
 }
@@ -875,6 +876,7 @@ void ht_dispatch_b3(Context *c, int ot, const float *L, const __bf16 *ch, const 
     case 8: launch_ht_b3<8, 64>(c, L, ch, cl, j0, jn, X); break;
     case 10: launch_ht_b3<10, 64>(c, L, ch, cl, j0, jn, X); break;
     case 12: launch_ht_b3<12, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 14: launch_ht_b3<14, 64>(c, L, ch, cl, j0, jn, X); break;
     default: launch_ht_b3<16, 64>(c, L, ch, cl, j0, jn, X); break;
   }
 }
@@ -925,7 +927,7 @@ int bank_layout(Context *c, int32_t nk, const int32_t *nocc, std::vector<int> &o
   *tot_out = tot;
   *ot_out = 0;
   if (tot == 0) return MJK_OK;
-  static const int kOT[] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
+  static const int kOT[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16};
   for (int v : kOT)
     if (16 * v >= tot) {
       *ot_out = v;
@@ -988,7 +990,6 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
       if ((rc = fill_zero(c, klt + d * ldk, sizeof(double) * c->np))) return rc;
     return MJK_OK;
   }
-  if (nk > kMaxSets) return set_error(c, MJK_ERR_ARG, "exchange: more than 16 orbital sets");
   const int ldc = 16 * ot;
   bool use[3] = {false, false, false};
   for (const Segment &s : segs)

@@ -111,3 +111,21 @@ def test_maximum_sets_and_densities(gpu):
     with pytest.raises(MjkError):
         jk.coulomb(torch.cat([D, D[:1]]))
     jk.close()
+
+
+@pytest.mark.parametrize("occs", [(50,), (90,), (150,), (190,), (120, 80), (210,), (250,)])
+def test_bank_widths(gpu, occs):
+    """Every width of the concatenated orbital bank (4 .. 16 tiles of 16
+    columns), with one set or two sets at a non-zero column offset."""
+    n, M = 70, 9
+    L = chol_vectors_np(n, M, seed=sum(occs))
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(L)
+    rng = np.random.default_rng(len(occs))
+    Cs = [rng.standard_normal((n, o)) / np.sqrt(n) for o in occs]
+    K = jk.exchange([torch.from_numpy(c) for c in Cs]).cpu().numpy()
+    st = oracle.CholStore.full(L)
+    for d, c in enumerate(Cs):
+        ref = oracle.cho_fscf(st, np.zeros((1, npair(n))), [c.shape[1]], [c], np.zeros(npair(n)), -1.0)[0]
+        assert rel(K[d], ref) < RTOL
+    jk.close()

@@ -24,7 +24,8 @@ def oracle_K(L, C):
                            np.zeros(L.shape[1]), -1.0)[0]
 
 
-@pytest.mark.parametrize("n,M,occs", [(64, 33, [16, 1]), (200, 40, [64, 13]), (300, 64, [150])])
+@pytest.mark.parametrize("n,M,occs", [(64, 33, [16, 1]), (200, 40, [64, 13]), (300, 64, [150]),
+                                      (160, 30, [120, 80])])
 def test_exchange_fp32_matches_fp64_oracle(gpu, n, M, occs):
     L = chol_vectors_np(n, M, seed=3 * n + M)
     Cs = [orbitals_np(n, o, seed=500 + i) for i, o in enumerate(occs)]
@@ -53,7 +54,8 @@ def test_fp32_shadow_refreshed_after_upload(gpu):
     assert np.allclose(k2, 4.0 * k1, rtol=1e-6, atol=1e-9 * np.max(np.abs(k1)))
 
 
-@pytest.mark.parametrize("n,M,occs", [(64, 33, [16, 1]), (200, 40, [64, 13]), (300, 64, [150])])
+@pytest.mark.parametrize("n,M,occs", [(64, 33, [16, 1]), (200, 40, [64, 13]), (300, 64, [150]),
+                                      (160, 30, [120, 80])])
 def test_exchange_bf16x3_matches_fp64_oracle(gpu, n, M, occs):
     L = chol_vectors_np(n, M, seed=5 * n + M)
     Cs = [orbitals_np(n, o, seed=700 + i) for i, o in enumerate(occs)]
