@@ -170,136 +170,162 @@ __global__ void k_pack_dens(const double *__restrict__ dlt, int64_t ldd, int64_t
 }
 
 
-// ---- many densities (3..16): both passes as FP64-MFMA GEMMs ---------------
+// ---- many densities (3..16): both passes on FP64 MFMA ---------------------
 // With nd densities each vector element is used nd times; the per-vector
-// kernels above would re-read every density for every vector.  Here a
-// workgroup stages a tile of L and a tile of the densities (or of V) in LDS and
-// contracts them with v_mfma_f64_16x16x4f64 (A[l%16][l/16], B[l/16][l%16],
-// result row 4*r + l/16, column l%16), so each density element is read once
-// per 64 vectors and each L element once per pass: still HBM-bound, at the
-// single-density byte count.
-constexpr int kVM = 64;   // vectors per workgroup (pass 1)
-constexpr int kVK = 32;   // pairs per LDS chunk (pass 1)
-constexpr int kJP = 256;  // pairs per workgroup (pass 2)
-constexpr int kJK = 16;   // vectors per LDS chunk (pass 2)
-
+// kernels above would re-read every density for every vector.  Both passes
+// below read each L element once per pass and each density (or V) element
+// once per 64 vectors: still HBM-bound, at the single-density byte count.
+// MFMA v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// result register r holds row 4r + l/16, column l%16.
 __device__ inline v4d mfma64(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// grid = (ceil(M/64), nsplit); vpart[(s*M + J)*nd + d] = sum_{p in split s} L[J][p] D[d][p]
-__global__ __launch_bounds__(kThreads) void k_coulomb_v_mfma(
+// Pass 1 as V = L D^T.  grid = (ceil(M/VM), nsplit); workgroup (x, s) stages
+// VM vectors x VK pairs of L (rows read in VK*8-byte runs with 16-byte loads)
+// and the same VK pairs of the densities in LDS, the next chunk being fetched
+// to registers meanwhile.  Row tile rt = w / WPT of 16 vectors is contracted
+// by WPT waves, each taking every WPT-th group of k-steps; their partial
+// tiles are summed through LDS.  vpart[(s*M + J)*nd + d].
+template <int VM, int VK>
+__global__ __launch_bounds__(kThreads) void k_coulomb_v_tile(
     const double *__restrict__ L, int64_t ldl, int64_t M, const double *__restrict__ D, int nd,
     int64_t pchunk, double *__restrict__ vpart) {
-  __shared__ double sL[kVM][kVK + 1];
-  __shared__ double sD[16][kVK + 1];
-  const int64_t J0 = int64_t(blockIdx.x) * kVM;
+  constexpr int RT = VM / 16, WPT = 4 / RT, KS = VK / 4 / WPT;
+  constexpr int NL = VM * VK / 2 / kThreads, NDD = 16 * VK / 2 / kThreads;
+  static_assert(NL >= 1 && NDD >= 1 && KS >= 1, "tile shape");
+  __shared__ double sL[VM][VK + 1];
+  __shared__ double sD[16][VK + 1];
+  const int64_t J0 = int64_t(blockIdx.x) * VM;
   const int64_t p0 = int64_t(blockIdx.y) * pchunk;
   const int64_t p1 = min(p0 + pchunk, ldl);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int rt = w / WPT, wk = w % WPT;
   v4d acc = {0.0, 0.0, 0.0, 0.0};
-  constexpr int NL = kVM * kVK / kThreads, NDD = 16 * kVK / kThreads;
-  double rl[NL], rd[NDD];
+  v2d rl[NL], rd[NDD];
   auto fetch = [&](int64_t pc) {
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
-      const int e = tid + kThreads * i, r = e / kVK, k = e % kVK;
-      const bool ok = (J0 + r < M);
-      const double v = L[(ok ? J0 + r : 0) * ldl + pc + k];
-      rl[i] = ok ? v : 0.0;
+      const int e = tid + kThreads * i, r = e / (VK / 2), k2 = e % (VK / 2);
+      const bool ok = J0 + r < M;
+      const v2d v = __builtin_nontemporal_load(
+          reinterpret_cast<const v2d *>(L + (ok ? J0 + r : 0) * ldl + pc + 2 * k2));
+      rl[i] = ok ? v : v2d{0.0, 0.0};
     }
 #pragma unroll
     for (int i = 0; i < NDD; ++i) {
-      const int e = tid + kThreads * i, d = e / kVK, k = e % kVK;
-      const double v = D[(d < nd ? d : 0) * ldl + pc + k];
-      rd[i] = d < nd ? v : 0.0;
+      const int e = tid + kThreads * i, d = e / (VK / 2), k2 = e % (VK / 2);
+      const v2d v = *reinterpret_cast<const v2d *>(D + (d < nd ? d : 0) * ldl + pc + 2 * k2);
+      rd[i] = d < nd ? v : v2d{0.0, 0.0};
     }
   };
   if (p0 < p1) fetch(p0);
-  for (int64_t pc = p0; pc < p1; pc += kVK) {
+  for (int64_t pc = p0; pc < p1; pc += VK) {
     if (pc != p0) __syncthreads();
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
-      const int e = tid + kThreads * i;
-      sL[e / kVK][e % kVK] = rl[i];
+      const int e = tid + kThreads * i, r = e / (VK / 2), k2 = e % (VK / 2);
+      sL[r][2 * k2] = rl[i].x;
+      sL[r][2 * k2 + 1] = rl[i].y;
     }
 #pragma unroll
     for (int i = 0; i < NDD; ++i) {
-      const int e = tid + kThreads * i;
-      sD[e / kVK][e % kVK] = rd[i];
+      const int e = tid + kThreads * i, d = e / (VK / 2), k2 = e % (VK / 2);
+      sD[d][2 * k2] = rd[i].x;
+      sD[d][2 * k2 + 1] = rd[i].y;
     }
     __syncthreads();
-    if (pc + kVK < p1) fetch(pc + kVK);
+    if (pc + VK < p1) fetch(pc + VK);
 #pragma unroll
-    for (int ks = 0; ks < kVK / 4; ++ks) {
-      const int k = ks * 4 + (lane >> 4);
-      acc = mfma64(sL[16 * w + (lane & 15)][k], sD[lane & 15][k], acc);
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = (ks * WPT + wk) * 4 + (lane >> 4);
+      acc = mfma64(sL[16 * rt + (lane & 15)][k], sD[lane & 15][k], acc);
     }
+  }
+  if constexpr (WPT > 1) {
+    __syncthreads();
+    double *red = &sL[0][0];  // WPT-1 partial tiles per row tile: 4 x 64 doubles each
+    if (wk > 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[((rt * (WPT - 1) + wk - 1) * 4 + r) * 64 + lane] = acc[r];
+    }
+    __syncthreads();
+    if (wk > 0) return;
+#pragma unroll
+    for (int q = 0; q < WPT - 1; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] += red[((rt * (WPT - 1) + q) * 4 + r) * 64 + lane];
   }
   const int d = lane & 15;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int64_t J = J0 + 16 * w + 4 * r + (lane >> 4);
+    const int64_t J = J0 + 16 * rt + 4 * r + (lane >> 4);
     if (J < M && d < nd) vpart[(int64_t(blockIdx.y) * M + J) * nd + d] = acc[r];
   }
 }
 
-// grid = (ldl/256, nsplit); jpart[(s*nd + d)*ldl + p] = sum_{J in split s} V[J][d] L[J][p]
-__global__ __launch_bounds__(kThreads) void k_coulomb_j_mfma(
+// Pass 2 as J^T = V^T L, L streamed from HBM straight into registers.
+// grid = (ldl/512, nsplit).  Wave w of workgroup (x, s) owns the 128 pairs
+// from 512x + 128w; per step of 4 vectors J..J+3, lane l (n = l%16, g = l/16)
+// loads L[J + g][base + 32i + 2n .. + 1] for i < 4 (16 lanes cover 256
+// contiguous bytes of a row), and MFMA (i, c) produces output columns
+// base + 32i + 2n + c.  A operand: V[J + l/16][l%16].  Groups of U steps are
+// fetched one group ahead.  jpart[(s*nd + d)*ldl + p].
+template <int U>
+__global__ __launch_bounds__(kThreads) void k_coulomb_j_stream(
     const double *__restrict__ L, int64_t ldl, int64_t M, const double *__restrict__ V, int nd,
     int64_t jchunk, double *__restrict__ jpart) {
-  __shared__ double sL[kJK][kJP];
-  __shared__ double sV[kJK][16];
-  const int64_t p0 = int64_t(blockIdx.x) * kJP;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int n16 = lane & 15, g = lane >> 4;
+  const int64_t pw = int64_t(blockIdx.x) * 512 + 128 * w + 2 * n16;
   const int64_t ja = int64_t(blockIdx.y) * jchunk;
   const int64_t jz = min(ja + jchunk, M);
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  v4d acc[4];
+  const int ngrp = int(jz > ja ? (jz - ja + 4 * U - 1) / (4 * U) : 0);
+  v4d acc[8];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-  constexpr int NL = kJK * kJP / kThreads;
-  double rl[NL], rv;
-  auto fetch = [&](int64_t jc) {
+  for (int t = 0; t < 8; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  v2d la[U][4], lb[U][4];
+  double va[U], vb[U];
+  auto fetch = [&](int q, v2d(&x)[U][4], double(&v)[U]) {
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int e = tid + kThreads * i, r = e / kJP, k = e % kJP;
-      const bool ok = jc + r < jz;
-      const double v = L[(ok ? jc + r : 0) * ldl + p0 + k];
-      rl[i] = ok ? v : 0.0;
+    for (int u = 0; u < U; ++u) {
+      const int64_t J = ja + int64_t(q) * 4 * U + 4 * u + g;
+      const bool ok = J < jz;
+      const double *__restrict__ row = L + (ok ? J : ja) * ldl + pw;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        x[u][i] = ok ? __builtin_nontemporal_load(reinterpret_cast<const v2d *>(row + 32 * i))
+                     : v2d{0.0, 0.0};
+      v[u] = (ok && n16 < nd) ? V[J * nd + n16] : 0.0;
     }
-    const int r = tid / 16, d = tid % 16;  // 256 threads = 16 x 16
-    const bool ok = (jc + r < jz) && d < nd;
-    const double v = V[(ok ? (jc + r) * nd + d : 0)];
-    rv = ok ? v : 0.0;
   };
-  if (ja < jz) fetch(ja);
-  for (int64_t jc = ja; jc < jz; jc += kJK) {
-    if (jc != ja) __syncthreads();
+  auto body = [&](int q, v2d(&cur)[U][4], double(&vc)[U], v2d(&nxt)[U][4], double(&vn)[U]) {
+    if (q + 1 < ngrp) fetch(q + 1, nxt, vn);
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int e = tid + kThreads * i;
-      sL[e / kJP][e % kJP] = rl[i];
-    }
-    sV[tid / 16][tid % 16] = rv;
-    __syncthreads();
-    if (jc + kJK < jz) fetch(jc + kJK);
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-    for (int ks = 0; ks < kJK / 4; ++ks) {
-      const int k = ks * 4 + (lane >> 4);
-      const double b = sV[k][lane & 15];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) acc[t] = mfma64(sL[k][64 * w + 16 * t + (lane & 15)], b, acc[t]);
+      for (int i = 0; i < 4; ++i) {
+        acc[2 * i] = mfma64(vc[u], cur[u][i].x, acc[2 * i]);
+        acc[2 * i + 1] = mfma64(vc[u], cur[u][i].y, acc[2 * i + 1]);
+      }
+  };
+  if (ngrp > 0) {
+    fetch(0, la, va);
+    for (int q = 0; q < ngrp; q += 2) {
+      body(q, la, va, lb, vb);
+      if (q + 1 < ngrp) body(q + 1, lb, vb, la, va);
     }
   }
-  const int d = lane & 15;
-  if (d < nd) {
+  // acc[t][rr]: density 4rr + g, pair pw + t
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+  for (int rr = 0; rr < 4; ++rr) {
+    const int m = 4 * rr + g;
+    if (m < nd) {
+      double *__restrict__ out = jpart + (int64_t(blockIdx.y) * nd + m) * ldl + pw;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t p = p0 + 64 * w + 16 * t + 4 * r + (lane >> 4);
-        jpart[(int64_t(blockIdx.y) * nd + d) * ldl + p] = acc[t][r];
-      }
+      for (int i = 0; i < 4; ++i)
+        *reinterpret_cast<v2d *>(out + 32 * i) = v2d{acc[2 * i][rr], acc[2 * i + 1][rr]};
+    }
   }
 }
 
@@ -369,34 +395,34 @@ int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
 
   const int64_t nseg = ldl / kSeg;
   if (nd >= 3) {
-    // ---- pass 1 (MFMA): V = L D^T, split over pairs to fill the device
-    const int64_t nblk = ceil_div(M, kVM);
-    const int64_t nk = ldl / kVK;
+    // ---- pass 1: V = L D^T, split over pairs to fill the device
+    constexpr int VM = 64, VK = 64, U = 4;
+    const int64_t nblk = ceil_div(M, VM), nk = ldl / VK;
     int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nblk), 1), nk));
-    const int64_t pchunk = ceil_div(nk, nsplit1) * kVK;
+    const int64_t pchunk = ceil_div(nk, nsplit1) * VK;  // whole chunks: ldl % VK == 0
     nsplit1 = int(ceil_div(ldl, pchunk));
     if ((rc = ensure(c, c->w->vpart, sizeof(double) * nsplit1 * M * nd))) return rc;
     if ((rc = ensure(c, c->w->v, sizeof(double) * M * nd))) return rc;
     double *vpart = static_cast<double *>(c->w->vpart.ptr);
     double *V = v_out ? v_out : static_cast<double *>(c->w->v.ptr);
     phase_begin(c);
-    hipLaunchKernelGGL(k_coulomb_v_mfma, dim3(unsigned(nblk), unsigned(nsplit1)), dim3(kThreads), 0,
-                       c->stream, c->L, ldl, M, Dp, nd, pchunk, vpart);
+    hipLaunchKernelGGL((k_coulomb_v_tile<VM, VK>), dim3(unsigned(nblk), unsigned(nsplit1)),
+                       dim3(kThreads), 0, c->stream, c->L, ldl, M, Dp, nd, pchunk, vpart);
     phase_end(c, MJK_PHASE_COULOMB_V);
     phase_begin(c);
     hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M * nd, 256))), dim3(256), 0, c->stream,
                        vpart, M, nd, nsplit1, V);
     phase_end(c, MJK_PHASE_REDUCE);
-    // ---- pass 2 (MFMA): J = L^T V, split over vectors
-    const int64_t npc = ldl / kJP;
-    int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, npc), 1), ceil_div(M, kJK)));
-    const int64_t jchunk = round_up(ceil_div(M, nsplit2), kJK);
+    // ---- pass 2: J = L^T V, split over vectors
+    const int64_t npc = ldl / 512;
+    int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, npc), 1), ceil_div(M, 4 * U)));
+    const int64_t jchunk = round_up(ceil_div(M, nsplit2), 4 * U);
     nsplit2 = int(ceil_div(M, jchunk));
     if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
     double *jpart = static_cast<double *>(c->w->jpart.ptr);
     phase_begin(c);
-    hipLaunchKernelGGL(k_coulomb_j_mfma, dim3(unsigned(npc), unsigned(nsplit2)), dim3(kThreads), 0,
-                       c->stream, c->L, ldl, M, V, nd, jchunk, jpart);
+    hipLaunchKernelGGL(k_coulomb_j_stream<U>, dim3(unsigned(npc), unsigned(nsplit2)), dim3(kThreads),
+                       0, c->stream, c->L, ldl, M, V, nd, jchunk, jpart);
     phase_end(c, MJK_PHASE_COULOMB_J);
     phase_begin(c);
     hipLaunchKernelGGL(k_reduce_j, dim3(unsigned(ceil_div(c->np, 256)), unsigned(nd)), dim3(256), 0,

@@ -60,7 +60,8 @@ def test_exchange_single_mfma_tile(gpu):
     assert rel(K, oracle_K(L, C)) < RTOL
 
 
-@pytest.mark.parametrize("n,M,nd", [(5, 3, 1), (37, 50, 2), (130, 41, 3), (200, 64, 9)])
+@pytest.mark.parametrize("n,M,nd", [(5, 3, 1), (37, 50, 2), (130, 41, 3), (200, 64, 9), (300, 130, 5),
+                                    (33, 200, 16)])
 def test_coulomb_matches_oracle(gpu, n, M, nd):
     L = chol_vectors_np(n, M, seed=n + M)
     rng = np.random.default_rng(n)
