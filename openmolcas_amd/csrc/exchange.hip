@@ -251,10 +251,12 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   typedef typename P::Acc Acc;
   constexpr int WT = TM / 2;
   constexpr int MT = WT / 16;
-  constexpr int PN = BKF * TM / kThreads;
-  static_assert(BKF * TM % kThreads == 0, "tile shape");
-  __shared__ T sA[2][BKF * TM];
-  __shared__ T sB[2][BKF * TM];
+  constexpr int VW = 16 / int(sizeof(T));  // elements per 16-byte load
+  typedef T VT __attribute__((ext_vector_type(VW)));
+  constexpr int PN = BKF * TM / kThreads / VW;
+  static_assert(PN >= 1 && BKF * TM % (kThreads * VW) == 0, "tile shape");
+  __shared__ __attribute__((aligned(16))) T sA[2][BKF * TM];
+  __shared__ __attribute__((aligned(16))) T sB[2][BKF * TM];
   const int set = find_set(sets, blockIdx.x);
   const int local = blockIdx.x - sets.item0[set];
   const int tile = local % ntiles;
@@ -322,11 +324,12 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
     }
   };
 
-  T ra[PN], rb[PN];
-  // Element i of a thread is row f0 + ff0 + STEP*i, column cc; the row's
-  // (vector, bank column) pair is stepped incrementally (one division per chunk).
-  constexpr int STEP = kThreads / TM;
-  const int cc = tid % TM, ff0 = tid / TM;
+  VT ra[PN], rb[PN];
+  // Piece i of a thread (VW consecutive columns from cc) is row
+  // f0 + ff0 + STEP*i; the row's (vector, bank column) pair is stepped
+  // incrementally (one division per chunk).
+  constexpr int STEP = kThreads * VW / TM;
+  const int cc = (tid % (TM / VW)) * VW, ff0 = tid / (TM / VW);
   const int64_t row0 = (ja * ldc + off) * int64_t(n_pad);
   auto fetch = [&](int f0) {
     int f = f0 + ff0;
@@ -335,13 +338,13 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
     for (int i = 0; i < PN; ++i) {
       const bool ok = f < nf;
       const int64_t ro = ok ? ((ja + J) * ldc + off + kk) * int64_t(n_pad) : row0;  // masked load
-      const T va = X[ro + A * TM + cc];
-      const T vb = X[ro + B * TM + cc];
-      ra[i] = ok ? va : T(0);
-      rb[i] = ok ? vb : T(0);
+      const VT va = *reinterpret_cast<const VT *>(X + ro + A * TM + cc);
+      const VT vb = *reinterpret_cast<const VT *>(X + ro + B * TM + cc);
+      ra[i] = ok ? va : VT(0);
+      rb[i] = ok ? vb : VT(0);
       f += STEP;
       kk += STEP;
-      if (kk >= opad) {  // STEP <= 4 <= opad: at most one wrap
+      while (kk >= opad) {  // opad >= 4: at most STEP / 4 wraps
         kk -= opad;
         ++J;
       }
@@ -349,10 +352,12 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   };
   auto stash = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < PN; ++i) sA[buf][tid + kThreads * i] = ra[i];
+    for (int i = 0; i < PN; ++i)
+      *reinterpret_cast<VT *>(&sA[buf][(ff0 + STEP * i) * TM + cc]) = ra[i];
     if (!diag) {
 #pragma unroll
-      for (int i = 0; i < PN; ++i) sB[buf][tid + kThreads * i] = rb[i];
+      for (int i = 0; i < PN; ++i)
+        *reinterpret_cast<VT *>(&sB[buf][(ff0 + STEP * i) * TM + cc]) = rb[i];
     }
   };
   // Double-buffered LDS: chunk c is multiplied out of buffer c&1 while chunk
