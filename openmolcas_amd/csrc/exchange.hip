@@ -563,8 +563,16 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
     for (int j = 0; j < NB; ++j) acc[t][j] = v4f{0, 0, 0, 0};
 
   if (a0 < n) {
-    float rl[LN];
-    int lo[LN];
+    // L staging: thread t owns LG groups of four consecutive contraction
+    // indices b (one AO column a each) and writes each group as one 8-byte LDS
+    // store per hi/lo array.  Row chunks (packed row b holds a contiguous) put
+    // a on consecutive lanes; column chunks (packed row a holds b contiguous)
+    // put the eight groups of one column on consecutive lanes.
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    constexpr int LG = LN / 4;
+    static_assert(LN % 4 == 0, "staging shape");
+    float rl[LG][4];
+    int lo[LG];
     uint4 rch[CN], rcl[CN];
     auto fetch = [&](int b0) {
 #pragma unroll
@@ -576,28 +584,27 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
           rcl[i] = *reinterpret_cast<const uint4 *>(CbL + src);
         }
       }
-      if (b0 >= a0 + TA) {  // every b > a
+      if (b0 + kB3K <= a0) {  // every b < a
 #pragma unroll
-        for (int i = 0; i < LN; ++i) {
-          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
-          rl[i] = (b < n && a < n) ? Lj[tri0(b) + a] : 0.f;
-          lo[i] = aa * kB3S + bb;
-        }
-      } else if (b0 + kB3K <= a0) {  // every b < a
+        for (int i = 0; i < LG; ++i) {
+          const int g = tid + kThreads * i, q = g % (kB3K / 4), aa = g / (kB3K / 4), a = a0 + aa;
+          const float *__restrict__ row = Lj + tri0(a) + b0 + 4 * q;
 #pragma unroll
-        for (int i = 0; i < LN; ++i) {
-          const int e = tid + kThreads * i, bb = e % kB3K, aa = e / kB3K, b = b0 + bb, a = a0 + aa;
-          rl[i] = (a < n) ? Lj[tri0(a) + b] : 0.f;
-          lo[i] = aa * kB3S + bb;
+          for (int k = 0; k < 4; ++k) rl[i][k] = (a < n) ? row[k] : 0.f;
+          lo[i] = aa * kB3S + 4 * q;
         }
       } else {
 #pragma unroll
-        for (int i = 0; i < LN; ++i) {
-          const int e = tid + kThreads * i, bb = e / TA, aa = e % TA, b = b0 + bb, a = a0 + aa;
-          float v = 0.f;
-          if (b < n && a < n) v = (b >= a) ? Lj[tri0(b) + a] : Lj[tri0(a) + b];
-          rl[i] = v;
-          lo[i] = aa * kB3S + bb;
+        for (int i = 0; i < LG; ++i) {
+          const int g = tid + kThreads * i, aa = g % TA, q = g / TA, a = a0 + aa;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int b = b0 + 4 * q + k;
+            float v = 0.f;
+            if (b < n && a < n) v = (b >= a) ? Lj[tri0(b) + a] : Lj[tri0(a) + b];
+            rl[i][k] = v;
+          }
+          lo[i] = aa * kB3S + 4 * q;
         }
       }
     };
@@ -614,7 +621,18 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
         }
       }
 #pragma unroll
-      for (int i = 0; i < LN; ++i) split_bf16(rl[i], sLh[lo[i]], sLl[lo[i]]);
+      for (int i = 0; i < LG; ++i) {
+        bf16x4 h, l;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          __bf16 hh, ll;
+          split_bf16(rl[i][k], hh, ll);
+          h[k] = hh;
+          l[k] = ll;
+        }
+        *reinterpret_cast<bf16x4 *>(sLh + lo[i]) = h;
+        *reinterpret_cast<bf16x4 *>(sLl + lo[i]) = l;
+      }
       __syncthreads();
       if (b0 + kB3K < n) fetch(b0 + kB3K);
       const int ko = 8 * (lane >> 4);
@@ -654,7 +672,6 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
     double *__restrict__ kpart_all, int chunk) {
   constexpr int WT = TM / 2;
   constexpr int MT = WT / 16;
-  constexpr int PN = kB3K * TM / kThreads;
   __shared__ __attribute__((aligned(16))) __bf16 sAh[TM * kB3S];
   __shared__ __attribute__((aligned(16))) __bf16 sAl[TM * kB3S];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[TM * kB3S];
@@ -688,40 +705,53 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
 #pragma unroll
       for (int r = 0; r < 4; ++r) accd[i][j][r] = 0.0;
     }
-  float ra[PN], rb[PN];
-  // element i: row f0 + ff0 + STEP*i, column cc (rows stepped incrementally)
-  constexpr int STEP = kThreads / TM;
-  const int cc0 = tid % TM, ff0 = tid / TM;
+  // Thread t stages columns 4cq..4cq+3 of rows RQ*fq .. RQ*fq+RQ-1 of each
+  // chunk (cq = t % (TM/4), fq = t / (TM/4)): RQ 16-byte row loads per operand
+  // and, per column, one LDS write of RQ consecutive contraction indices for
+  // each of hi and lo.  The RQ rows share one vector J (chunk starts, RQ*fq
+  // and opad are multiples of RQ, which divides 4).
+  constexpr int RQ = kB3K * TM / (4 * kThreads);
+  static_assert(RQ == 2 || RQ == 4, "staging shape");
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 bfq __attribute__((ext_vector_type(RQ)));
+  const int cq = tid % (TM / 4), fq = tid / (TM / 4);
+  f4 ra[RQ], rb[RQ];
   auto fetch = [&](int f0) {
-    int f = f0 + ff0;
-    int J = f / opad, kk = f - J * opad;
+    const int f = f0 + RQ * fq;
+    const int J = f / opad, kk = f - J * opad;
+    const bool ok = f < nf;  // nf is a multiple of 4: the group is all in or all out
+    const int64_t ro = ok ? ((ja + J) * ldc + off + kk) * int64_t(n_pad) : 0;
 #pragma unroll
-    for (int i = 0; i < PN; ++i) {
-      float va = 0.f, vb = 0.f;
-      if (f < nf) {
-        const int64_t ro = ((ja + J) * ldc + off + kk) * int64_t(n_pad);
-        va = X[ro + A * TM + cc0];
-        if (!diag) vb = X[ro + B * TM + cc0];
+    for (int k = 0; k < RQ; ++k) {
+      f4 va = f4(0.f), vb = f4(0.f);
+      if (ok) {
+        va = *reinterpret_cast<const f4 *>(X + ro + k * int64_t(n_pad) + A * TM + 4 * cq);
+        if (!diag) vb = *reinterpret_cast<const f4 *>(X + ro + k * int64_t(n_pad) + B * TM + 4 * cq);
       }
-      ra[i] = va;
-      rb[i] = vb;
-      f += STEP;
-      kk += STEP;
-      if (kk >= opad) {  // STEP <= 4 <= opad: at most one wrap
-        kk -= opad;
-        ++J;
+      ra[k] = va;
+      rb[k] = vb;
+    }
+  };
+  auto stash_one = [&](const f4 (&r)[RQ], __bf16 *sh, __bf16 *sl) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      bfq h, l;
+#pragma unroll
+      for (int k = 0; k < RQ; ++k) {
+        __bf16 hh, ll;
+        split_bf16(r[k][c], hh, ll);
+        h[k] = hh;
+        l[k] = ll;
       }
+      *reinterpret_cast<bfq *>(sh + (4 * cq + c) * kB3S + RQ * fq) = h;
+      *reinterpret_cast<bfq *>(sl + (4 * cq + c) * kB3S + RQ * fq) = l;
     }
   };
   if (nf > 0) fetch(0);
   for (int f0 = 0; f0 < nf; f0 += kB3K) {
     if (f0) __syncthreads();
-#pragma unroll
-    for (int i = 0; i < PN; ++i) {
-      const int e = tid + kThreads * i, ff = e / TM, cc = e % TM;
-      split_bf16(ra[i], sAh[cc * kB3S + ff], sAl[cc * kB3S + ff]);
-      if (!diag) split_bf16(rb[i], sBh[cc * kB3S + ff], sBl[cc * kB3S + ff]);
-    }
+    stash_one(ra, sAh, sAl);
+    if (!diag) stash_one(rb, sBh, sBl);
     __syncthreads();
     if (f0 + kB3K < nf) fetch(f0 + kB3K);
     if (!idle) {
