@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kThreads) void k_ht(
   constexpr int CN = BK * LDC / kThreads;
   constexpr int LN = BK * TA / kThreads;
   static_assert(BK * LDC % kThreads == 0 && BK * TA % kThreads == 0, "tile shape");
-  __shared__ T sC[2][BK * LDC];
+  __shared__ __attribute__((aligned(16))) T sC[2][BK * LDC];
   __shared__ T sL[2][BK * SLD];
   const int64_t lb = xcd_logical(nblocks);
   if (lb < 0) return;
@@ -156,7 +156,12 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     for (int j = 0; j < NB; ++j) acc[t][j] = Acc{0, 0, 0, 0};
 
   if (a0 < n) {
-    T rc[CN], rl[LN];
+    // orbital bank rows are contiguous: 16-byte pieces where the tile divides
+    constexpr int VW = (CN * int(sizeof(T))) % 16 == 0 ? 16 / int(sizeof(T)) : 1;
+    typedef T VT __attribute__((ext_vector_type(VW)));
+    constexpr int CV = CN / VW;
+    VT rc[CV];
+    T rl[LN];
     // Element i of a thread: "row-major" chunks (b > a and diagonal blocks)
     // walk e -> (bb = e / TA, aa = e % TA); "column" chunks (b < a, where the
     // packed row a holds b contiguous) walk e -> (bb = e % BK, aa = e / BK).
@@ -164,7 +169,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
       const bool colmajor = b0 + BK <= a0;
       const T *__restrict__ cs = Cb + int64_t(b0) * LDC;
 #pragma unroll
-      for (int i = 0; i < CN; ++i) rc[i] = cs[tid + kThreads * i];
+      for (int i = 0; i < CV; ++i)
+        rc[i] = *reinterpret_cast<const VT *>(cs + (tid + kThreads * i) * VW);
 #pragma unroll
       for (int i = 0; i < LN; ++i) {
         const int e = tid + kThreads * i;
@@ -180,7 +186,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     auto stash = [&](int b0, int buf) {
       const bool colmajor = b0 + BK <= a0;
 #pragma unroll
-      for (int i = 0; i < CN; ++i) sC[buf][tid + kThreads * i] = rc[i];
+      for (int i = 0; i < CV; ++i)
+        *reinterpret_cast<VT *>(&sC[buf][(tid + kThreads * i) * VW]) = rc[i];
 #pragma unroll
       for (int i = 0; i < LN; ++i) {
         const int e = tid + kThreads * i;
