@@ -226,8 +226,11 @@ def extra_workloads(dev):
     K = jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
     Gp = torch.zeros_like(J)
     jk.fock_update(Gp, J, [0], [1.0], K, [-1.0])
+    dC64 = jk.grad_orbitals(C, G, 4.0)
+    dC_ladder = jk.grad_orbitals(C, Gp, 4.0)
     res["ladder"] = {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "exchange_ms": round(1e3 * t, 3),
                      "dE": abs(jk.energy(h, dlt, Gp).item() - e64),
+                     "dEdC_maxabs_err": float(torch.max(torch.abs(dC_ladder - dC64)).item()),
                      "speedup_vs_fp64": round(res["exchange_fp64_ms"] / (1e3 * t), 3)}
     dl = torch.empty((M, npair(n)), dtype=torch.float64, device=dev)
     res["backward_dC_ms"] = round(1e3 * _timeit(lambda: jk.grad_orbitals(C, G, 4.0), 3), 3)

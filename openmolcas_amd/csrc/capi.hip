@@ -586,7 +586,7 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
   }
   // orbitals to the orbital-major layout the exchange engine reads
   std::vector<double> corb(static_cast<size_t>(tot));
-  for (int d = 0, o0 = 0; d < nden; o0 += nocc[d] * int32_t(n), ++d)
+  for (int64_t d = 0, o0 = 0; d < nden; o0 += int64_t(nocc[d]) * n, ++d)
     for (int64_t k = 0; k < nocc[d]; ++k)
       for (int64_t b = 0; b < n; ++b)
         corb[size_t(o0 + k + nocc[d] * b)] = porb[size_t(d) * n * n + b + n * k];
