@@ -60,7 +60,8 @@ def cpu_baseline(n, M, o, seed, sample, repeats=5):
     dlt = fold(C @ C.T)
     st = oracle.CholStore.full(L)
     flt = np.zeros((1, npair(n)))
-    oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)  # warm
+    want = oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)  # warm; the parity reference
+    ref = (L, C, dlt, want)
     times = []
     for _ in range(repeats):
         t0 = time.perf_counter()
@@ -76,7 +77,25 @@ def cpu_baseline(n, M, o, seed, sample, repeats=5):
         "sample": f"oracle CHO_FSCF restatement (C, OpenMP, {threads} threads) on {ms} of {M} vectors, "
                   f"nbas={n} nocc={o}, batch 256 vectors, median of {repeats}: {dt:.2f} s per build"
                   + (f", scaled x{M / ms:.1f}" if ms < M else ""),
-    }
+    }, ref
+
+
+def gpu_parity(ref, dev):
+    """The FP64 drop-in (mjk_cho_fscf) on the CPU baseline's sample against the
+    oracle's FLT of the same inputs: max-abs error of F = J[D] - 1/2 K[D]."""
+    import numpy as np
+
+    from openmolcas_amd.fock import JKContext
+
+    L, C, dlt, want = ref
+    jk = JKContext(C.shape[0], L.shape[0], device=dev)
+    jk.upload_packed(L)
+    got = np.zeros_like(want)
+    jk.cho_fscf(1, got, [0], [C.shape[1]], [C], dlt, 0.5)
+    jk.close()
+    return {"against": "oracle CHO_FSCF FLT on the cpu_baseline sample (FP64 drop-in)",
+            "flt_maxabs_err": float(np.max(np.abs(got - want))),
+            "flt_maxabs": float(np.max(np.abs(want)))}
 
 
 def energy_of(ctx, dlt, h, J, K):
@@ -437,7 +456,8 @@ def main():
     if args.precision == "fp64" and not args.no_mixed and not args.extra:
         out["mixed"] = mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_step, args)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, M, o, args.seed, args.cpu_sample_vectors)
+        out["cpu_baseline"], ref = cpu_baseline(n, M, o, args.seed, args.cpu_sample_vectors)
+        out["parity"] = gpu_parity(ref, dev)
     print(json.dumps(out), flush=True)
     if not args.extra:
         ctx.close()
