@@ -162,6 +162,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     constexpr int CV = CN / VW;
     VT rc[CV];
     T rl[LN];
+    uint32_t lok = 0;  // bit i: element i of rl lies inside the matrix
+    static_assert(LN <= 32, "element mask");
     // Element i of a thread: "row-major" chunks (b > a and diagonal blocks)
     // walk e -> (bb = e / TA, aa = e % TA); "column" chunks (b < a, where the
     // packed row a holds b contiguous) walk e -> (bb = e % BK, aa = e / BK).
@@ -179,8 +181,10 @@ __global__ __launch_bounds__(kThreads) void k_ht(
         const bool ok = (b < n) && (a < n);
         const int hi = b >= a ? b : a, lo = b >= a ? a : b;
         const int64_t idx = ok ? tri0(hi) + lo : 0;  // branch-free masked load
-        const T v = Lj[idx];
-        rl[i] = ok ? v : T(0);
+        rl[i] = Lj[idx];
+        // the mask is applied when the chunk is stashed, so the loads stay in
+        // flight across the multiply of the current chunk
+        lok = ok ? (lok | (1u << i)) : (lok & ~(1u << i));
       }
     };
     auto stash = [&](int b0, int buf) {
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void k_ht(
       for (int i = 0; i < LN; ++i) {
         const int e = tid + kThreads * i;
         const int bb = colmajor ? e % BK : e / TA, aa = colmajor ? e / BK : e % TA;
-        sL[buf][bb * SLD + aa] = rl[i];
+        sL[buf][bb * SLD + aa] = ((lok >> i) & 1u) ? rl[i] : T(0);
       }
     };
     // Double-buffered LDS: chunk c is multiplied out of buffer c&1 while
@@ -332,6 +336,8 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   };
 
   VT ra[PN], rb[PN];
+  uint32_t rok = 0;  // bit i: piece i lies inside the slice (applied at stash time)
+  static_assert(PN <= 32, "piece mask");
   // Piece i of a thread (VW consecutive columns from cc) is row
   // f0 + ff0 + STEP*i; the row's (vector, bank column) pair is stepped
   // incrementally (one division per chunk).
@@ -345,10 +351,9 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
     for (int i = 0; i < PN; ++i) {
       const bool ok = f < nf;
       const int64_t ro = ok ? ((ja + J) * ldc + off + kk) * int64_t(n_pad) : row0;  // masked load
-      const VT va = *reinterpret_cast<const VT *>(X + ro + A * TM + cc);
-      const VT vb = *reinterpret_cast<const VT *>(X + ro + B * TM + cc);
-      ra[i] = ok ? va : VT(0);
-      rb[i] = ok ? vb : VT(0);
+      ra[i] = *reinterpret_cast<const VT *>(X + ro + A * TM + cc);
+      rb[i] = *reinterpret_cast<const VT *>(X + ro + B * TM + cc);
+      rok = ok ? (rok | (1u << i)) : (rok & ~(1u << i));
       f += STEP;
       kk += STEP;
       while (kk >= opad) {  // opad >= 4: at most STEP / 4 wraps
@@ -360,11 +365,13 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
   auto stash = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < PN; ++i)
-      *reinterpret_cast<VT *>(&sA[buf][(ff0 + STEP * i) * TM + cc]) = ra[i];
+      *reinterpret_cast<VT *>(&sA[buf][(ff0 + STEP * i) * TM + cc]) =
+          ((rok >> i) & 1u) ? ra[i] : VT(0);
     if (!diag) {
 #pragma unroll
       for (int i = 0; i < PN; ++i)
-        *reinterpret_cast<VT *>(&sB[buf][(ff0 + STEP * i) * TM + cc]) = rb[i];
+        *reinterpret_cast<VT *>(&sB[buf][(ff0 + STEP * i) * TM + cc]) =
+            ((rok >> i) & 1u) ? rb[i] : VT(0);
     }
   };
   // Double-buffered LDS: chunk c is multiplied out of buffer c&1 while chunk
