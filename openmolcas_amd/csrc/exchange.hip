@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 // On diagonal tiles (A == B) only one operand is staged and the MFMA tiles
 // strictly above the diagonal are skipped (those entries are never read).
 template <class P, int TM, int BKF>
-__global__ __launch_bounds__(kThreads) void k_syrk(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_syrk(
     const typename P::T *__restrict__ X, int n_pad, int ldc, int64_t jc, int ntiles,
     const SyrkSets sets, double *__restrict__ kpart_all, int chunk, const Screen sc) {
   typedef typename P::T T;
@@ -398,17 +398,24 @@ __global__ __launch_bounds__(kThreads) void k_syrk(
 #pragma unroll
         for (int j = 0; j < MT; ++j) bf[q][j] = sb[kr * TM + wn * WT + j * 16 + (lane & 15)];
       };
-      ld(0, 0);
+      // LO: a wave on the diagonal of a diagonal tile, MFMA tiles i >= j only
+      // (a compile-time pattern, so the MFMAs issue back to back)
+      auto steps = [&](auto lower_only) {
+        constexpr bool LO = decltype(lower_only)::value;
+        ld(0, 0);
 #pragma unroll
-      for (int ks = 0; ks < BKF / 4; ++ks) {
-        const int q = ks & 1;
-        if (ks + 1 < BKF / 4) ld(ks + 1, q ^ 1);
+        for (int ks = 0; ks < BKF / 4; ++ks) {
+          const int q = ks & 1;
+          if (ks + 1 < BKF / 4) ld(ks + 1, q ^ 1);
 #pragma unroll
-        for (int i = 0; i < MT; ++i)
+          for (int i = 0; i < MT; ++i)
 #pragma unroll
-          for (int j = 0; j < MT; ++j)
-            if (!diag || wm > wn || i >= j) acc[i][j] = P::mma(af[q][i], bf[q][j], acc[i][j]);
-      }
+            for (int j = 0; j < MT; ++j)
+              if (!LO || i >= j) acc[i][j] = P::mma(af[q][i], bf[q][j], acc[i][j]);
+        }
+      };
+      if (!diag || wm > wn) steps(std::false_type{});
+      else steps(std::true_type{});
     }
     if constexpr (P::kFlush) {
       if (((c + 1) * BKF) % kFlushRows == 0) flush();
@@ -785,11 +792,17 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
         bh[j] = *reinterpret_cast<const bf16x8 *>(sbh + r * kB3S + ko);
         bl[j] = *reinterpret_cast<const bf16x8 *>(sbl + r * kB3S + ko);
       }
+      if (!diag || wm > wn) {
 #pragma unroll
-      for (int i = 0; i < MT; ++i)
+        for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int j = 0; j < MT; ++j)
-          if (!diag || wm > wn || i >= j) acc[i][j] = mma_b3(ah[i], al[i], bh[j], bl[j], acc[i][j]);
+          for (int j = 0; j < MT; ++j) acc[i][j] = mma_b3(ah[i], al[i], bh[j], bl[j], acc[i][j]);
+      } else {  // diagonal wave of a diagonal tile: MFMA tiles i >= j
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+          for (int j = 0; j <= i; ++j) acc[i][j] = mma_b3(ah[i], al[i], bh[j], bl[j], acc[i][j]);
+      }
     }
     if ((f0 + kB3K) % kFlushRows == 0) {
 #pragma unroll
