@@ -72,31 +72,44 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
     for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
   constexpr int PN = kTM * kBK / kThreads;
+  static_assert(2 * PN <= 32, "element mask");
   double ra[PN], rb[PN];
+  // Out-of-range elements load element 0 and are zeroed when stashed (bit it
+  // of the mask for A, bit PN + it for B), so a chunk's loads stay in flight
+  // across the multiply of the previous chunk.
+  uint32_t okm = 0;
   auto fetch = [&](int k0) {
+    okm = 0;
 #pragma unroll
     for (int it = 0; it < PN; ++it) {
       const int e = tid + kThreads * it;
       const int r = e / kBK, kk = e % kBK;  // A: row r, column kk
       const int gi = i0 + r, gk = k0 + kk;
-      ra[it] = (gi < g.m && gk < g.k) ? A[int64_t(gi) * g.lda + gk] : 0.0;
+      const bool oka = gi < g.m && gk < g.k;
+      ra[it] = A[oka ? int64_t(gi) * g.lda + gk : 0];
+      bool okb;
       if (BT) {  // B stored [j][k]: row r of the j tile
         const int gj = j0 + r;
-        rb[it] = (gj < g.n && gk < g.k) ? B[int64_t(gj) * g.ldb + gk] : 0.0;
+        okb = gj < g.n && gk < g.k;
+        rb[it] = B[okb ? int64_t(gj) * g.ldb + gk : 0];
       } else {  // B stored [k][j]
         const int kr = e / kTM, jj = e % kTM;
         const int gk2 = k0 + kr, gj = j0 + jj;
-        rb[it] = (gk2 < g.k && gj < g.n) ? B[int64_t(gk2) * g.ldb + gj] : 0.0;
+        okb = gk2 < g.k && gj < g.n;
+        rb[it] = B[okb ? int64_t(gk2) * g.ldb + gj : 0];
       }
+      okm |= (oka ? 1u << it : 0u) | (okb ? 1u << (PN + it) : 0u);
     }
   };
   auto stash = [&](int buf) {
 #pragma unroll
     for (int it = 0; it < PN; ++it) {
       const int e = tid + kThreads * it;
-      sA[buf][e / kBK][e % kBK] = ra[it];
-      if (BT) sB[buf][e % kBK][e / kBK] = rb[it];
-      else sB[buf][e / kTM][e % kTM] = rb[it];
+      const double va = ((okm >> it) & 1u) ? ra[it] : 0.0;
+      const double vb = ((okm >> (PN + it)) & 1u) ? rb[it] : 0.0;
+      sA[buf][e / kBK][e % kBK] = va;
+      if (BT) sB[buf][e % kBK][e / kBK] = vb;
+      else sB[buf][e / kTM][e % kTM] = vb;
     }
   };
   const int nch = (g.k + kBK - 1) / kBK;

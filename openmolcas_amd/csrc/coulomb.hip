@@ -203,20 +203,20 @@ __global__ __launch_bounds__(kThreads) void k_coulomb_v_tile(
   const int rt = w / WPT, wk = w % WPT;
   v4d acc = {0.0, 0.0, 0.0, 0.0};
   v2d rl[NL], rd[NDD];
+  // Out-of-range rows (vectors >= M, densities >= nd) load row 0 and are
+  // zeroed when stashed, so the chunk's loads stay in flight across the
+  // multiply of the previous chunk.
   auto fetch = [&](int64_t pc) {
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
       const int e = tid + kThreads * i, r = e / (VK / 2), k2 = e % (VK / 2);
-      const bool ok = J0 + r < M;
-      const v2d v = __builtin_nontemporal_load(
-          reinterpret_cast<const v2d *>(L + (ok ? J0 + r : 0) * ldl + pc + 2 * k2));
-      rl[i] = ok ? v : v2d{0.0, 0.0};
+      rl[i] = __builtin_nontemporal_load(
+          reinterpret_cast<const v2d *>(L + (J0 + r < M ? J0 + r : 0) * ldl + pc + 2 * k2));
     }
 #pragma unroll
     for (int i = 0; i < NDD; ++i) {
       const int e = tid + kThreads * i, d = e / (VK / 2), k2 = e % (VK / 2);
-      const v2d v = *reinterpret_cast<const v2d *>(D + (d < nd ? d : 0) * ldl + pc + 2 * k2);
-      rd[i] = d < nd ? v : v2d{0.0, 0.0};
+      rd[i] = *reinterpret_cast<const v2d *>(D + (d < nd ? d : 0) * ldl + pc + 2 * k2);
     }
   };
   if (p0 < p1) fetch(p0);
@@ -225,14 +225,15 @@ __global__ __launch_bounds__(kThreads) void k_coulomb_v_tile(
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
       const int e = tid + kThreads * i, r = e / (VK / 2), k2 = e % (VK / 2);
-      sL[r][2 * k2] = rl[i].x;
-      sL[r][2 * k2 + 1] = rl[i].y;
+      const bool ok = J0 + r < M;
+      sL[r][2 * k2] = ok ? rl[i].x : 0.0;
+      sL[r][2 * k2 + 1] = ok ? rl[i].y : 0.0;
     }
 #pragma unroll
     for (int i = 0; i < NDD; ++i) {
       const int e = tid + kThreads * i, d = e / (VK / 2), k2 = e % (VK / 2);
-      sD[d][2 * k2] = rd[i].x;
-      sD[d][2 * k2 + 1] = rd[i].y;
+      sD[d][2 * k2] = d < nd ? rd[i].x : 0.0;
+      sD[d][2 * k2 + 1] = d < nd ? rd[i].y : 0.0;
     }
     __syncthreads();
     if (pc + VK < p1) fetch(pc + VK);
