@@ -16,13 +16,17 @@ reduced-precision segment stays below a share of the energy tolerance, and
 partly cancel, so the linear model can under-predict a tail's error;
 ``tune`` therefore builds the planned ladder, measures its energy error
 against FP64 and tightens the budget until the measured error is within the
-acceptance share of the tolerance.
+acceptance share of the tolerance.  ``tune`` also times one build in each
+reduced mode and leaves split-BF16 out when it is not faster than FP32 on
+this system (FP32 is the more accurate of the two, so it then takes the whole
+reduced tail).
 
 The reference has no mixed-precision exchange: src/fock_util/cho_fscf.F90:
 250,276 runs every vector in FP64.
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -59,7 +63,7 @@ def plan(w, kappa_fp32: float, kappa_bf16: float, tol: float = 1e-6,
     j3 = jb
     while j3 > 0 and kappa_fp32 * (t[j3 - 1] - t[jb]) <= budget:
         j3 -= 1
-    pred = kappa_fp32 * (t[j3] - t[jb]) + kappa_bf16 * t[jb]
+    pred = kappa_fp32 * (t[j3] - t[jb]) + (kappa_bf16 * t[jb] if jb < m else 0.0)
     return Ladder(int(j3), int(jb), float(pred))
 
 
@@ -87,12 +91,29 @@ def calibrate(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, w=None):
     return kappas[0], kappas[1], w, k64
 
 
+def mode_times(jk, corbs, repeats: int = 2):
+    """Seconds per exchange build in FP32 and in split-BF16 on this system."""
+    out = []
+    for prec in ("fp32", "bf16x3"):
+        jk.exchange(corbs, precision=prec)
+        jk.sync()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            jk.exchange(corbs, precision=prec)
+        jk.sync()
+        out.append((time.perf_counter() - t0) / repeats)
+    return out[0], out[1]
+
+
 def tune(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, tol: float = 1e-6,
          share: float = 0.25, accept: float = 0.5, rounds: int = 6) -> Ladder:
     """Calibrate, plan and verify: returns a ladder whose measured |dE|
     against the FP64 build is at most accept * tol (FP64 throughout if no
     ladder within ``rounds`` tightening steps qualifies)."""
     k32, kb3, w, k64 = calibrate(jk, corbs, dlt, factor)
+    t32, tb3 = mode_times(jk, corbs)
+    if tb3 >= t32:
+        kb3 = float("inf")  # no split-BF16 segment: FP32 is as fast and more accurate
     m = int(w.numel())
     s = share
     for _ in range(rounds):
