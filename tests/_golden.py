@@ -22,3 +22,15 @@ def unpack(c):
     vec_rs = c["vector_reduced_set"] or [0] * L.shape[0]
     return (n, L, Cs, np.array(c["dlt"]), np.array(c["flt0"]), c["exfac"], sets, vec_rs,
             np.array(c["flt_expected"]))
+
+
+def fmcscf_cases():
+    with open(PATH) as fh:
+        return json.load(fh)["fmcscf"]
+
+
+def unpack_fmcscf(c):
+    a = {k: np.array(v) for k, v in c.items()
+         if k in ("L_packed", "porb1", "porb2", "porb3", "cmo", "dlt", "flt0", "flt_expected",
+                  "w_expected")}
+    return c["nbas"], c["factxi"], c["nfro"], c["norb"], a

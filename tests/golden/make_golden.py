@@ -11,7 +11,8 @@ All inputs are small integers or halves, so every expected value is an exact
 rational, stored as a float (and as its exact fraction for reference).
 Cases: one density; two densities sharing DLT (UHF-like, nDen = 2); and a
 store with two reduced sets (the second covering half of the pairs), where the
-missing pairs of a vector are zero.
+missing pairs of a vector are zero.  Plus one CHO_FMCSCF case (inactive and
+active Fock matrices and the (pw|xy) block, see fmcscf_case).
 
 Run:  python tests/golden/make_golden.py
 """
@@ -94,8 +95,67 @@ def main():
              rsets=[list(range(npr)), rs1], vec_rs=[0, 0, 1])
     cases.append(c)
     with open(os.path.join(HERE, "jk_exact.json"), "w") as fh:
-        json.dump({"source": "tests/golden/make_golden.py (exact rational arithmetic)", "cases": cases},
-                  fh, indent=1)
+        json.dump({"source": "tests/golden/make_golden.py (exact rational arithmetic)", "cases": cases,
+                   "fmcscf": [fmcscf_case(n, [L1, L2, L3])]}, fh, indent=1)
+
+
+def kmat(n, Ls, C):
+    """K[C] = sum_J L^J C C^T L^J, packed."""
+    o = len(C[0])
+    K = [Fr(0)] * npair(n)
+    for L in Ls:
+        Lf = full(L, n)
+        X = [[sum(Lf[a][c] * C[c][k] for c in range(n)) for k in range(o)] for a in range(n)]
+        for a in range(n):
+            for b in range(a + 1):
+                K[lt_index(a, b)] += sum(X[a][k] * X[b][k] for k in range(o))
+    return K
+
+
+def jmat(n, Ls, dlt):
+    V = [sum(L[p] * dlt[p] for p in range(npair(n))) for L in Ls]
+    return [sum(Ls[j][p] * V[j] for j in range(len(Ls))) for p in range(npair(n))]
+
+
+def fmcscf_case(n, Ls):
+    """CHO_FMCSCF (src/fock_util/cho_fmcscf.F90): FI = FLT0(1) + J[DI] + FactXI K[POrb(1)],
+    FA = FLT0(2) + J[DA] - 1/2 K[POrb(2)], and W_PWXY[(xy, w, p)] = sum_J (pw|J)(xy|J) with
+    (pw|J) = sum_ab L^J_ab CMO(a, nfro+p) Ash(b, w), (xy|J) = sum_ab L^J_ab Ash(a, x) Ash(b, y),
+    xy = y(y+1)/2 + x (x <= y), stored xy-major, then w, then p."""
+    npr = npair(n)
+    Ci = [[Fr(1)], [Fr(-1)], [Fr(0)], [Fr(2)]]
+    Cm = [[Fr(1), Fr(0)], [Fr(0), Fr(1, 2)], [Fr(1), Fr(1)], [Fr(0), Fr(-1)]]
+    Ca = [[Fr(0), Fr(1)], [Fr(1), Fr(0)], [Fr(1), Fr(-1)], [Fr(0), Fr(1)]]
+    cmo = [[Fr(v) for v in row] for row in ((1, 0, 1, 0), (0, 1, 1, -1), (1, 1, 0, 2), (0, -1, 1, 1))]
+    nfro, norb, na = 1, 3, 2
+    factxi = Fr(-1, 2)
+    DI = fold([[2 * sum(Ci[a][k] * Ci[b][k] for k in range(1)) for b in range(n)] for a in range(n)], n)
+    DA = fold([[sum(Cm[a][k] * Cm[b][k] for k in range(2)) for b in range(n)] for a in range(n)], n)
+    flt0 = [[Fr(p, 4) for p in range(npr)], [Fr(-1)] * npr]
+    KI, KA = kmat(n, Ls, Ci), kmat(n, Ls, Cm)
+    JI, JA = jmat(n, Ls, DI), jmat(n, Ls, DA)
+    FI = [flt0[0][p] + JI[p] + factxi * KI[p] for p in range(npr)]
+    FA = [flt0[1][p] + JA[p] - Fr(1, 2) * KA[p] for p in range(npr)]
+    W = []
+    Lfs = [full(L, n) for L in Ls]
+    for y in range(na):
+        for x in range(y + 1):
+            xyJ = [sum(Lf[a][b] * Ca[a][x] * Ca[b][y] for a in range(n) for b in range(n)) for Lf in Lfs]
+            for w in range(na):
+                for pp in range(norb):
+                    W.append(sum(xyJ[j] * sum(Lfs[j][a][b] * cmo[a][nfro + pp] * Ca[b][w]
+                                               for a in range(n) for b in range(n))
+                                 for j in range(len(Ls))))
+    f = lambda v: [float(x) for x in v]
+    return {
+        "name": "fmcscf inactive + active, (pw|xy)", "nbas": n, "nchol": len(Ls),
+        "L_packed": [f(L) for L in Ls], "factxi": float(factxi), "nfro": nfro, "norb": norb,
+        "porb1": [f(r) for r in Ci], "porb2": [f(r) for r in Cm], "porb3": [f(r) for r in Ca],
+        "cmo": [f(r) for r in cmo], "dlt": [f(DI), f(DA)], "flt0": [f(x) for x in flt0],
+        "flt_expected": [f(FI), f(FA)], "w_expected": f(W),
+        "flt_expected_exact": [[str(x) for x in FI], [str(x) for x in FA]],
+        "w_expected_exact": [str(x) for x in W],
+    }
 
 
 if __name__ == "__main__":
