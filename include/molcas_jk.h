@@ -130,6 +130,24 @@ int mjk_cho_fscf(mjk_context *ctx, int32_t nden, double *flt, const int32_t *nfo
                  const int32_t *niorb, const double *porb, const double *dlt,
                  double exfac, int32_t precision);
 
+/* CHO_FSCF (src/fock_util/cho_fscf.F90:14) with point-group symmetry,
+ * nsym in 1..8 irreps of nbas[s] functions (sum = the context's nbas; the
+ * basis is ordered irrep by irrep).  Storage as the reference's DSBA types
+ * flatten it (src/system_util/data_structures.F90:62-70):
+ *   flt, dlt   per density, the LT blocks of irreps 1..nsym back to back
+ *              (FLT(d)%SB(s)%A1), sum_s nbas[s](nbas[s]+1)/2 values;
+ *   nforb,niorb (8, nden) column-major, as nForb(8,nDen) / nIorb(8,nDen);
+ *   porb       per density, per irrep A2(nocc(s), nbas[s]) with
+ *              nocc(s) = nforb + niorb, back to back (Porb(d)%SB(s)%A2).
+ * Vectors of every symmetry JSYM go into the context through reduced sets
+ * whose maps index the full C1 lower triangle (global AO indices).  The
+ * build runs on the C1 engine (the irrep blocks embedded in the full basis):
+ * FLT(d)%SB(s) += J - ExFac K on every irrep block; K and J are block
+ * diagonal because the densities and orbitals are. */
+int mjk_cho_fscf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, int32_t nden,
+                     double *flt, const int32_t *nforb, const int32_t *niorb,
+                     const double *porb, const double *dlt, double exfac, int32_t precision);
+
 /* CHO_LK_SCF (src/fock_util/cho_lk_scf.F90:14), nSym = 1:
  *   FLT(d) += J[PLT(1)] - FactXI * sum_{J,k} L(a,J)[k] L(b,J)[k],  d < nden,
  * nden in {1,2}; porb holds nden square matrices A2(nbas,nbas) back to back,

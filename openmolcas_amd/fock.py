@@ -161,6 +161,33 @@ class JKContext:
         self._check(rc)
         return rc
 
+    def cho_fscf_sym(self, nbas, nden: int, flt: np.ndarray, nforb, niorb, porb,
+                     dlt: np.ndarray, exfac: float, precision="fp64") -> int:
+        """CHO_FSCF with point-group symmetry (nSym = len(nbas) irreps, basis
+        ordered irrep by irrep).
+
+        flt: (nden, sum_s nbas[s](nbas[s]+1)/2) float64, the irrep LT blocks
+        back to back, updated in place; dlt: the same layout (folded).  nforb,
+        niorb: (nden, nsym) counts.  porb[d][s]: (nbas[s], nocc(s)) =
+        Porb(d)%SB(s)%A2(nOcc,nBas) memory.  The vectors (every JSYM) must be
+        in the context already, with reduced-set maps into the C1 triangle."""
+        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
+            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        nsym = len(nbas)
+        nb = np.ascontiguousarray(nbas, dtype=np.int32)
+        nf = np.zeros((max(nden, 1), 8), dtype=np.int32)
+        ni = np.zeros((max(nden, 1), 8), dtype=np.int32)
+        nf[:nden, :nsym] = np.asarray(nforb, dtype=np.int32).reshape(nden, nsym)
+        ni[:nden, :nsym] = np.asarray(niorb, dtype=np.int32).reshape(nden, nsym)
+        parts = [_host_f64(porb[d][k]).reshape(-1) for d in range(nden) for k in range(nsym)]
+        pc = np.concatenate(parts) if parts else np.zeros(1)
+        d = _host_f64(dlt).reshape(-1)
+        rc = self.lib.mjk_cho_fscf_sym(self._h, int(nsym), _ptr(nb), int(nden), _ptr(flt), _ptr(nf),
+                                       _ptr(ni), _ptr(pc), _ptr(d), float(exfac),
+                                       precision_code(precision))
+        self._check(rc)
+        return rc
+
     def cho_fock_dft_red(self, dlt: np.ndarray, flt: np.ndarray) -> int:
         """CHO_FOCK_DFT_RED(irc,DLT,FLT): Coulomb-only Fock matrix, flt (npair,)
         overwritten with J[DLT] (the reference zeroes FLT first)."""
