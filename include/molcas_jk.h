@@ -148,6 +148,19 @@ int mjk_cho_fscf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, int32_
                      double *flt, const int32_t *nforb, const int32_t *niorb,
                      const double *porb, const double *dlt, double exfac, int32_t precision);
 
+/* CHO_LK_SCF (src/fock_util/cho_lk_scf.F90:14) with point-group symmetry:
+ * storage and embedding as mjk_cho_fscf_sym, with porb per density the irrep
+ * blocks Porb(d)%SB(s)%A2(nbas[s], nbas[s]) (AO index fastest, the first
+ * nforb + niorb columns are the orbitals) back to back and plt the folded
+ * total density in irrep LT blocks.  FLT(d)%SB(s) += J[PLT] - FactXI K.  The
+ * KLT scratch of the reference is not returned (its content is folded into
+ * FLT by the reference itself, cho_lk_scf.F90:963-1028); LK screening as for
+ * mjk_cho_lk_scf, with nOcc the total over irreps. */
+int mjk_cho_lk_scf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, int32_t nden,
+                       double *flt, const int32_t *nforb, const int32_t *niorb,
+                       const double *porb, const double *plt, double factxi, int32_t nscreen,
+                       double dmpk, double dfmat, int32_t precision);
+
 /* CHO_LK_SCF (src/fock_util/cho_lk_scf.F90:14), nSym = 1:
  *   FLT(d) += J[PLT(1)] - FactXI * sum_{J,k} L(a,J)[k] L(b,J)[k],  d < nden,
  * nden in {1,2}; porb holds nden square matrices A2(nbas,nbas) back to back,

@@ -167,6 +167,58 @@ static void lk_thresholds(const mjk_context *c, int nden, const int32_t *nocc, d
   }
 }
 
+// Point-group symmetry on the C1 engine: irrep s holds the global AO indices
+// off[s] .. off[s+1]-1; its LT block sits at toff[s] of the symmetry-blocked
+// storage (the DSBA %A0 memory).  Blocks are embedded into (and read back from)
+// the full C1 lower triangle.
+namespace {
+struct SymMap {
+  int nsym = 0;
+  int64_t off[9] = {0}, toff[9] = {0};
+  int64_t ntri() const { return toff[nsym]; }
+  template <class F>
+  void each(F &&f) const {  // f(symmetry-blocked index, full LT index)
+    for (int s = 0; s < nsym; ++s) {
+      const int64_t nb = off[s + 1] - off[s];
+      for (int64_t a = 0; a < nb; ++a)
+        for (int64_t b = 0; b <= a; ++b) {
+          const int64_t ga = off[s] + a, gb = off[s] + b;
+          f(toff[s] + a * (a + 1) / 2 + b, ga * (ga + 1) / 2 + gb);
+        }
+    }
+  }
+};
+
+int sym_map(mjk_context *c, int32_t nsym, const int32_t *nbas, SymMap *m, const char *who) {
+  if (nsym < 1 || nsym > 8 || !nbas)
+    return set_error(c, MJK_ERR_ARG, std::string(who) + ": nsym must be in [1,8]");
+  m->nsym = nsym;
+  for (int s = 0; s < nsym; ++s) {
+    if (nbas[s] < 0) return set_error(c, MJK_ERR_ARG, std::string(who) + ": negative nbas");
+    m->off[s + 1] = m->off[s] + nbas[s];
+    m->toff[s + 1] = m->toff[s] + int64_t(nbas[s]) * (nbas[s] + 1) / 2;
+  }
+  if (m->off[nsym] != c->n)
+    return set_error(c, MJK_ERR_ARG, std::string(who) + ": sum of nbas differs from the context's");
+  return MJK_OK;
+}
+
+// orbital counts per density from the (8, nden) nForb/nIorb arrays
+int sym_counts(mjk_context *c, const SymMap &m, int nden, const int32_t *nforb,
+               const int32_t *niorb, int32_t *nocc, const char *who) {
+  for (int d = 0; d < nden; ++d) {
+    nocc[d] = 0;
+    for (int s = 0; s < m.nsym; ++s) {
+      const int32_t f = nforb[s + 8 * d], i = niorb[s + 8 * d];
+      if (f < 0 || i < 0 || f + i > m.off[s + 1] - m.off[s])
+        return set_error(c, MJK_ERR_ARG, std::string(who) + ": bad orbital count");
+      nocc[d] += f + i;
+    }
+  }
+  return MJK_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
@@ -537,72 +589,91 @@ int mjk_cho_fscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t 
                      const int32_t *nforb, const int32_t *niorb, const double *porb,
                      const double *dlt, double exfac, int32_t precision) {
   if (!c) return MJK_ERR_ARG;
-  if (nsym < 1 || nsym > 8 || !nbas)
-    return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: nsym must be in [1,8]");
   if (nden != 1 && nden != 2)
     return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: invalid parameter nDen");  // cho_fscf.F90:75-78
   if (!flt || !nforb || !niorb || !dlt) return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: null pointer");
-  const int64_t n = c->n, np = c->np;
-  // irrep offsets in the full basis and in the symmetry-blocked LT storage
-  int64_t off[9] = {0}, toff[9] = {0};
-  for (int s = 0; s < nsym; ++s) {
-    if (nbas[s] < 0) return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: negative nbas");
-    off[s + 1] = off[s] + nbas[s];
-    toff[s + 1] = toff[s] + int64_t(nbas[s]) * (nbas[s] + 1) / 2;
-  }
-  if (off[nsym] != n) return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: sum of nbas differs from the context's");
-  const int64_t ntri = toff[nsym];
-  // orbital counts per (irrep, density) and per density
+  SymMap m;
   int32_t nocc[2] = {0, 0};
-  int64_t porb_len[2] = {0, 0};
-  for (int d = 0; d < nden; ++d)
-    for (int s = 0; s < nsym; ++s) {
-      const int32_t f = nforb[s + 8 * d], i = niorb[s + 8 * d];
-      if (f < 0 || i < 0 || f + i > nbas[s])
-        return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: bad orbital count");
-      nocc[d] += f + i;
-      porb_len[d] += int64_t(f + i) * nbas[s];
-    }
-  if ((porb_len[0] + porb_len[1]) > 0 && !porb)
-    return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: null porb");
-  // embed: DLT and FLT blocks into the full LT, the orbitals into orbital-major
-  // A2(nocc_d, n) matrices that vanish outside their irrep
-  auto full_index = [&](int s, int64_t a, int64_t b) {  // a >= b, local to irrep s
-    const int64_t ga = off[s] + a, gb = off[s] + b;
-    return ga * (ga + 1) / 2 + gb;
-  };
+  int rc;
+  if ((rc = sym_map(c, nsym, nbas, &m, "cho_fscf_sym")) ||
+      (rc = sym_counts(c, m, nden, nforb, niorb, nocc, "cho_fscf_sym")))
+    return rc;
+  const int64_t n = c->n, np = c->np, ntri = m.ntri();
+  if (nocc[0] + nocc[1] > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: null porb");
+  // DLT and FLT blocks into the full LT; the orbitals of irrep s into columns of
+  // orbital-major A2(nocc_d, n) matrices that vanish outside the irrep
   std::vector<double> dfull(static_cast<size_t>(np), 0.0), ffull(static_cast<size_t>(np * nden), 0.0);
   std::vector<double> cfull(static_cast<size_t>((int64_t(nocc[0]) + nocc[1]) * n), 0.0);
-  for (int s = 0; s < nsym; ++s)
-    for (int64_t a = 0; a < nbas[s]; ++a)
-      for (int64_t b = 0; b <= a; ++b) {
-        const int64_t t = toff[s] + a * (a + 1) / 2 + b, g = full_index(s, a, b);
-        dfull[size_t(g)] = dlt[t];
-        for (int d = 0; d < nden; ++d) ffull[size_t(d * np + g)] = flt[d * ntri + t];
-      }
+  m.each([&](int64_t t, int64_t g) {
+    dfull[size_t(g)] = dlt[t];
+    for (int d = 0; d < nden; ++d) ffull[size_t(d * np + g)] = flt[d * ntri + t];
+  });
   const double *pd = porb;
   double *cd = cfull.data();
   for (int d = 0; d < nden; ++d) {
     int64_t k0 = 0;  // first orbital of irrep s within density d
     for (int s = 0; s < nsym; ++s) {
-      const int64_t no = nforb[s + 8 * d] + niorb[s + 8 * d];
-      for (int64_t b = 0; b < nbas[s]; ++b)
-        for (int64_t k = 0; k < no; ++k) cd[(k0 + k) + nocc[d] * (off[s] + b)] = pd[k + no * b];
-      pd += no * nbas[s];
+      const int64_t no = nforb[s + 8 * d] + niorb[s + 8 * d], nb = nbas[s];
+      for (int64_t b = 0; b < nb; ++b)
+        for (int64_t k = 0; k < no; ++k) cd[(k0 + k) + nocc[d] * (m.off[s] + b)] = pd[k + no * b];
+      pd += no * nb;
       k0 += no;
     }
     cd += int64_t(nocc[d]) * n;
   }
   const int32_t nf0[2] = {0, 0};
-  int rc = mjk_cho_fscf(c, nden, ffull.data(), nf0, nocc, cfull.data(), dfull.data(), exfac,
-                        precision);
-  if (rc) return rc;
-  for (int s = 0; s < nsym; ++s)
-    for (int64_t a = 0; a < nbas[s]; ++a)
-      for (int64_t b = 0; b <= a; ++b) {
-        const int64_t t = toff[s] + a * (a + 1) / 2 + b, g = full_index(s, a, b);
-        for (int d = 0; d < nden; ++d) flt[d * ntri + t] = ffull[size_t(d * np + g)];
-      }
+  if ((rc = mjk_cho_fscf(c, nden, ffull.data(), nf0, nocc, cfull.data(), dfull.data(), exfac,
+                         precision)))
+    return rc;
+  m.each([&](int64_t t, int64_t g) {
+    for (int d = 0; d < nden; ++d) flt[d * ntri + t] = ffull[size_t(d * np + g)];
+  });
+  return MJK_OK;
+}
+
+int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t nden,
+                       double *flt, const int32_t *nforb, const int32_t *niorb, const double *porb,
+                       const double *plt, double factxi, int32_t nscreen, double dmpk, double dfmat,
+                       int32_t precision) {
+  if (!c) return MJK_ERR_ARG;
+  if (nden != 1 && nden != 2)
+    return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: invalid parameter nDen");  // cho_lk_scf.F90:93-96
+  if (!flt || !nforb || !niorb || !plt) return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: null pointer");
+  SymMap m;
+  int32_t nocc[2] = {0, 0};
+  int rc;
+  if ((rc = sym_map(c, nsym, nbas, &m, "cho_lk_scf_sym")) ||
+      (rc = sym_counts(c, m, nden, nforb, niorb, nocc, "cho_lk_scf_sym")))
+    return rc;
+  const int64_t n = c->n, np = c->np, ntri = m.ntri();
+  if (nocc[0] + nocc[1] > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: null porb");
+  // Porb(d)%SB(s)%A2(nBas(s),nBas(s)), AO index fastest, its first nForb+nIorb
+  // columns the orbitals: they become columns k0(s).. of an n x n matrix
+  std::vector<double> pfull(static_cast<size_t>(np), 0.0), ffull(static_cast<size_t>(np * nden), 0.0);
+  std::vector<double> cfull(static_cast<size_t>(nden * n * n), 0.0);
+  m.each([&](int64_t t, int64_t g) {
+    pfull[size_t(g)] = plt[t];
+    for (int d = 0; d < nden; ++d) ffull[size_t(d * np + g)] = flt[d * ntri + t];
+  });
+  const double *pd = porb;
+  for (int d = 0; d < nden; ++d) {
+    double *cd = cfull.data() + d * n * n;
+    int64_t k0 = 0;
+    for (int s = 0; s < nsym; ++s) {
+      const int64_t no = nforb[s + 8 * d] + niorb[s + 8 * d], nb = nbas[s];
+      for (int64_t k = 0; k < no; ++k)
+        for (int64_t b = 0; b < nb; ++b) cd[(m.off[s] + b) + n * (k0 + k)] = pd[b + nb * k];
+      pd += nb * nb;
+      k0 += no;
+    }
+  }
+  const int32_t nf0[2] = {0, 0};
+  if ((rc = mjk_cho_lk_scf(c, nden, ffull.data(), nullptr, nf0, nocc, cfull.data(), pfull.data(),
+                           factxi, nscreen, dmpk, dfmat, 0, nullptr, precision)))
+    return rc;
+  m.each([&](int64_t t, int64_t g) {
+    for (int d = 0; d < nden; ++d) flt[d * ntri + t] = ffull[size_t(d * np + g)];
+  });
   return MJK_OK;
 }
 

@@ -188,6 +188,37 @@ class JKContext:
         self._check(rc)
         return rc
 
+    def cho_lk_scf_sym(self, nbas, nden: int, flt: np.ndarray, nforb, niorb, porb,
+                       plt: np.ndarray, factxi: float, nscreen: int = 10, dmpk: float = 0.1,
+                       dfmat: float = 0.0, precision="fp64") -> int:
+        """CHO_LK_SCF with point-group symmetry (storage as cho_fscf_sym).
+
+        porb[d][s]: (nbas[s], >= nocc(s)) AO x orbital matrices (the columns of
+        Porb(d)%SB(s)%A2(nBas,nBas)); plt: folded total density in irrep LT
+        blocks; flt updated in place.  No KLT output."""
+        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
+            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        nsym = len(nbas)
+        nb = np.ascontiguousarray(nbas, dtype=np.int32)
+        nf = np.zeros((max(nden, 1), 8), dtype=np.int32)
+        ni = np.zeros((max(nden, 1), 8), dtype=np.int32)
+        nf[:nden, :nsym] = np.asarray(nforb, dtype=np.int32).reshape(nden, nsym)
+        ni[:nden, :nsym] = np.asarray(niorb, dtype=np.int32).reshape(nden, nsym)
+        parts = []
+        for d in range(nden):
+            for k in range(nsym):
+                c = _host_f64(porb[d][k])
+                sq = np.zeros((nbas[k], nbas[k]))
+                sq[:, :c.shape[1]] = c
+                parts.append(sq.T.reshape(-1))  # AO index fastest
+        pc = np.concatenate(parts) if parts else np.zeros(1)
+        p = _host_f64(plt).reshape(-1)
+        rc = self.lib.mjk_cho_lk_scf_sym(self._h, int(nsym), _ptr(nb), int(nden), _ptr(flt), _ptr(nf),
+                                         _ptr(ni), _ptr(pc), _ptr(p), float(factxi), int(nscreen),
+                                         float(dmpk), float(dfmat), precision_code(precision))
+        self._check(rc)
+        return rc
+
     def cho_fock_dft_red(self, dlt: np.ndarray, flt: np.ndarray) -> int:
         """CHO_FOCK_DFT_RED(irc,DLT,FLT): Coulomb-only Fock matrix, flt (npair,)
         overwritten with J[DLT] (the reference zeroes FLT first)."""

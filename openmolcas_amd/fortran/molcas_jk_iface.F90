@@ -15,7 +15,7 @@ module molcas_jk_iface
   private
   public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_cho_fscf, &
             mjk_cho_fmcscf, mjk_cho_lk_scf, mjk_cho_fock_dft_red, cho_fscf_gpu, cho_lk_scf_gpu, &
-            mjk_cho_fscf_sym, cho_fscf_sym_gpu, &
+            mjk_cho_fscf_sym, cho_fscf_sym_gpu, mjk_cho_lk_scf_sym, cho_lk_scf_sym_gpu, &
             cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_lk_casscf, cho_lk_casscf_gpu, jk_gpu_init, jk_gpu_final
 
   integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1
@@ -67,6 +67,16 @@ module molcas_jk_iface
       real(c_double), intent(in) :: porb(*), dlt(*)
       real(c_double), value :: exfac
     end function mjk_cho_fscf_sym
+    integer(c_int) function mjk_cho_lk_scf_sym(ctx, nsym, nbas, nden, flt, nforb, niorb, porb, plt, &
+        factxi, nscreen, dmpk, dfmat, prec) bind(C, name='mjk_cho_lk_scf_sym')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsym, nden, nscreen, prec
+      integer(c_int32_t), intent(in) :: nbas(*), nforb(*), niorb(*)
+      real(c_double), intent(inout) :: flt(*)
+      real(c_double), intent(in) :: porb(*), plt(*)
+      real(c_double), value :: factxi, dmpk, dfmat
+    end function mjk_cho_lk_scf_sym
     integer(c_int) function mjk_cho_lk_casscf(ctx, dlt, flt, msq, w_pwxy, factxi, nfiorb, naorb, nchm, &
         ash1, ash2, do_active, nscreen, dmpk, dfmat, cmo, nfro, norb, exfac, prec) &
         bind(C, name='mjk_cho_lk_casscf')
@@ -155,6 +165,27 @@ contains
     rc = int(mjk_cho_fscf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), FLT, nf, ni, &
                               Porb, DLT, real(ExFac, c_double), MJK_PREC_FP64))
   end subroutine cho_fscf_sym_gpu
+
+  ! CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,dFmat) with nSym
+  ! irreps: FLT, PLT and Porb (MSQ A2(nBas(s),nBas(s)) blocks) as the %A0 memory of the
+  ! DSBA types; KLT is not returned.
+  subroutine cho_lk_scf_sym_gpu(rc, nSym, nBas, nDen, FLT, nForb, nIorb, Porb, PLT, FactXI, nScreen, &
+                                dmpk, dFmat)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nSym, nBas(nSym), nDen, nForb(8, nDen), nIorb(8, nDen), nScreen
+    real(8), intent(inout) :: FLT(*)
+    real(8), intent(in) :: Porb(*), PLT(*), FactXI, dmpk, dFmat
+    integer(c_int32_t) :: nb(8), nf(8, 2), ni(8, 2)
+    nb = 0
+    nf = 0
+    ni = 0
+    nb(1:nSym) = int(nBas(1:nSym), c_int32_t)
+    nf(:, 1:min(nDen, 2)) = int(nForb(:, 1:min(nDen, 2)), c_int32_t)
+    ni(:, 1:min(nDen, 2)) = int(nIorb(:, 1:min(nDen, 2)), c_int32_t)
+    rc = int(mjk_cho_lk_scf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), FLT, nf, ni, &
+                                Porb, PLT, real(FactXI, c_double), int(nScreen, c_int32_t), &
+                                real(dmpk, c_double), real(dFmat, c_double), MJK_PREC_FP64))
+  end subroutine cho_lk_scf_sym_gpu
 
   ! CHO_FOCK_DFT_RED(irc,DLT,FLT), nSym = 1: FLT = J[DLT] (pure DFT, no exchange).
   subroutine cho_fock_dft_red_gpu(irc, DLT, FLT)

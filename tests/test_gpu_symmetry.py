@@ -1,5 +1,6 @@
-"""CHO_FSCF with point-group symmetry (mjk_cho_fscf_sym) against the oracle's
-C1 CHO_FSCF on the same problem written in the full basis.
+"""CHO_FSCF and CHO_LK_SCF with point-group symmetry (mjk_cho_fscf_sym,
+mjk_cho_lk_scf_sym) against the oracle's C1 CHO_FSCF on the same problem
+written in the full basis (unscreened LK equals CHO_FSCF with ExFac = FactXI).
 
 Two irreps (nbas 7 + 4): vectors of JSYM = 1 are block diagonal, vectors of
 JSYM = 2 couple the irreps (off-diagonal block); each lives in its own reduced
@@ -21,8 +22,9 @@ def tri(a, b):
     return hi * (hi + 1) // 2 + lo
 
 
+@pytest.mark.parametrize("lk", [False, True])
 @pytest.mark.parametrize("nden,exfac", [(1, 0.5), (2, 1.0)])
-def test_cho_fscf_symmetry_matches_c1_oracle(gpu, nden, exfac):
+def test_cho_fscf_symmetry_matches_c1_oracle(gpu, nden, exfac, lk):
     rng = np.random.default_rng(7 + nden)
     nbas = [7, 4]
     off = [0, 7]
@@ -91,6 +93,9 @@ def test_cho_fscf_symmetry_matches_c1_oracle(gpu, nden, exfac):
     jk.upload_vectors(0, 0, V1)
     jk.upload_vectors(1, m1, V2)
     got = fsym.copy()
-    jk.cho_fscf_sym(nbas, nden, got, nforb, niorb, porb, dsym, exfac)
+    if lk:
+        jk.cho_lk_scf_sym(nbas, nden, got, nforb, niorb, porb, dsym, exfac, dmpk=0.1, dfmat=1.0)
+    else:
+        jk.cho_fscf_sym(nbas, nden, got, nforb, niorb, porb, dsym, exfac)
     jk.close()
     assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want))
