@@ -42,11 +42,13 @@ def _compile(src: str) -> str:
 FORTRAN = os.path.join(HERE, "fortran")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FORTRAN_BIN = os.path.join(FORTRAN, "bin", "fscf_driver")
+FORTRAN_SYM_BIN = os.path.join(FORTRAN, "bin", "fscf_sym_driver")
 
 
 def build_fortran(verbose: bool = False):
-    """ISO_C_BINDING interface module + the Fortran driver of the CHO_FSCF
-    drop-in (amdflang).  Returns the driver path, or None without a compiler."""
+    """ISO_C_BINDING interface module + the Fortran drivers of the CHO_FSCF
+    drop-in, C1 and with symmetry (amdflang).  Returns the C1 driver path, or
+    None without a compiler."""
     if not os.path.exists(FLANG):
         if verbose:
             print("amdflang not found: Fortran interface not built")
@@ -59,6 +61,8 @@ def build_fortran(verbose: bool = False):
         [FLANG, "-O2", "-J", obj, "-c", os.path.join(FORTRAN, "molcas_jk_iface.F90"), "-o", iface_o],
         [FLANG, "-O2", "-J", obj, os.path.join(FORTRAN, "fscf_driver.F90"), iface_o, "-L" + LIBDIR,
          "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_BIN],
+        [FLANG, "-O2", "-J", obj, os.path.join(FORTRAN, "fscf_sym_driver.F90"), iface_o, "-L" + LIBDIR,
+         "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_SYM_BIN],
     ]
     for cmd in cmds:
         r = subprocess.run(cmd, capture_output=True, text=True)
