@@ -316,9 +316,9 @@ def extra_workloads(dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the J/K build "
-              "shards over Cholesky vectors with one all-reduce of the LT Fock matrix, which is "
-              "not part of this build.", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); the J/K build would "
+              "shard over Cholesky vectors with one all-reduce of the LT Fock matrix, which this "
+              "library does not implement.", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -1,7 +1,7 @@
 """CPU FP64 oracle for the Cholesky J/K Fock build.
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()``
-as the checker, and by ``bench.py``'s ``cpu_baseline`` leg -- never by the
+as the parity reference, and by ``bench.py``'s ``cpu_baseline`` leg -- never by the
 product package.  The C restatement lives in ``mjk_oracle.c`` (each function
 cites the OpenMolcas routine it follows); this module is its ctypes binding plus
 a few numpy helpers that the checks share.

@@ -1,7 +1,7 @@
 /*
  * CPU FP64 restatement of OpenMolcas's Cholesky J/K Fock build.
  *
- * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP kernels and the
+ * TEST INFRASTRUCTURE ONLY: the parity reference for the HIP kernels and the
  * CPU baseline timed beside them.  Nothing in the product path links it.
  *
  * Each function follows the reference routine named in its comment, step for

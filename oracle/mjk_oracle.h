@@ -1,7 +1,7 @@
 /*
  * CPU FP64 restatement of OpenMolcas's Cholesky J/K Fock build.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference for the HIP
  * kernels in openmolcas_amd/csrc and the CPU baseline timed by bench.py.  The
  * product path never links, loads or calls it.
  *
