@@ -53,14 +53,14 @@ __global__ __launch_bounds__(kThreads) void k_coulomb_v(
     for (int d = 0; d < ND; ++d)
       dv[d] = (d < nd) ? *reinterpret_cast<const v2d *>(D + d * ldl + p)
                        : v2d{0.0, 0.0};
+    // rows past M read the last row (their sums are never written): the loads
+    // stay unconditional, so all JB of them are in flight together
 #pragma unroll
     for (int jb = 0; jb < JB; ++jb) {
-      if (j0 + jb < M) {
-        const v2d l = __builtin_nontemporal_load(
-            reinterpret_cast<const v2d *>(L + (j0 + jb) * ldl + p));
+      const int64_t j = j0 + jb < M ? j0 + jb : M - 1;
+      const v2d l = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(L + j * ldl + p));
 #pragma unroll
-        for (int d = 0; d < ND; ++d) acc[jb * ND + d] += l.x * dv[d].x + l.y * dv[d].y;
-      }
+      for (int d = 0; d < ND; ++d) acc[jb * ND + d] += l.x * dv[d].x + l.y * dv[d].y;
     }
   }
   block_sum<JB * ND>(acc, lds);
