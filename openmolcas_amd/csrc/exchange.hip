@@ -123,7 +123,13 @@ __global__ __launch_bounds__(kThreads) void k_ht(
   typedef typename P::T T;
   typedef typename P::Acc Acc;
   constexpr int LDC = 16 * OT;
-  constexpr int NB = TA / 64;            // 16-column MFMA tiles per wave
+  // Wave layout.  Narrow banks: wave w owns AO columns w*TA/4 .. (w+1)*TA/4-1
+  // and all OT row tiles.  Wide banks (one vector, even OT >= 6, TA = 64): 2 x 2
+  // waves, wave w owns row tiles (w>>1)*OT/2 .. and AO columns (w&1)*TA/2 ..,
+  // so a step reads OT/2 + 2 fragments for OT MFMAs instead of OT + 1.
+  constexpr int SPLIT = (NJ == 1 && OT % 2 == 0 && OT >= 6 && TA == 64) ? 2 : 1;
+  constexpr int OTW = OT / SPLIT;         // row tiles per wave
+  constexpr int NB = TA * SPLIT / 64;     // 16-column MFMA tiles per wave
   constexpr int SLD = TA + 1;            // LDS row stride of the L tile
   constexpr int CN = BK * LDC / kThreads;
   constexpr int LN = BK * TA / kThreads;
@@ -135,6 +141,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
   const int64_t jr = lb / ntile_a;
   const int a0 = int(lb % ntile_a) * TA;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int t0 = SPLIT == 2 ? (w >> 1) * OTW : 0;
+  const int c0 = SPLIT == 2 ? (w & 1) * (TA / 2) : w * (TA / 4);
   // vectors of this workgroup: present (J < jc) and not screened out
   bool live[NJ];
   const T *__restrict__ Lq[NJ];
@@ -166,11 +174,11 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     if (!any) return;
   }
 
-  Acc acc[NJ][OT][NB];
+  Acc acc[NJ][OTW][NB];
 #pragma unroll
   for (int q = 0; q < NJ; ++q)
 #pragma unroll
-    for (int t = 0; t < OT; ++t)
+    for (int t = 0; t < OTW; ++t)
 #pragma unroll
       for (int j = 0; j < NB; ++j) acc[q][t][j] = Acc{0, 0, 0, 0};
 
@@ -234,16 +242,16 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     for (int c = 0; c < nch; ++c) {
       const int cur = c & 1;
       const T *__restrict__ sc = sC[cur];
-      T af[2][OT], bf[2][NJ][NB];
+      T af[2][OTW], bf[2][NJ][NB];
       auto ld = [&](int ks, int qq) {
         const int kr = ks * 4 + (lane >> 4);
 #pragma unroll
         for (int q = 0; q < NJ; ++q)
 #pragma unroll
           for (int j = 0; j < NB; ++j)
-            bf[qq][q][j] = sL[cur][q][kr * SLD + w * (TA / 4) + j * 16 + (lane & 15)];
+            bf[qq][q][j] = sL[cur][q][kr * SLD + c0 + j * 16 + (lane & 15)];
 #pragma unroll
-        for (int t = 0; t < OT; ++t) af[qq][t] = sc[kr * LDC + 16 * t + (lane & 15)];
+        for (int t = 0; t < OTW; ++t) af[qq][t] = sc[kr * LDC + 16 * (t0 + t) + (lane & 15)];
       };
       ld(0, 0);
 #pragma unroll
@@ -253,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 #pragma unroll
         for (int q = 0; q < NJ; ++q)
 #pragma unroll
-          for (int t = 0; t < OT; ++t)
+          for (int t = 0; t < OTW; ++t)
 #pragma unroll
             for (int j = 0; j < NB; ++j) acc[q][t][j] = P::mma(af[qq][t], bf[qq][q][j], acc[q][t][j]);
       }
@@ -268,12 +276,12 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     T *__restrict__ xj = X + (NJ * jr + q) * LDC * int64_t(n_pad);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      const int a = a0 + w * (TA / 4) + j * 16 + (lane & 15);
+      const int a = a0 + c0 + j * 16 + (lane & 15);
 #pragma unroll
-      for (int t = 0; t < OT; ++t)
+      for (int t = 0; t < OTW; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          xj[int64_t(16 * t + P::row(lane, r)) * n_pad + a] = acc[q][t][j][r];
+          xj[int64_t(16 * (t0 + t) + P::row(lane, r)) * n_pad + a] = acc[q][t][j][r];
     }
   }
 }
