@@ -84,7 +84,8 @@ int mjk_create_shared(mjk_context **ctx, int32_t nbas, int64_t nchol, mjk_contex
 int mjk_device_vectors(mjk_context *ctx, double **L, int64_t *ldl);
 
 /* Exchange tiling: SYRK tile (0 automatic, 64, 128), SYRK workgroups per
- * launch to aim for (0 automatic), half-transform AO tile (0, 64, 128). */
+ * launch at most (0 automatic; whole rounds of two per CU avoid a tail round),
+ * half-transform AO tile (0, 64, 128). */
 int mjk_set_tuning(mjk_context *ctx, int32_t tile_syrk, int32_t split_target, int32_t ht_ta);
 
 /* Times every exchange tiling for these orbital sets and keeps the fastest

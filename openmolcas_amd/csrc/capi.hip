@@ -243,6 +243,12 @@ int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
     return rc;
   }
   c->own_stream = true;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess &&
+      ncu > 0)
+    c->ncu = ncu;
+  else
+    (void)hipGetLastError();
   const size_t lbytes = sizeof(double) * size_t(nchol) * size_t(c->ldl);
   void *p = nullptr;
   if (hipMalloc(&p, lbytes) != hipSuccess) {

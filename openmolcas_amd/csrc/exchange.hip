@@ -1139,7 +1139,13 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
   if (n_pad % tm) tm = 64;
   const int ntile1 = n_pad / tm;
   const int ntiles = ntile1 * (ntile1 + 1) / 2;
-  const int target = c->split_target ? c->split_target : 1024;
+  // Workgroups per launch: at most the target, rounded down, so that a target of
+  // whole resident rounds (k_syrk keeps two workgroups per CU) leaves no handful
+  // of items to a round of its own.  Default: four rounds while each item still
+  // contracts >= 64 vectors, else two.
+  const int64_t slots = 2 * int64_t(c->ncu);
+  const int target = c->split_target ? c->split_target
+                     : int(jc * ntiles >= 64 * 4 * slots ? 4 * slots : 2 * slots);
   SyrkSets sets{};
   sets.n = nk;
   int64_t kb = 0;
@@ -1149,8 +1155,8 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
   for (int d = 0; d < nk; ++d)
     if (nocc[d] > 0) work += opad[size_t(d)];
   for (int d = 0; d < nk; ++d) {
-    const int64_t want = ceil_div(int64_t(target) * opad[size_t(d)],
-                                  int64_t(ntiles) * std::max<int64_t>(work, 1));
+    const int64_t want = int64_t(target) * opad[size_t(d)] /
+                         (int64_t(ntiles) * std::max<int64_t>(work, 1));
     sets.nsplit[d] = nocc[d] > 0 ? int(std::max<int64_t>(1, std::min<int64_t>(want, jc))) : 0;
     sets.off[d] = off[size_t(d)];
     sets.opad[d] = std::max(opad[size_t(d)], 1);

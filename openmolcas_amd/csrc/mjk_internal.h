@@ -74,6 +74,7 @@ struct Context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int device = 0;
+  int32_t ncu = 256;      // compute units of the device (SYRK work-item rounds)
 
   int64_t x_budget = int64_t(16) << 30;  // bytes for one chunk of X
 

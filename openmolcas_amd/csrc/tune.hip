@@ -1,7 +1,7 @@
 // Autotuner of the exchange tiling: for the orbital sets of one call it times
-// every combination of SYRK tile (64, 128), SYRK workgroups per launch (512 ..
-// 4096) and half-transform AO tile (64, 128) with HIP events and keeps the
-// fastest in the context.  Shapes differ enough between molecules (n 100-600,
+// every combination of SYRK tile (64, 128), SYRK workgroups per launch (1, 2, 4
+// or 8 rounds of two per CU) and half-transform AO tile (64, 128) with HIP
+// events and keeps the fastest in the context.  Shapes differ enough between molecules (n 100-600,
 // o 10-120) that no single tiling wins everywhere.
 #include "mjk_internal.h"
 
@@ -19,7 +19,7 @@ extern "C" int mjk_autotune(mjk_context *c, int32_t nk, const int32_t *nocc,
   if ((rc = hip_check(c, hipEventCreate(&e0), "event")) || (rc = hip_check(c, hipEventCreate(&e1), "event")))
     return rc;
   const int tms[] = {64, 128};
-  const int splits[] = {512, 1024, 2048, 4096};
+  const int splits[] = {2 * c->ncu, 4 * c->ncu, 8 * c->ncu, 16 * c->ncu};
   const int tas[] = {64, 128};
   float best = 1e30f;
   int bt = 0, bs = 0, ba = 0;
