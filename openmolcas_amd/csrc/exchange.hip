@@ -1139,13 +1139,6 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
   if (n_pad % tm) tm = 64;
   const int ntile1 = n_pad / tm;
   const int ntiles = ntile1 * (ntile1 + 1) / 2;
-  // Workgroups per launch: at most the target, rounded down, so that a target of
-  // whole resident rounds (k_syrk keeps two workgroups per CU) leaves no handful
-  // of items to a round of its own.  Default: four rounds while each item still
-  // contracts >= 64 vectors, else two.
-  const int64_t slots = 2 * int64_t(c->ncu);
-  const int target = c->split_target ? c->split_target
-                     : int(jc * ntiles >= 64 * 4 * slots ? 4 * slots : 2 * slots);
   SyrkSets sets{};
   sets.n = nk;
   int64_t kb = 0;
@@ -1154,6 +1147,15 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
   int64_t work = 0;
   for (int d = 0; d < nk; ++d)
     if (nocc[d] > 0) work += opad[size_t(d)];
+  // Workgroups per launch: at most the target, rounded down, so that a target of
+  // whole resident rounds (k_syrk keeps two workgroups per CU) leaves no handful
+  // of items to a round of its own.  Default: the most rounds (1, 2, 4 or 8) that
+  // still give every item >= 2048 contraction rows; shorter items lose more to
+  // their pipeline fill and store-out than finer balancing gains.
+  const int64_t slots = 2 * int64_t(c->ncu);
+  int rounds = 1;
+  while (rounds < 8 && jc * work * ntiles >= int64_t(2048) * slots * (2 * rounds)) rounds *= 2;
+  const int target = c->split_target ? c->split_target : int(slots * rounds);
   for (int d = 0; d < nk; ++d) {
     const int64_t want = int64_t(target) * opad[size_t(d)] /
                          (int64_t(ntiles) * std::max<int64_t>(work, 1));
