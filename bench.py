@@ -34,6 +34,8 @@ def parse():
     p.add_argument("--nocc", type=int, default=64)
     p.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--two-pass", action="store_true",
+                   help="FP64: run the Coulomb V pass on its own instead of inside the half-transform")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--extra", action="store_true",
@@ -347,9 +349,16 @@ def main():
     F = torch.empty((1, npr), dtype=torch.float64, device=dev)
     jsrc, fj, fk = [0], [1.0], [-0.5]
 
+    # FP64: J and K in one pass over the vectors (V = L^T D inside the
+    # half-transform), then the J = L V pass
+    fused = args.precision == "fp64" and not args.two_pass
+
     def step():
-        J = ctx.coulomb(dlt)
-        K = ctx.exchange([C], precision=args.precision)
+        if fused:
+            J, K = ctx.coulomb_exchange(dlt, [C])
+        else:
+            J = ctx.coulomb(dlt)
+            K = ctx.exchange([C], precision=args.precision)
         F.copy_(h)
         ctx.fock_update(F, J, jsrc, fj, K, fk)
         return J, K
@@ -370,7 +379,9 @@ def main():
     ms_step = 1000.0 * elapsed / args.steps
 
     # algorithmic work (per build)
-    bytes_j = 2.0 * M * npr * 8 + 2.0 * npr * 8 + 2.0 * M * 8
+    # the Coulomb passes of their own: V and J (two reads of L), or J alone when
+    # V rides in the half-transform
+    bytes_j = (1.0 if fused else 2.0) * M * npr * 8 + 2.0 * npr * 8 + 2.0 * M * 8
     flops_ht = 2.0 * M * o * n * n
     flops_syrk = 1.0 * M * o * n * (n + 1)
     peak = PEAK_TFLOPS[args.precision]
@@ -381,7 +392,9 @@ def main():
 
     ht_ms, ht_calls = avg("half_transform")
     sy_ms, sy_calls = avg("exchange")
-    jv_ms, _ = avg("coulomb_v")
+    jv_ms, jv_calls = avg("coulomb_v")
+    if not jv_calls:
+        jv_ms = 0.0
     jj_ms, _ = avg("coulomb_j")
     ht_per_launch = flops_ht / (ht_calls / args.steps)
     sy_per_launch = flops_syrk / (sy_calls / args.steps)
@@ -432,6 +445,7 @@ def main():
             "coulomb_J_GBs": round(j_gbs, 1),
             "coulomb_J_frac_hbm": round(j_gbs / HBM_PEAK_GBS, 4),
             "coulomb_ms": round(jv_ms + jj_ms, 4),
+            "coulomb_v_in_half_transform": fused,
             "half_transform_TFs": round(ht_tf, 2),
             "half_transform_ms": round(ht_ms, 4),
             "syrk_TFs": round(sy_tf, 2),

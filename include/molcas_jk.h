@@ -245,6 +245,18 @@ int mjk_exchange(mjk_context *ctx, int32_t nk, const int32_t *nocc,
                  const double *const *corb, double *klt, int64_t ldk,
                  int32_t precision);
 
+/* Coulomb of one folded density and FP64 exchange of nk orbital sets in one
+ * pass over the vectors: jlt as mjk_coulomb (nd = 1), klt as mjk_exchange in
+ * FP64.  The V_J = sum_p L^J_p D_p pass runs inside the half-transform, which
+ * reads every L^J anyway, so the vectors are streamed once for V and X and
+ * once for J.  v_out (optional, device, nchol values) receives V.  One vector
+ * loop of CHO_FSCF: the DGEMV pair and Cho_X_getVtra + DGEMM_Tri
+ * (src/fock_util/cho_fscf.F90:208,215,250,276). */
+int mjk_coulomb_exchange(mjk_context *ctx, const double *dlt, int64_t ldd, double *jlt,
+                         int64_t ldj, int32_t nk, const int32_t *nocc,
+                         const double *const *corb, double *klt, int64_t ldk,
+                         double *v_out);
+
 /* Exchange as mjk_exchange with a precision ladder over the vector index:
  * J in [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32 and [j_bf16, nchol) in
  * split-BF16 (three products); every partial sum is reduced in FP64.  With

@@ -58,6 +58,7 @@ SIGNATURES = {
                                   _vp, _vp, _vp, _vp, _i32, _i32, _c.c_double, _i32]),
     "mjk_coulomb": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp]),
     "mjk_exchange": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32]),
+    "mjk_coulomb_exchange": (_c.c_int, [_vp, _vp, _i64, _vp, _i64, _i32, _vp, _vp, _vp, _i64, _vp]),
     "mjk_exchange_mixed": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _i64]),
     "mjk_vector_norms": (_c.c_int, [_vp, _vp]),
     "mjk_exchange_screened": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32, _c.c_double,

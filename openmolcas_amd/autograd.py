@@ -27,8 +27,11 @@ class ClosedShellEnergy(torch.autograd.Function):
         C = C.detach().to(torch.float64).contiguous()
         h = h.detach().to(torch.float64).reshape(1, -1).contiguous()
         dlt = jk.density(C, scale=2.0)              # fold(P)
-        J, V = jk.coulomb(dlt, want_v=True)
-        K = jk.exchange([C], precision=precision)   # K[C C^T] = 1/2 K[P]
+        if precision == "fp64":                     # V pass inside the half-transform
+            J, K, V = jk.coulomb_exchange(dlt, [C], want_v=True)
+        else:
+            J, V = jk.coulomb(dlt, want_v=True)
+            K = jk.exchange([C], precision=precision)   # K[C C^T] = 1/2 K[P]
         G = torch.zeros_like(J)
         jk.fock_update(G, J, [0], [1.0], K, [-1.0])  # G = J - 1/2 K[P]
         E = jk.energy(h, dlt, G)

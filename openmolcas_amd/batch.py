@@ -7,7 +7,8 @@ contexts of one lane share one set of device workspaces and that lane's stream
 (mjk_create_shared), so the lanes run concurrently and small systems fill the
 GPU together.  ``step()`` runs, for every system, the closed-shell energy build
 (density, Coulomb, exchange, Fock and energy steps) followed by the
-hand-written backward (dE/dC = 4FC and the packed dE/dL).  ``capture()``
+hand-written backward (dE/dC = 4FC and the packed dE/dL); in FP64 the Coulomb
+V pass runs inside the half-transform (mjk_coulomb_exchange).  ``capture()``
 records one step -- the fork to the lane streams and the join back included --
 into a torch CUDAGraph (a hipGraph on ROCm) after a warm-up step has sized
 every workspace, so a replay is a single graph launch with no allocation, no
@@ -78,8 +79,11 @@ class MoleculeBatch:
     def _system(self, i, precision):
         c, C, h = self.ctx[i], self.C[i], self.h[i]
         dlt = c.density(C, 2.0)                      # fold(P), P = 2 C C^T
-        J, V = c.coulomb(dlt, want_v=True)
-        K = c.exchange([C], precision=precision)     # K[C C^T]
+        if precision == "fp64":                      # V pass inside the half-transform
+            J, K, V = c.coulomb_exchange(dlt, [C], want_v=True)
+        else:
+            J, V = c.coulomb(dlt, want_v=True)
+            K = c.exchange([C], precision=precision)  # K[C C^T]
         G = torch.zeros_like(J)
         c.fock_update(G, J, [0], [1.0], K, [-1.0])   # G = J - 1/2 K[P]
         self.E[i:i + 1].copy_(c.energy(h, dlt, G))

@@ -451,6 +451,18 @@ int mjk_exchange(mjk_context *c, int32_t nk, const int32_t *nocc, const double *
   return exchange(c, nk, nocc, corb, klt, ldk, precision);
 }
 
+int mjk_coulomb_exchange(mjk_context *c, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+                         int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
+                         int64_t ldk, double *v_out) {
+  if (!c) return MJK_ERR_ARG;
+  if (!dlt || !jlt || !nocc || !corb || !klt)
+    return set_error(c, MJK_ERR_ARG, "coulomb_exchange: null pointer");
+  if (nk < 1 || nk > 16) return set_error(c, MJK_ERR_ARG, "exchange: nk must be in [1,16]");
+  for (int d = 0; d < nk; ++d)
+    if (nocc[d] > 0 && !corb[d]) return set_error(c, MJK_ERR_ARG, "exchange: null orbital matrix");
+  return coulomb_exchange(c, dlt, ldd, jlt, ldj, nk, nocc, corb, klt, ldk, v_out);
+}
+
 int mjk_exchange_mixed(mjk_context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
                        double *klt, int64_t ldk, int64_t j_fp32, int64_t j_bf16) {
   if (!c) return MJK_ERR_ARG;
@@ -578,9 +590,13 @@ int mjk_cho_fscf(mjk_context *c, int32_t nden, double *flt, const int32_t *nforb
                                                    hipMemcpyHostToDevice, c->stream),
                                  "copy Porb")))
     return rc;
-  if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
   const double *cptr[2] = {dC, dC + int64_t(nocc[0]) * c->n};
-  if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
+  if (precision == MJK_PREC_FP64) {  // V pass inside the half-transform
+    if ((rc = coulomb_exchange(c, dD, np, dJ, np, nden, nocc, cptr, dK, np, nullptr))) return rc;
+  } else {
+    if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
+    if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
+  }
   const int32_t jsrc[2] = {0, 0};
   const double fj[2] = {1.0, 1.0};                 // FactCI, cho_fscf.F90:64
   const double fk[2] = {-exfac, -exfac};           // FactXI, cho_fscf.F90:65

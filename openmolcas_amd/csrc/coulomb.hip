@@ -169,7 +169,6 @@ __global__ void k_pack_dens(const double *__restrict__ dlt, int64_t ldd, int64_t
   out[d * ldl + p] = (p < np) ? dlt[d * ldd + p] : 0.0;
 }
 
-
 // ---- many densities (3..16): both passes on FP64 MFMA ---------------------
 // With nd densities each vector element is used nd times; the per-vector
 // kernels above would re-read every density for every vector.  Both passes
@@ -377,6 +376,34 @@ int vector_norms(Context *c, double *w) {
   return hip_check(c, hipGetLastError(), "vector_norms launch");
 }
 
+// Pass 2 for one or two densities (ndt = 1, 2) or their padded counts:
+// J = L V (about 1024 workgroups of two segments, 2048 of one), then the
+// fixed-order reduction of the vector slices into jlt.
+static int j_pass(Context *c, int nd, int ndt, const double *V, double *jlt, int64_t ldj) {
+  const int64_t M = c->nchol, ldl = c->ldl, nseg = ldl / kSeg;
+  const int64_t nxj = ceil_div(nseg, j_width(ndt));
+  int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048 / j_width(ndt), nxj), 1), M));
+  const int64_t jchunk = ceil_div(M, nsplit2);
+  nsplit2 = int(ceil_div(M, jchunk));
+  int rc;
+  if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
+  double *jpart = static_cast<double *>(c->w->jpart.ptr);
+  phase_begin(c);
+  switch (ndt) {
+    case 1: launch_j<1>(c, V, nd, jpart, nsplit2, jchunk); break;
+    case 2: launch_j<2>(c, V, nd, jpart, nsplit2, jchunk); break;
+    case 4: launch_j<4>(c, V, nd, jpart, nsplit2, jchunk); break;
+    case 8: launch_j<8>(c, V, nd, jpart, nsplit2, jchunk); break;
+    default: launch_j<16>(c, V, nd, jpart, nsplit2, jchunk); break;
+  }
+  phase_end(c, MJK_PHASE_COULOMB_J);
+  phase_begin(c);
+  hipLaunchKernelGGL(k_reduce_j, dim3(unsigned(ceil_div(c->np, 256)), unsigned(nd)), dim3(256), 0,
+                     c->stream, jpart, ldl, c->np, nd, nsplit2, jlt, ldj);
+  phase_end(c, MJK_PHASE_REDUCE);
+  return hip_check(c, hipGetLastError(), "coulomb launch");
+}
+
 int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
             double *v_out) {
   if (nd < 1 || nd > 16) return set_error(c, MJK_ERR_ARG, "coulomb: nd must be in [1,16]");
@@ -456,28 +483,55 @@ int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
   hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M * nd, 256))), dim3(256), 0, c->stream,
                      vpart, M, nd, nsplit1, V);
   phase_end(c, MJK_PHASE_REDUCE);
+  return j_pass(c, nd, ndt, V, jlt, ldj);
+}
 
-  // ---- pass 2: J = L V (about 1024 workgroups of two segments, 2048 of one)
-  const int64_t nxj = ceil_div(nseg, j_width(ndt));
-  int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048 / j_width(ndt), nxj), 1), M));
-  const int64_t jchunk = ceil_div(M, nsplit2);
-  nsplit2 = int(ceil_div(M, jchunk));
-  if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
-  double *jpart = static_cast<double *>(c->w->jpart.ptr);
-  phase_begin(c);
-  switch (ndt) {
-    case 1: launch_j<1>(c, V, nd, jpart, nsplit2, jchunk); break;
-    case 2: launch_j<2>(c, V, nd, jpart, nsplit2, jchunk); break;
-    case 4: launch_j<4>(c, V, nd, jpart, nsplit2, jchunk); break;
-    case 8: launch_j<8>(c, V, nd, jpart, nsplit2, jchunk); break;
-    default: launch_j<16>(c, V, nd, jpart, nsplit2, jchunk); break;
+int coulomb_exchange(Context *c, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+                     int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
+                     int64_t ldk, double *v_out) {
+  if (ldd < c->np || ldj < c->np) return set_error(c, MJK_ERR_ARG, "coulomb: ld < nbas(nbas+1)/2");
+  if (c->nloaded != c->nchol)
+    return set_error(c, MJK_ERR_SHORT_READ, "coulomb: not all Cholesky vectors were uploaded");
+  if (nk < 1 || !nocc) return set_error(c, MJK_ERR_ARG, "exchange: nk < 1");
+  int64_t tot = 0;
+  for (int d = 0; d < nk; ++d) tot += std::max(nocc[d], 0);
+  int rc;
+  if (tot == 0) {  // no half-transform to carry the V pass
+    if ((rc = coulomb(c, 1, dlt, ldd, jlt, ldj, v_out))) return rc;
+    return exchange(c, nk, nocc, corb, klt, ldk, MJK_PREC_FP64);
   }
-  phase_end(c, MJK_PHASE_COULOMB_J);
+  const int64_t M = c->nchol, ldl = c->ldl;
+  const int ntmax = c->n_pad / 64;  // AO tiles of the narrowest half-transform tile
+  if ((rc = ensure(c, c->w->d, sizeof(double) * ldl)) ||
+      (rc = ensure(c, c->w->vpart, sizeof(double) * size_t(ntmax) * M)) ||
+      (rc = ensure(c, c->w->v, sizeof(double) * M)))
+    return rc;
+  double *Dp = static_cast<double *>(c->w->d.ptr);
+  double *vpart = static_cast<double *>(c->w->vpart.ptr);
+  double *V = v_out ? v_out : static_cast<double *>(c->w->v.ptr);
   phase_begin(c);
-  hipLaunchKernelGGL(k_reduce_j, dim3(unsigned(ceil_div(c->np, 256)), unsigned(nd)), dim3(256), 0,
-                     c->stream, jpart, ldl, c->np, nd, nsplit2, jlt, ldj);
+  hipLaunchKernelGGL(k_pack_dens, dim3(unsigned(ceil_div(ldl, 256)), 1u), dim3(256), 0, c->stream,
+                     dlt, ldd, c->np, ldl, Dp);
+  phase_end(c, MJK_PHASE_PACK);
+  // the fused pass must not outlive this call on any return path
+  struct FuseReset {
+    Context *c;
+    ~FuseReset() { c->fv = FuseV{}; }
+  } fuse_reset{c};
+  c->fv.p = Dp;
+  c->fv.vpart = vpart;
+  c->fv.m = M;
+  c->fv.ntile = 0;
+  if ((rc = exchange(c, nk, nocc, corb, klt, ldk, MJK_PREC_FP64))) return rc;
+  const int ntile = c->fv.ntile;
+  c->fv = FuseV{};
+  if (ntile < 1 || ntile > ntmax)
+    return set_error(c, MJK_ERR_ARG, "coulomb_exchange: the half-transform did not run");
+  phase_begin(c);
+  hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M, 256))), dim3(256), 0, c->stream, vpart,
+                     M, 1, ntile, V);
   phase_end(c, MJK_PHASE_REDUCE);
-  return hip_check(c, hipGetLastError(), "coulomb launch");
+  return j_pass(c, 1, 1, V, jlt, ldj);
 }
 
 }  // namespace mjk

@@ -115,13 +115,19 @@ __global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst
 // (TA/64 MFMA column tiles) and all OT row tiles; each orbital fragment read
 // from LDS feeds the MFMAs of all NJ vectors.  The next b-chunk is fetched
 // into registers while the current one is multiplied out of LDS.
-template <class P, int OT, int TA, int BK, int NJ>
+// FV: the workgroup also sums L^J_ba D_ba over the staged elements with b >= a
+// (only "row-major" chunks hold such elements; each packed element is staged
+// there by exactly one workgroup) into fv.vpart[tile * fv.m + j0 + J] -- the
+// Coulomb pass V = L^T D without a read of L of its own, and the folded
+// density read for half of the chunks only.
+template <class P, int OT, int TA, int BK, int NJ, bool FV>
 __global__ __launch_bounds__(kThreads) void k_ht(
     const typename P::T *__restrict__ L, int64_t ldl, int n, int n_pad,
     const typename P::T *__restrict__ Cb, int64_t j0, int64_t jc, int ntile_a, int64_t nblocks,
-    typename P::T *__restrict__ X, const Screen sc) {
+    typename P::T *__restrict__ X, const Screen sc, const FuseV fv) {
   typedef typename P::T T;
   typedef typename P::Acc Acc;
+  static_assert(!FV || std::is_same<T, double>::value, "fused Coulomb pass is FP64 only");
   constexpr int LDC = 16 * OT;
   // Wave layout.  Narrow banks: wave w owns AO columns w*TA/4 .. (w+1)*TA/4-1
   // and all OT row tiles.  Wide banks (one vector, even OT >= 6, TA = 64): 2 x 2
@@ -181,6 +187,9 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     for (int t = 0; t < OTW; ++t)
 #pragma unroll
       for (int j = 0; j < NB; ++j) acc[q][t][j] = Acc{0, 0, 0, 0};
+  T vp[NJ];  // FV: this thread's share of sum_ab L^J_ab P_ab
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) vp[q] = T(0);
 
   if (a0 < n) {
     // orbital bank rows are contiguous: 16-byte pieces where the tile divides
@@ -189,6 +198,8 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     constexpr int CV = CN / VW;
     VT rc[CV];
     T rl[NJ][LN];
+    T rd[FV ? LN : 1];  // FV: density elements of the staged L elements
+    uint32_t vok = 0;   // FV: bit i: element i lies in the matrix with b >= a
     // bit i: element i lies inside the matrix.  The mask is applied when the
     // chunk is stashed, so the loads stay in flight across the multiply of
     // the current chunk (a select right after the load would wait for it).
@@ -214,6 +225,12 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 #pragma unroll
         for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx];
         lok = ok ? (lok | (1u << i)) : (lok & ~(1u << i));
+        if constexpr (FV) {
+          if (!colmajor) {  // uniform per chunk: no select on the loaded value
+            rd[i] = fv.p[idx];
+            vok = (ok && b >= a) ? (vok | (1u << i)) : (vok & ~(1u << i));
+          }
+        }
       }
     };
     auto stash = [&](int b0, int buf) {
@@ -228,6 +245,13 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 #pragma unroll
         for (int q = 0; q < NJ; ++q)
           sL[buf][q][bb * SLD + aa] = ((lok >> i) & 1u) ? rl[q][i] : T(0);
+        if constexpr (FV) {
+          if (!colmajor) {
+            const T d = ((vok >> i) & 1u) ? rd[i] : T(0);
+#pragma unroll
+            for (int q = 0; q < NJ; ++q) vp[q] += rl[q][i] * d;
+          }
+        }
       }
     };
     // Double-buffered LDS: chunk c is multiplied out of buffer c&1 while
@@ -269,6 +293,25 @@ __global__ __launch_bounds__(kThreads) void k_ht(
       if (c + 2 < nch) fetch((c + 2) * BK);
       __syncthreads();
     }
+  }
+  if constexpr (FV) {  // workgroup sum in a fixed order: wave butterflies, then waves 0..3
+    __shared__ double sV[NJ][kThreads / 64];
+#pragma unroll
+    for (int q = 0; q < NJ; ++q) {
+      double v = vp[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) sV[q][w] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NJ; ++q)
+      if (tid == q && live[q]) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < kThreads / 64; ++k) v += sV[q][k];
+        fv.vpart[int64_t(lb % ntile_a) * fv.m + j0 + NJ * jr + q] = v;
+      }
   }
 #pragma unroll
   for (int q = 0; q < NJ; ++q) {
@@ -912,8 +955,18 @@ void launch_ht(Context *c, const typename P::T *L, const typename P::T *cb, int6
   const int ntile_a = c->n_pad / TA;
   const int64_t nblocks = ceil_div(jc, NJ) * ntile_a;
   const int64_t grid = round_up(nblocks, 8);
-  hipLaunchKernelGGL((k_ht<P, OT, TA, BK, NJ>), dim3(unsigned(grid)), dim3(kThreads), 0, c->stream,
-                     L, c->ldl, c->n, c->n_pad, cb, j0, jc, ntile_a, nblocks, X, c->scr);
+  if constexpr (std::is_same<P, F64>::value) {
+    if (c->fv.p) {  // fused Coulomb pass (never together with screening)
+      c->fv.ntile = ntile_a;
+      hipLaunchKernelGGL((k_ht<P, OT, TA, BK, NJ, true>), dim3(unsigned(grid)), dim3(kThreads), 0,
+                         c->stream, L, c->ldl, c->n, c->n_pad, cb, j0, jc, ntile_a, nblocks, X,
+                         Screen{}, c->fv);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_ht<P, OT, TA, BK, NJ, false>), dim3(unsigned(grid)), dim3(kThreads), 0,
+                     c->stream, L, c->ldl, c->n, c->n_pad, cb, j0, jc, ntile_a, nblocks, X, c->scr,
+                     FuseV{});
   if (c->scr.lnorm) c->x_tiles += jc * ntile_a;
 }
 

@@ -52,6 +52,17 @@ struct Screen {
   unsigned long long *cnt = nullptr;  // [K items skipped, X tiles skipped]
 };
 
+// Coulomb pass V = L^T D fused into the FP64 half-transform (p null = off):
+// workgroup (vector J, AO tile t) adds sum_{a in t} sum_{b >= a} L^J_ba D_ba
+// into vpart[t * m + J], with D the folded density in packed storage (zero
+// padded to ldl), so sum_t vpart[t * m + J] = sum_{b >= a} L^J_ba D_ba.
+struct FuseV {
+  const double *p = nullptr;
+  double *vpart = nullptr;
+  int64_t m = 0;   // row stride of vpart (vector capacity)
+  int ntile = 0;   // AO tiles per vector of the last launch (set by the launcher)
+};
+
 struct ReducedSet {
   int64_t nrs = 0;
   bool identity = false;      // rs position r maps to LT index r for all r
@@ -89,6 +100,7 @@ struct Context {
   double lk_thrcom = 1.0e-6;
   DevBuf scount;                  // device: [K items skipped, X tiles skipped]
   Screen scr;                     // screening of the exchange launch in progress
+  FuseV fv;                       // Coulomb V fused into the exchange in progress
   int64_t k_items = 0, x_tiles = 0;  // work items of the last screened exchange
   std::shared_ptr<Workspace> wsp;  // workspaces, possibly shared with other contexts
   Workspace *w = nullptr;
@@ -131,6 +143,13 @@ int hip_check(Context *c, hipError_t e, const char *what);
 // Coulomb: jlt(d, p) = sum_J L(J,p) * sum_q L(J,q) dlt(d,q), for nd densities.
 int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
             int64_t ldj, double *v_out /* optional [M][nd] */);
+
+// Coulomb (one density) and FP64 exchange in one pass over the vectors: the
+// V = L^T D pass runs inside the half-transform, which reads every L^J anyway;
+// then the J = L V pass.  Same results as coulomb() followed by exchange().
+int coulomb_exchange(Context *c, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+                     int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
+                     int64_t ldk, double *v_out /* optional [M] */);
 
 // Exchange: klt(d) = sum_J sum_k X_d(J,k,a) X_d(J,k,b) (LT), X_d = C_d^T L^J.
 // corb[d] is (nocc[d] x n) with the orbital index fastest, element (k,a) at

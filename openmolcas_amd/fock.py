@@ -340,6 +340,25 @@ class JKContext:
                                           precision_code(precision)))
         return k
 
+    def coulomb_exchange(self, dlt: torch.Tensor, corbs, want_v: bool = False):
+        """J of one folded LT density dlt (npair,) and the FP64 K_d of each
+        orbital matrix C_d in one pass over the vectors (the V = L^T D pass
+        runs inside the half-transform).  Returns (J (1, npair), K (nk, npair))
+        and V (nchol, 1) when want_v."""
+        d = _dev_f64(dlt, self.device).reshape(1, -1)
+        if d.shape[1] != self.npair:
+            raise ValueError("dlt must hold one folded density of nbas(nbas+1)/2 elements")
+        cs = [_dev_f64(c, self.device) for c in corbs]
+        nk = len(cs)
+        nocc = np.array([c.shape[1] if c.dim() == 2 else 0 for c in cs], dtype=np.int32)
+        ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
+        j = torch.empty((1, self.npair), dtype=torch.float64, device=self.device)
+        k = torch.empty((nk, self.npair), dtype=torch.float64, device=self.device)
+        v = torch.empty((self.nchol, 1), dtype=torch.float64, device=self.device) if want_v else None
+        self._check(self.lib.mjk_coulomb_exchange(self._h, _ptr(d), _ld(d), _ptr(j), _ld(j), nk,
+                                                  _ptr(nocc), ptrs, _ptr(k), _ld(k), _ptr(v)))
+        return (j, k, v) if want_v else (j, k)
+
     def exchange_mixed(self, corbs, j_fp32: int, j_bf16: int) -> torch.Tensor:
         """Exchange with vectors [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32
         and [j_bf16, nchol) in split-BF16, reduced in FP64."""
