@@ -771,11 +771,15 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
                                                    hipMemcpyHostToDevice, c->stream),
                                  "copy Porb")))
     return rc;
-  if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
   const double *cptr[2] = {dC, dC + int64_t(nocc[0]) * n};
   double tau = 0.0, thrv = 0.0;
   if (c->lk_on) lk_thresholds(c, nden, nocc, dmpk, dfmat, &tau, &thrv);
-  if ((rc = exchange_screened(c, nden, nocc, cptr, dK, np, precision, tau, thrv))) return rc;
+  if (precision == MJK_PREC_FP64 && tau <= 0.0 && thrv <= 0.0) {  // V pass inside the half-transform
+    if ((rc = coulomb_exchange(c, dD, np, dJ, np, nden, nocc, cptr, dK, np, nullptr))) return rc;
+  } else {
+    if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
+    if ((rc = exchange_screened(c, nden, nocc, cptr, dK, np, precision, tau, thrv))) return rc;
+  }
   const int32_t jsrc[2] = {0, 0};
   const double fj[2] = {1.0, 1.0};               // FactCI, cho_lk_scf.F90:102
   const double fk[2] = {-factxi, -factxi};       // -FactXI, cho_lk_scf.F90:803,830
