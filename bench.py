@@ -135,9 +135,13 @@ def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
         e = energy_of(ctx, dlt, h, J, K)
         res[prec] = {"exchange_ms": round(1e3 * tk[prec], 4), "E": e, "dE": abs(e - e64),
                      "K_maxabs_err": float(torch.max(torch.abs(K - K64)).item())}
+    fk = M * o * (2.0 * n * n + n * (n + 1))
     for prec in tk:
+        res[prec]["exchange_TFs"] = round(fk / tk[prec] / 1e12, 2)
         if prec != "fp64":
             res[prec]["exchange_speedup_vs_fp64"] = round(tk["fp64"] / tk[prec], 3)
+    if "fp64" in tk:  # the plain FP64 exchange (mjk_exchange, no V work) against the MFMA peak
+        res["fp64"]["exchange_frac_peak"] = round(fk / tk["fp64"] / 1e12 / PEAK_TFLOPS["fp64"], 4)
     if "fp32" in tk and "bf16x3" in tk:
         res["ladder"] = ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, tk["fp64"], res, args.steps)
     return res
