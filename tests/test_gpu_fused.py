@@ -4,10 +4,11 @@ J and V against the oracle's Coulomb build (FP64, relative 1e-11) and against
 the two-pass mjk_coulomb; K bit-identical to mjk_exchange (the fused kernel
 adds independent work to the same MFMA accumulation).  Densities unrelated to
 the orbitals, so V can only come from D.  Cases: tile boundaries (n = 37, 129,
-200), one and two orbital sets, a wide bank (12 column tiles, the 2 x 2 wave
-layout), 128-wide AO tiles, several vector chunks (small X budget), a set
-without orbitals, a reduced set without pairs, and the CHO_FSCF drop-in that
-routes through the fused path."""
+200, 730: a packed triangle of 2 MB, larger than an L2 share), one and two
+orbital sets, a wide bank (12 column tiles, the 2 x 2 wave layout), 128-wide
+AO tiles, several vector chunks (small X budget), a set without orbitals, a
+reduced set without pairs, and the CHO_FSCF drop-in that routes through the
+fused path."""
 import numpy as np
 import pytest
 import torch
@@ -63,7 +64,7 @@ def check(jk, L, dlt, Cs, n):
 
 
 @pytest.mark.parametrize("n,M,occs", [(37, 21, (5,)), (129, 17, (1,)), (200, 33, (40, 9)),
-                                      (70, 9, (190,)), (256, 40, (64,))])
+                                      (70, 9, (190,)), (256, 40, (64,)), (730, 7, (20,))])
 def test_fused_matches_oracle(gpu, n, M, occs):
     L = chol_vectors_np(n, M, seed=n + M)
     Cs = [orbitals_np(n, o, seed=o + 3) for o in occs]
@@ -84,7 +85,7 @@ def test_fused_wide_ao_tiles_and_chunks(gpu):
     jk.set_tuning(0, 0, 128)          # 128-wide AO tiles of the half-transform
     check(jk, L, dlt, Cs, n)
     jk.set_tuning(0, 0, 0)
-    jk.set_x_budget(8 * 64 * 192 * 7)  # chunks of 7 vectors: 8 half-transform launches
+    jk.set_x_budget(8 * 48 * 256 * 7)  # 7 vectors of X (48 x 256): 8 half-transform launches
     check(jk, L, dlt, Cs, n)
     jk.close()
 
