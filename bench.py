@@ -410,7 +410,8 @@ def main():
     # dominant kernel: the larger of the two exchange kernels by time
     ptag = {"fp64": "F64", "fp32": "F32", "bf16x3": "BF16"}[args.precision]
     if ht_ms * ht_calls >= sy_ms * sy_calls:
-        dom = (f"k_ht<{ptag}> (half-transform X = C^T L)", ht_tf, f"k_ht<{ptag},")
+        what = "half-transform X = C^T L" + (" with the Coulomb V = L^T D; MFMA flops only" if fused else "")
+        dom = (f"k_ht<{ptag}> ({what})", ht_tf, f"k_ht<{ptag},")
     else:
         dom = (f"k_syrk<{ptag}> (K = sum X^T X)", sy_tf, f"k_syrk<{ptag},")
     peaks = {name: round(ctx.measure_peak(kind), 2) for name, kind in
