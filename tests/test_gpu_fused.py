@@ -7,8 +7,8 @@ the orbitals, so V can only come from D.  Cases: tile boundaries (n = 37, 129,
 200, 730: a packed triangle of 2 MB, larger than an L2 share), one and two
 orbital sets, a wide bank (12 column tiles, the 2 x 2 wave layout), 128-wide
 AO tiles, several vector chunks (small X budget), a set without orbitals, a
-reduced set without pairs, and the CHO_FSCF drop-in that routes through the
-fused path."""
+reduced set without pairs, the CHO_FSCF drop-in that routes through the fused
+path, and the argument errors."""
 import numpy as np
 import pytest
 import torch
@@ -139,4 +139,22 @@ def test_cho_fscf_dropin_fused(gpu):
     got = flt0.copy()
     jk.cho_fscf(2, got, [0, 0], [11, 8], [Ca, Cb], dlt, 1.0)
     assert rel(got, want) < RTOL
+    jk.close()
+
+
+def test_fused_argument_errors(gpu):
+    """Error paths of mjk_coulomb_exchange: more than 16 orbital sets, a density
+    of the wrong length; the context stays usable afterwards."""
+    from openmolcas_amd import MjkError
+    n, M = 30, 5
+    L = chol_vectors_np(n, M, seed=16)
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(L)
+    C = torch.from_numpy(orbitals_np(n, 3, seed=17))
+    dlt = random_density(n, seed=18)
+    with pytest.raises(MjkError):
+        jk.coulomb_exchange(torch.from_numpy(dlt), [C] * 17)
+    with pytest.raises(ValueError):
+        jk.coulomb_exchange(torch.from_numpy(dlt[:-1]), [C])
+    check(jk, L, dlt, [C.numpy()], n)
     jk.close()
