@@ -35,6 +35,7 @@ enum {
   MJK_ERR_ARG = 1,        /* invalid argument / shape */
   MJK_ERR_HIP = 2,        /* HIP runtime failure */
   MJK_ERR_STATE = 3,      /* call out of order (e.g. no reduced set) */
+  MJK_ERR_IO = 4,         /* vector file could not be opened or read */
   MJK_ERR_NOMEM = 33,     /* device memory exhausted */
   MJK_ERR_SHORT_READ = 77 /* vectors requested that were never uploaded */
 };
@@ -113,6 +114,18 @@ int mjk_set_x_budget(mjk_context *ctx, int64_t bytes);
  * swap_full2rs/swap_rs2full consume (src/fock_util/swap_full2rs.F90:37-46). */
 int mjk_set_reduced_set(mjk_context *ctx, int32_t jred, int64_t nrs,
                         const int64_t *rs_to_lt);
+
+/* Uploads vectors jvec..jvec+jnum-1 (0-based) of reduced set jred straight
+ * from the reference's vector file (CHVEC<iSym>, word addressed: vector
+ * jvec+i starts at byte 8*iadr[i] and holds nrs doubles, nrs the dimension
+ * declared for jred).  iadr[i] is InfVec(jvec+i+1,3,iSym), the address
+ * CHO_VECRD1 hands to DDAFILE (src/cholesky_util/cho_vecrd1.F90:93-147;
+ * word addressing: src/io_util/ddafile.F90:35-36 with MBl_wa = 8,
+ * src/io_util/fast_io.F90:81).  Files split by MOLCAS_MAXFILESIZE are not
+ * read.  MJK_ERR_IO if the file cannot be opened or read, MJK_ERR_SHORT_READ
+ * if it ends before an addressed vector does. */
+int mjk_upload_vectors_file(mjk_context *ctx, const char *path, int32_t jred, int64_t jvec,
+                            int64_t jnum, const int64_t *iadr);
 
 /* Uploads vectors jvec..jvec+jnum-1 (0-based) stored on reduced set jred as
  * lrs(nrs, jnum), column-major -- the Lrs buffer CHO_VECRD fills

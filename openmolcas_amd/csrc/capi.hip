@@ -1,5 +1,9 @@
 // C ABI: context lifecycle, resident vector store, reference-shaped drop-ins
 // and the device engine entry points.
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -431,6 +435,55 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
   for (int64_t j = jvec; j < jvec + jnum; ++j) {
     if (!c->vset[size_t(j)]) ++c->nloaded;
     c->vset[size_t(j)] = jred + 1;
+  }
+  return MJK_OK;
+}
+
+int mjk_upload_vectors_file(mjk_context *c, const char *path, int32_t jred, int64_t jvec,
+                            int64_t jnum, const int64_t *iadr) {
+  if (!c) return MJK_ERR_ARG;
+  if (!path || (jnum > 0 && !iadr)) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: null pointer");
+  if (jred < 0 || size_t(jred) >= c->rsets.size())
+    return set_error(c, MJK_ERR_STATE, "upload_vectors_file: reduced set not declared");
+  if (jvec < 0 || jnum < 0 || jvec + jnum > c->nchol)
+    return set_error(c, MJK_ERR_ARG, "upload_vectors_file: vector range outside the context capacity");
+  if (jnum == 0) return MJK_OK;
+  for (int64_t i = 0; i < jnum; ++i)
+    if (iadr[i] < 0) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: negative disk address");
+  const int64_t nrs = c->rsets[size_t(jred)].nrs;
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0)
+    return set_error(c, MJK_ERR_IO, std::string("upload_vectors_file: cannot open ") + path + ": " +
+                                        std::strerror(errno));
+  struct Closer {
+    int fd;
+    ~Closer() { close(fd); }
+  } closer{fd};
+  // batches of at most 256 MiB through one host buffer, each uploaded like Lrs
+  const int64_t per = std::max<int64_t>(1, (int64_t(256) << 20) / (8 * std::max<int64_t>(nrs, 1)));
+  std::vector<double> buf(size_t(std::min(per, jnum) * nrs));
+  for (int64_t j0 = 0; j0 < jnum; j0 += per) {
+    const int64_t jn = std::min(per, jnum - j0);
+    for (int64_t i = 0; i < jn; ++i) {
+      char *dst = reinterpret_cast<char *>(buf.data() + i * nrs);
+      size_t left = size_t(nrs) * sizeof(double);
+      off_t at = off_t(iadr[j0 + i]) * off_t(sizeof(double));  // word address, 8-byte words
+      while (left > 0) {
+        const ssize_t got = pread(fd, dst, left, at);
+        if (got < 0) {
+          if (errno == EINTR) continue;
+          return set_error(c, MJK_ERR_IO, std::string("upload_vectors_file: read failed: ") +
+                                              std::strerror(errno));
+        }
+        if (got == 0)
+          return set_error(c, MJK_ERR_SHORT_READ, "upload_vectors_file: file ends inside a vector");
+        dst += got;
+        left -= size_t(got);
+        at += got;
+      }
+    }
+    const int rc = mjk_upload_vectors(c, jred, jvec + j0, jn, buf.data(), 0);
+    if (rc) return rc;
   }
   return MJK_OK;
 }

@@ -16,6 +16,7 @@ Every call goes through the C ABI of ``libmolcas_jk.so``; there is no CPU path.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -130,6 +131,14 @@ class JKContext:
         else:
             a = _host_f64(lrs)
             self._check(self.lib.mjk_upload_vectors(self._h, int(jred), int(jvec), a.shape[0], _ptr(a), 0))
+
+    def upload_vectors_file(self, path: str, jred: int, jvec: int, iadr) -> None:
+        """Vectors jvec.. of reduced set jred read from the reference's vector
+        file CHVEC<iSym>: iadr[i] is the word address (InfVec(J,3,iSym)) of
+        vector jvec+i, each holding the reduced set's nrs doubles."""
+        a = np.ascontiguousarray(np.asarray(iadr, dtype=np.int64))
+        self._check(self.lib.mjk_upload_vectors_file(self._h, os.fsencode(path), int(jred), int(jvec),
+                                                     a.shape[0], _ptr(a)))
 
     def upload_packed(self, L) -> None:
         """All vectors on one reduced set covering every LT pair: L is (nchol, npair)."""

@@ -13,7 +13,7 @@ module molcas_jk_iface
                                          c_char, c_loc
   implicit none
   private
-  public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_cho_fscf, &
+  public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_upload_vectors_file, mjk_cho_fscf, &
             mjk_cho_fmcscf, mjk_cho_lk_scf, mjk_cho_fock_dft_red, cho_fscf_gpu, cho_lk_scf_gpu, &
             mjk_cho_fscf_sym, cho_fscf_sym_gpu, mjk_cho_lk_scf_sym, cho_lk_scf_sym_gpu, &
             cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_lk_casscf, cho_lk_casscf_gpu, jk_gpu_init, jk_gpu_final
@@ -47,6 +47,16 @@ module molcas_jk_iface
       integer(c_int64_t), value :: jvec, jnum
       real(c_double), intent(in) :: lrs(*)
     end function mjk_upload_vectors
+    ! path: file name terminated by c_null_char; iadr(i) = InfVec(jvec+i,3,iSym)
+    integer(c_int) function mjk_upload_vectors_file(ctx, path, jred, jvec, jnum, iadr) &
+        bind(C, name='mjk_upload_vectors_file')
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: jred
+      integer(c_int64_t), value :: jvec, jnum
+      integer(c_int64_t), intent(in) :: iadr(*)
+    end function mjk_upload_vectors_file
     integer(c_int) function mjk_cho_fscf(ctx, nden, flt, nforb, niorb, porb, dlt, exfac, prec) &
         bind(C, name='mjk_cho_fscf')
       import :: c_ptr, c_int, c_int32_t, c_double
