@@ -121,11 +121,14 @@ int mjk_set_reduced_set(mjk_context *ctx, int32_t jred, int64_t nrs,
  * declared for jred).  iadr[i] is InfVec(jvec+i+1,3,iSym), the address
  * CHO_VECRD1 hands to DDAFILE (src/cholesky_util/cho_vecrd1.F90:93-147;
  * word addressing: src/io_util/ddafile.F90:35-36 with MBl_wa = 8,
- * src/io_util/fast_io.F90:81).  Files split by MOLCAS_MAXFILESIZE are not
- * read.  MJK_ERR_IO if the file cannot be opened or read, MJK_ERR_SHORT_READ
- * if it ends before an addressed vector does. */
+ * src/io_util/fast_io.F90:81).  split_bytes = 0 for one file; a unit split
+ * by MOLCAS_MAXFILESIZE passes its piece size min(Max_File_Length,
+ * MaxFileSize*10^6) and the pieces are read from path, path'1'..'9',
+ * path'A'.. (src/io_util/mpdafile.F90:92-140).  MJK_ERR_IO if a file cannot
+ * be opened or read, MJK_ERR_SHORT_READ if it ends before an addressed
+ * vector does. */
 int mjk_upload_vectors_file(mjk_context *ctx, const char *path, int32_t jred, int64_t jvec,
-                            int64_t jnum, const int64_t *iadr);
+                            int64_t jnum, const int64_t *iadr, int64_t split_bytes);
 
 /* Uploads vectors jvec..jvec+jnum-1 (0-based) stored on reduced set jred as
  * lrs(nrs, jnum), column-major -- the Lrs buffer CHO_VECRD fills

@@ -43,7 +43,7 @@ SIGNATURES = {
     "mjk_set_x_budget": (_c.c_int, [_vp, _i64]),
     "mjk_set_reduced_set": (_c.c_int, [_vp, _i32, _i64, _vp]),
     "mjk_upload_vectors": (_c.c_int, [_vp, _i32, _i64, _i64, _vp, _i32]),
-    "mjk_upload_vectors_file": (_c.c_int, [_vp, _c.c_char_p, _i32, _i64, _i64, _vp]),
+    "mjk_upload_vectors_file": (_c.c_int, [_vp, _c.c_char_p, _i32, _i64, _i64, _vp, _i64]),
     "mjk_cho_fscf": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
     "mjk_cho_fscf_sym": (_c.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
     "mjk_cho_lk_scf_sym": (_c.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32,

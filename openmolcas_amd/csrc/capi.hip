@@ -440,25 +440,47 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
 }
 
 int mjk_upload_vectors_file(mjk_context *c, const char *path, int32_t jred, int64_t jvec,
-                            int64_t jnum, const int64_t *iadr) {
+                            int64_t jnum, const int64_t *iadr, int64_t split_bytes) {
   if (!c) return MJK_ERR_ARG;
   if (!path || (jnum > 0 && !iadr)) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: null pointer");
   if (jred < 0 || size_t(jred) >= c->rsets.size())
     return set_error(c, MJK_ERR_STATE, "upload_vectors_file: reduced set not declared");
   if (jvec < 0 || jnum < 0 || jvec + jnum > c->nchol)
     return set_error(c, MJK_ERR_ARG, "upload_vectors_file: vector range outside the context capacity");
+  if (split_bytes < 0) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: negative split size");
   if (jnum == 0) return MJK_OK;
   for (int64_t i = 0; i < jnum; ++i)
     if (iadr[i] < 0) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: negative disk address");
   const int64_t nrs = c->rsets[size_t(jred)].nrs;
-  const int fd = open(path, O_RDONLY);
-  if (fd < 0)
-    return set_error(c, MJK_ERR_IO, std::string("upload_vectors_file: cannot open ") + path + ": " +
-                                        std::strerror(errno));
-  struct Closer {
-    int fd;
-    ~Closer() { close(fd); }
-  } closer{fd};
+  // Piece k of a split unit: byte range [k*split, (k+1)*split) of the logical
+  // file, stored in the base name for k = 0 and in the name with '1'..'9',
+  // 'A'.. appended for k >= 1 (src/io_util/mpdafile.F90:92-140).
+  constexpr int kMaxPieces = 20;  // MaxSplitFile, src/io_util/fast_io.F90:49
+  struct Files {
+    int fd[kMaxPieces];
+    Files() { std::fill(fd, fd + kMaxPieces, -1); }
+    ~Files() {
+      for (int f : fd)
+        if (f >= 0) close(f);
+    }
+  } files;
+  auto open_piece = [&](int k, int *rc) -> int {
+    if (k >= kMaxPieces) {
+      *rc = set_error(c, MJK_ERR_SHORT_READ, "upload_vectors_file: address beyond the last file piece");
+      return -1;
+    }
+    if (files.fd[k] < 0) {
+      std::string name(path);
+      if (k > 0) name += char(k <= 9 ? '0' + k : 'A' + (k - 10));
+      files.fd[k] = open(name.c_str(), O_RDONLY);
+      if (files.fd[k] < 0) {
+        *rc = set_error(c, MJK_ERR_IO, "upload_vectors_file: cannot open " + name + ": " +
+                                           std::strerror(errno));
+        return -1;
+      }
+    }
+    return files.fd[k];
+  };
   // batches of at most 256 MiB through one host buffer, each uploaded like Lrs
   const int64_t per = std::max<int64_t>(1, (int64_t(256) << 20) / (8 * std::max<int64_t>(nrs, 1)));
   std::vector<double> buf(size_t(std::min(per, jnum) * nrs));
@@ -466,10 +488,16 @@ int mjk_upload_vectors_file(mjk_context *c, const char *path, int32_t jred, int6
     const int64_t jn = std::min(per, jnum - j0);
     for (int64_t i = 0; i < jn; ++i) {
       char *dst = reinterpret_cast<char *>(buf.data() + i * nrs);
-      size_t left = size_t(nrs) * sizeof(double);
-      off_t at = off_t(iadr[j0 + i]) * off_t(sizeof(double));  // word address, 8-byte words
+      int64_t left = nrs * int64_t(sizeof(double));
+      int64_t at = iadr[j0 + i] * int64_t(sizeof(double));  // word address, 8-byte words
       while (left > 0) {
-        const ssize_t got = pread(fd, dst, left, at);
+        const int k = split_bytes > 0 ? int(at / split_bytes) : 0;
+        const int64_t in = split_bytes > 0 ? at - k * split_bytes : at;
+        const int64_t room = split_bytes > 0 ? std::min(left, split_bytes - in) : left;
+        int rc = MJK_OK;
+        const int fd = open_piece(k, &rc);
+        if (fd < 0) return rc;
+        const ssize_t got = pread(fd, dst, size_t(room), off_t(in));
         if (got < 0) {
           if (errno == EINTR) continue;
           return set_error(c, MJK_ERR_IO, std::string("upload_vectors_file: read failed: ") +
@@ -478,7 +506,7 @@ int mjk_upload_vectors_file(mjk_context *c, const char *path, int32_t jred, int6
         if (got == 0)
           return set_error(c, MJK_ERR_SHORT_READ, "upload_vectors_file: file ends inside a vector");
         dst += got;
-        left -= size_t(got);
+        left -= got;
         at += got;
       }
     }

@@ -132,13 +132,14 @@ class JKContext:
             a = _host_f64(lrs)
             self._check(self.lib.mjk_upload_vectors(self._h, int(jred), int(jvec), a.shape[0], _ptr(a), 0))
 
-    def upload_vectors_file(self, path: str, jred: int, jvec: int, iadr) -> None:
+    def upload_vectors_file(self, path: str, jred: int, jvec: int, iadr, split_bytes: int = 0) -> None:
         """Vectors jvec.. of reduced set jred read from the reference's vector
         file CHVEC<iSym>: iadr[i] is the word address (InfVec(J,3,iSym)) of
-        vector jvec+i, each holding the reduced set's nrs doubles."""
+        vector jvec+i, each holding the reduced set's nrs doubles.  split_bytes:
+        piece size of a unit split by MOLCAS_MAXFILESIZE (0: one file)."""
         a = np.ascontiguousarray(np.asarray(iadr, dtype=np.int64))
         self._check(self.lib.mjk_upload_vectors_file(self._h, os.fsencode(path), int(jred), int(jvec),
-                                                     a.shape[0], _ptr(a)))
+                                                     a.shape[0], _ptr(a), int(split_bytes)))
 
     def upload_packed(self, L) -> None:
         """All vectors on one reduced set covering every LT pair: L is (nchol, npair)."""

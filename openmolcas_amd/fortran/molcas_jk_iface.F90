@@ -47,14 +47,15 @@ module molcas_jk_iface
       integer(c_int64_t), value :: jvec, jnum
       real(c_double), intent(in) :: lrs(*)
     end function mjk_upload_vectors
-    ! path: file name terminated by c_null_char; iadr(i) = InfVec(jvec+i,3,iSym)
-    integer(c_int) function mjk_upload_vectors_file(ctx, path, jred, jvec, jnum, iadr) &
+    ! path: file name terminated by c_null_char; iadr(i) = InfVec(jvec+i,3,iSym);
+    ! split_bytes = 0, or the piece size of a unit split by MOLCAS_MAXFILESIZE
+    integer(c_int) function mjk_upload_vectors_file(ctx, path, jred, jvec, jnum, iadr, split_bytes) &
         bind(C, name='mjk_upload_vectors_file')
       import :: c_ptr, c_int, c_int32_t, c_int64_t, c_char
       type(c_ptr), value :: ctx
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int32_t), value :: jred
-      integer(c_int64_t), value :: jvec, jnum
+      integer(c_int64_t), value :: jvec, jnum, split_bytes
       integer(c_int64_t), intent(in) :: iadr(*)
     end function mjk_upload_vectors_file
     integer(c_int) function mjk_cho_fscf(ctx, nden, flt, nforb, niorb, porb, dlt, exfac, prec) &

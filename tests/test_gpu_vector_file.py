@@ -7,8 +7,10 @@ nnBstR(iSym,iRed) words (src/cholesky_util/cho_vecrd1.F90:93-147, word
 addressing src/io_util/ddafile.F90:35-36).  Two reduced sets (all pairs, and a
 screened subset), vectors at running addresses with gaps.  The build from the
 file must be bit-identical to the build from the same vectors uploaded from
-memory; a missing file gives MJK_ERR_IO (4) and an address past the end of
-the file MJK_ERR_SHORT_READ (77)."""
+memory, also when the unit is split into pieces of a fixed byte size the way
+MOLCAS_MAXFILESIZE splits it (src/io_util/mpdafile.F90:92-140: pieces named
+path, path'1'.., vectors straddling pieces); a missing file gives MJK_ERR_IO
+(4) and an address past the end of the file MJK_ERR_SHORT_READ (77)."""
 import numpy as np
 import pytest
 import torch
@@ -67,6 +69,47 @@ def test_vectors_from_file_match_memory(gpu, tmp_path):
     fil.upload_vectors_file(path, 0, 0, iadr[:M0])
     fil.upload_vectors_file(path, 1, M0, iadr[M0:])
 
+    J0, K0 = mem.coulomb_exchange(dlt, [C])
+    J1, K1 = fil.coulomb_exchange(dlt, [C])
+    assert torch.equal(J0, J1) and torch.equal(K0, K1)
+    mem.close()
+    fil.close()
+
+
+def split_file(path, piece):
+    """Splits a logical unit into pieces of `piece` bytes: path, path + '1', ..."""
+    data = open(path, "rb").read()
+    names = []
+    for k in range(0, (len(data) + piece - 1) // piece):
+        name = path if k == 0 else path + (chr(ord("0") + k) if k <= 9 else chr(ord("A") + k - 10))
+        names.append(name)
+        with open(name, "wb") as f:
+            f.write(data[k * piece:(k + 1) * piece])
+    return names
+
+
+def test_vectors_from_split_file(gpu, tmp_path):
+    n, M0, M1 = 40, 10, 7
+    npr = npair(n)
+    rs1, L0, L1 = layout(n, M0, M1, seed=8)
+    path = str(tmp_path / "CHVEC1")
+    iadr = write_vector_file(path, [L0, L1])
+    piece = 8000  # a multiple of 8 bytes, as MaxFileSize * 10^6 is
+    names = split_file(path, piece)
+    assert 10 <= len(names) <= 20
+    C = torch.from_numpy(orbitals_np(n, 5, seed=9))
+    A = np.random.default_rng(10).standard_normal((n, n))
+    dlt = torch.from_numpy(fold((A + A.T) / (2.0 * n)))
+    mem = JKContext(n, M0 + M1, device=gpu)
+    mem.set_reduced_set(0, np.arange(npr))
+    mem.set_reduced_set(1, rs1)
+    mem.upload_vectors(0, 0, L0)
+    mem.upload_vectors(1, M0, L1)
+    fil = JKContext(n, M0 + M1, device=gpu)
+    fil.set_reduced_set(0, np.arange(npr))
+    fil.set_reduced_set(1, rs1)
+    fil.upload_vectors_file(path, 0, 0, iadr[:M0], split_bytes=piece)
+    fil.upload_vectors_file(path, 1, M0, iadr[M0:], split_bytes=piece)
     J0, K0 = mem.coulomb_exchange(dlt, [C])
     J1, K1 = fil.coulomb_exchange(dlt, [C])
     assert torch.equal(J0, J1) and torch.equal(K0, K1)
