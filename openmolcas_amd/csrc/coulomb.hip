@@ -500,25 +500,21 @@ int coulomb_exchange(Context *c, const double *dlt, int64_t ldd, double *jlt, in
     if ((rc = coulomb(c, 1, dlt, ldd, jlt, ldj, v_out))) return rc;
     return exchange(c, nk, nocc, corb, klt, ldk, MJK_PREC_FP64);
   }
-  const int64_t M = c->nchol, ldl = c->ldl;
+  const int64_t M = c->nchol;
   const int ntmax = c->n_pad / 64;  // AO tiles of the narrowest half-transform tile
-  if ((rc = ensure(c, c->w->d, sizeof(double) * ldl)) ||
-      (rc = ensure(c, c->w->vpart, sizeof(double) * size_t(ntmax) * M)) ||
+  if ((rc = ensure(c, c->w->vpart, sizeof(double) * size_t(ntmax) * M)) ||
       (rc = ensure(c, c->w->v, sizeof(double) * M)))
     return rc;
-  double *Dp = static_cast<double *>(c->w->d.ptr);
   double *vpart = static_cast<double *>(c->w->vpart.ptr);
   double *V = v_out ? v_out : static_cast<double *>(c->w->v.ptr);
-  phase_begin(c);
-  hipLaunchKernelGGL(k_pack_dens, dim3(unsigned(ceil_div(ldl, 256)), 1u), dim3(256), 0, c->stream,
-                     dlt, ldd, c->np, ldl, Dp);
-  phase_end(c, MJK_PHASE_PACK);
   // the fused pass must not outlive this call on any return path
   struct FuseReset {
     Context *c;
     ~FuseReset() { c->fv = FuseV{}; }
   } fuse_reset{c};
-  c->fv.p = Dp;
+  // the half-transform reads the caller's density at packed indices < nbas(nbas+1)/2
+  // only (masked elements read index 0), so no padded copy is needed
+  c->fv.p = dlt;
   c->fv.vpart = vpart;
   c->fv.m = M;
   c->fv.ntile = 0;

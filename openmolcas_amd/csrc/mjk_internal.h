@@ -54,8 +54,8 @@ struct Screen {
 
 // Coulomb pass V = L^T D fused into the FP64 half-transform (p null = off):
 // workgroup (vector J, AO tile t) adds sum_{a in t} sum_{b >= a} L^J_ba D_ba
-// into vpart[t * m + J], with D the folded density in packed storage (zero
-// padded to ldl), so sum_t vpart[t * m + J] = sum_{b >= a} L^J_ba D_ba.
+// into vpart[t * m + J], with D the folded density in packed storage, so
+// sum_t vpart[t * m + J] = sum_{b >= a} L^J_ba D_ba.
 struct FuseV {
   const double *p = nullptr;
   double *vpart = nullptr;
