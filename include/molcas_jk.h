@@ -45,7 +45,9 @@ enum {
 enum {
   MJK_PREC_FP64 = 0,   /* FP64 MFMA */
   MJK_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulate, FP64 reduction over J */
-  MJK_PREC_BF16X3 = 2  /* split-BF16, 3 products, FP32 accumulate */
+  MJK_PREC_BF16X3 = 2, /* split-BF16, 3 products, FP32 accumulate */
+  MJK_PREC_LADDER = 3  /* FP64 / FP32 / split-BF16 by vector index, split
+                          points of mjk_set_ladder (FP64 until set) */
 };
 
 /* Phases reported by mjk_phase_times(). */
@@ -102,6 +104,14 @@ const char *mjk_last_error(const mjk_context *ctx);
  * its own.  Library-internal design: no reference counterpart. */
 int mjk_set_stream(mjk_context *ctx, void *hip_stream);
 
+/* Split points of the MJK_PREC_LADDER exchange: vectors [0, j_fp32) in
+ * FP64, [j_fp32, j_bf16) in FP32, [j_bf16, nchol) in split-BF16, as
+ * mjk_exchange_mixed.  Until set, MJK_PREC_LADDER computes FP64.  No
+ * counterpart in the reference (its exchange is FP64 throughout,
+ * src/fock_util/cho_fscf.F90:250,276); it lets the reference-shaped drop-ins
+ * run the ladder. */
+int mjk_set_ladder(mjk_context *ctx, int64_t j_fp32, int64_t j_bf16);
+
 /* Bytes of device memory one chunk of half-transformed vectors may use. */
 int mjk_set_x_budget(mjk_context *ctx, int64_t bytes);
 
@@ -116,19 +126,22 @@ int mjk_set_reduced_set(mjk_context *ctx, int32_t jred, int64_t nrs,
                         const int64_t *rs_to_lt);
 
 /* Uploads vectors jvec..jvec+jnum-1 (0-based) of reduced set jred straight
- * from the reference's vector file (CHVEC<iSym>, word addressed: vector
- * jvec+i starts at byte 8*iadr[i] and holds nrs doubles, nrs the dimension
- * declared for jred).  iadr[i] is InfVec(jvec+i+1,3,iSym), the address
- * CHO_VECRD1 hands to DDAFILE (src/cholesky_util/cho_vecrd1.F90:93-147;
- * word addressing: src/io_util/ddafile.F90:35-36 with MBl_wa = 8,
- * src/io_util/fast_io.F90:81).  split_bytes = 0 for one file; a unit split
+ * from the reference's vector file (CHVEC<iSym>: vector jvec+i starts at byte
+ * mbl*iadr[i] and holds nrs doubles, nrs the dimension declared for jred).
+ * iadr[i] is InfVec(jvec+i+1,3,iSym), the address CHO_VECRD1 hands to
+ * DDAFILE (src/cholesky_util/cho_vecrd1.F90:93-147), and mbl the file's bytes
+ * per address unit, MBL(Lu) of src/io_util/ddafile.F90:36: 8 for the word-
+ * addressed file of CHO_ADRVEC = 1 (DANAME_MF_WA), 512 for the block-
+ * addressed file of CHO_ADRVEC = 2 (DANAME_MF; src/cholesky_util/
+ * cho_openvr.F90:46-55, src/io_util/fast_io.F90:81); mbl = 0 means 8, other
+ * values are rejected.  split_bytes = 0 for one file; a unit split
  * by MOLCAS_MAXFILESIZE passes its piece size min(Max_File_Length,
  * MaxFileSize*10^6) and the pieces are read from path, path'1'..'9',
  * path'A'.. (src/io_util/mpdafile.F90:92-140).  MJK_ERR_IO if a file cannot
  * be opened or read, MJK_ERR_SHORT_READ if it ends before an addressed
  * vector does. */
 int mjk_upload_vectors_file(mjk_context *ctx, const char *path, int32_t jred, int64_t jvec,
-                            int64_t jnum, const int64_t *iadr, int64_t split_bytes);
+                            int64_t jnum, const int64_t *iadr, int32_t mbl, int64_t split_bytes);
 
 /* Uploads vectors jvec..jvec+jnum-1 (0-based) stored on reduced set jred as
  * lrs(nrs, jnum), column-major -- the Lrs buffer CHO_VECRD fills
@@ -139,12 +152,20 @@ int mjk_upload_vectors(mjk_context *ctx, int32_t jred, int64_t jvec, int64_t jnu
 
 /* ---- reference-shaped drop-ins (host arrays, synchronous) -------------- */
 
+/* Per-density arrays.  The reference passes nDen separately allocated DSBA
+ * arrays (FLT(1:nDen), Porb(1:nDen), KLT(1:nDen): src/scf/choscf_drv.F90:
+ * 285-291,401-402; FI/FA and DI/DA of src/fock_util/cho_cas_drv.F90:42-43,
+ * 101-102), so every per-density argument of a drop-in is an array of nden
+ * pointers, element d pointing at that density's %A0 storage.  Nothing is
+ * assumed about where the densities' arrays sit relative to each other. */
+
 /* CHO_FSCF (src/fock_util/cho_fscf.F90:14), nSym = 1:
  *   FLT(d) += J[DLT(1)] - ExFac * sum_{J,k} X(k,J,a) X(k,J,b),  d < nden,
- * nden in {1,2}; porb holds the nden orbital matrices back to back, density d
- * with nforb[d]+niorb[d] rows.  Called from src/scf/choscf_drv.F90:197,420. */
-int mjk_cho_fscf(mjk_context *ctx, int32_t nden, double *flt, const int32_t *nforb,
-                 const int32_t *niorb, const double *porb, const double *dlt,
+ * nden in {1,2}; flt[d] -> FLT(d)%A0 (LT, updated), porb[d] -> Porb(d)%A0
+ * with nforb[d]+niorb[d] rows (A2(nOcc,nBas)); dlt -> DLT(1)%A0, the only
+ * density that enters Coulomb.  Called from src/scf/choscf_drv.F90:197,420. */
+int mjk_cho_fscf(mjk_context *ctx, int32_t nden, double *const *flt, const int32_t *nforb,
+                 const int32_t *niorb, const double *const *porb, const double *dlt,
                  double exfac, int32_t precision);
 
 /* CHO_FSCF (src/fock_util/cho_fscf.F90:14) with point-group symmetry,
@@ -162,35 +183,40 @@ int mjk_cho_fscf(mjk_context *ctx, int32_t nden, double *flt, const int32_t *nfo
  * FLT(d)%SB(s) += J - ExFac K on every irrep block; K and J are block
  * diagonal because the densities and orbitals are. */
 int mjk_cho_fscf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, int32_t nden,
-                     double *flt, const int32_t *nforb, const int32_t *niorb,
-                     const double *porb, const double *dlt, double exfac, int32_t precision);
+                     double *const *flt, const int32_t *nforb, const int32_t *niorb,
+                     const double *const *porb, const double *dlt, double exfac,
+                     int32_t precision);
 
 /* CHO_LK_SCF (src/fock_util/cho_lk_scf.F90:14) with point-group symmetry:
- * storage and embedding as mjk_cho_fscf_sym, with porb per density the irrep
- * blocks Porb(d)%SB(s)%A2(nbas[s], nbas[s]) (AO index fastest, the first
+ * storage and embedding as mjk_cho_fscf_sym, with porb[d] the irrep blocks
+ * Porb(d)%SB(s)%A2(nbas[s], nbas[s]) (AO index fastest, the first
  * nforb + niorb columns are the orbitals) back to back and plt the folded
  * total density in irrep LT blocks.  FLT(d)%SB(s) += J[PLT] - FactXI K.  The
  * KLT scratch of the reference is not returned (its content is folded into
  * FLT by the reference itself, cho_lk_scf.F90:963-1028); LK screening as for
  * mjk_cho_lk_scf, with nOcc the total over irreps. */
 int mjk_cho_lk_scf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, int32_t nden,
-                       double *flt, const int32_t *nforb, const int32_t *niorb,
-                       const double *porb, const double *plt, double factxi, int32_t nscreen,
-                       double dmpk, double dfmat, int32_t precision);
+                       double *const *flt, const int32_t *nforb, const int32_t *niorb,
+                       const double *const *porb, const double *plt, double factxi,
+                       int32_t nscreen, double dmpk, double dfmat, int32_t precision);
 
 /* CHO_LK_SCF (src/fock_util/cho_lk_scf.F90:14), nSym = 1:
  *   FLT(d) += J[PLT(1)] - FactXI * sum_{J,k} L(a,J)[k] L(b,J)[k],  d < nden,
- * nden in {1,2}; porb holds nden square matrices A2(nbas,nbas) back to back,
- * AO index fastest, columns 0..nforb[d]+niorb[d]-1 used (the MSQ layout of
- * src/scf/choscf_drv.F90:208).  klt (nden blocks of nbas(nbas+1)/2, or NULL)
- * is accumulated with -FactXI*K in the reference's shell-pair-blocked
- * storage (cho_lk_scf.F90:274-292) for the nshell shells of sizes nbassh
- * (nshell = 0: one shell, i.e. plain LT order).  Every shell pair and orbital
- * is kept (no LK screening), which is the reference's result in the limit
- * dmpk -> 0; nscreen, dmpk >= 0 and dfmat are accepted for the signature.
+ * nden in {1,2}; flt[d] -> FLT(d)%A0; porb[d] -> Porb(d)%A0, the square
+ * A2(nbas,nbas) with the AO index fastest, columns 0..nforb[d]+niorb[d]-1
+ * used (the MSQ layout of src/scf/choscf_drv.F90:208); plt -> PLT(1)%A0.
+ * klt is NULL or an array of nden pointers (KLT(d)%A0, nbas(nbas+1)/2
+ * values each), accumulated with -FactXI*K in the reference's shell-pair-
+ * blocked storage (cho_lk_scf.F90:274-292) for the nshell shells of sizes
+ * nbassh (nshell = 0: one shell, i.e. plain LT order).
+ * Without mjk_set_lk_screening every shell pair and orbital is kept, the
+ * reference's result for dmpk -> 0.  With it, the reference's LK screening
+ * runs (per-orbital shell lists from Ylk = DiaH |C|, the vector and pair
+ * cuts, cho_lk_scf.F90:547-627,659-660,713-781), on the shells of
+ * mjk_set_shells and the diagonal of mjk_set_lk_diag.
  * Called from src/scf/choscf_drv.F90:208,433. */
-int mjk_cho_lk_scf(mjk_context *ctx, int32_t nden, double *flt, double *klt,
-                   const int32_t *nforb, const int32_t *niorb, const double *porb,
+int mjk_cho_lk_scf(mjk_context *ctx, int32_t nden, double *const *flt, double *const *klt,
+                   const int32_t *nforb, const int32_t *niorb, const double *const *porb,
                    const double *plt, double factxi, int32_t nscreen, double dmpk,
                    double dfmat, int32_t nshell, const int32_t *nbassh,
                    int32_t precision);
@@ -205,11 +231,14 @@ int mjk_cho_fock_dft_red(mjk_context *ctx, const double *dlt, double *flt);
  *   FI += J[DI] + FactXI * sum X_I X_I^T      (porb1: nforb+niorb rows)
  *   FA += J[DA] - 1/2 * sum X_A X_A^T         (porb2: nchm rows, if do_active)
  *   W_PWXY(p,w,xy) = sum_J (pw|J)(xy|J)        (porb3: naorb rows)
- * flt = [FI, FA], dlt = [DI, DA] (folded), cmo is CMO(n,n) column-major and
- * p runs over norb MOs after nfro frozen ones; w_pwxy may be NULL when
+ * flt[0] -> FLT(1)%A0 = FI, flt[1] -> FLT(2)%A0 = FA; dlt[0] -> DLT(1)%A0 =
+ * DI, dlt[1] -> DLT(2)%A0 = DA (folded), the second entries read only if
+ * do_active (the separately allocated FI/FA and DI/DA of
+ * src/fock_util/cho_cas_drv.F90:42-43,101-102); cmo is CMO(n,n) column-major
+ * and p runs over norb MOs after nfro frozen ones; w_pwxy may be NULL when
  * naorb == 0.  Called from src/fock_util/cho_cas_drv.F90:277. */
-int mjk_cho_fmcscf(mjk_context *ctx, double *flt, int32_t nforb, int32_t niorb,
-                   int32_t naorb, double factxi, const double *dlt, int32_t do_active,
+int mjk_cho_fmcscf(mjk_context *ctx, double *const *flt, int32_t nforb, int32_t niorb,
+                   int32_t naorb, double factxi, const double *const *dlt, int32_t do_active,
                    const double *porb1, int32_t nchm, const double *porb2,
                    const double *porb3, double *w_pwxy, const double *cmo,
                    int32_t nfro, int32_t norb, double exfac, int32_t precision);
@@ -233,12 +262,13 @@ int mjk_cd_incore(mjk_context *ctx, int32_t n, double *x, double *vec, int32_t m
  * active MOs ash1 (orbital-major A2(naorb,nBas)) and cmo (CMO(n,n),
  * column-major, p over norb MOs after nfro).  msq and ash2 are square
  * A2(nBas,nBas) with the AO index fastest (first nfiorb / nchm columns used).
- * dlt = [DI, DA], flt = [FI, FA].  Every shell pair and orbital is kept (the
- * reference's result for dmpk -> 0).  Called from
+ * dlt[0], dlt[1] -> DI, DA and flt[0], flt[1] -> FI, FA (separate arrays, the
+ * second entries read only if do_active).  Screening as mjk_cho_lk_scf, with
+ * the CASSCF list cut MLk >= sqrt(tau) (cho_lk_casscf.F90:688).  Called from
  * src/fock_util/cho_cas_drv.F90:285. */
-int mjk_cho_lk_casscf(mjk_context *ctx, const double *dlt, double *flt, const double *msq,
-                      double *w_pwxy, double factxi, int32_t nfiorb, int32_t naorb,
-                      int32_t nchm, const double *ash1, const double *ash2,
+int mjk_cho_lk_casscf(mjk_context *ctx, const double *const *dlt, double *const *flt,
+                      const double *msq, double *w_pwxy, double factxi, int32_t nfiorb,
+                      int32_t naorb, int32_t nchm, const double *ash1, const double *ash2,
                       int32_t do_active, int32_t nscreen, double dmpk, double dfmat,
                       const double *cmo, int32_t nfro, int32_t norb, double exfac,
                       int32_t precision);
@@ -284,9 +314,9 @@ int mjk_exchange_mixed(mjk_context *ctx, int32_t nk, const int32_t *nocc,
                        const double *const *corb, double *klt, int64_t ldk,
                        int64_t j_fp32, int64_t j_bf16);
 
-/* Exchange as mjk_exchange with tile screening, the MI355X form of LK
- * screening (src/fock_util/cho_lk_scf.F90:547-627,713-781 skip shell pairs
- * and orbitals below thresholds): a K tile (TM x TM AOs) of a vector slice
+/* Exchange as mjk_exchange with tile screening, a tile-granular screening of
+ * the device engine (the drop-ins screen the reference's way, see
+ * mjk_set_lk_screening): a K tile (TM x TM AOs) of a vector slice
  * is skipped when sum_J ||X^J_A||_F ||X^J_B||_F < tau (an elementwise bound
  * of its contribution), a half-transform tile (one vector, 64 AOs) when
  * ||L^J[:, tile]||_F max_d ||C_d||_F < thr_x.  tau = thr_x = 0 computes
@@ -305,10 +335,41 @@ int mjk_screening_stats(mjk_context *ctx, int64_t *k_items, int64_t *k_skipped,
 /* Turns LK screening of mjk_cho_lk_scf / mjk_cho_lk_casscf on (enable != 0)
  * or off (default).  thrcom is the decomposition threshold ThrCom that
  * Cho_LK_ScreeningThreshold reads from the Cholesky module
- * (src/fock_util/cho_lk_screeningthreshold.F90:24-39); the drop-ins then use
- * tau = LKThr/nOcc*dmpk and thrv = sqrt(LKThr/(nOcc*NumCho))*dmpk
- * (src/fock_util/cho_lk_scf.F90:143-171) as the two tile thresholds. */
+ * (src/fock_util/cho_lk_screeningthreshold.F90:24-39).  The drop-ins then
+ * screen the way the reference does, per density d:
+ *   tau(d) = LKThr/max(1,nOcc_d)*dmpk (/MaxRed with Estimate),
+ *   thrv(d) = sqrt(LKThr/(max(1,nOcc_d)*NumCho))*dmpk
+ * (src/fock_util/cho_lk_scf.F90:143-171, cho_lk_casscf.F90:164-200); per
+ * batch of 25 vectors of one reduced set (Cho_LK_MaxVecPerBatch) and per
+ * orbital: shell lists from Y = sqrt|D| |C| recomputed every nScreen reduced
+ * sets (:547-627), half-transform shell pairs with sqrt|SumAClk SvShp| >=
+ * thrv (:659-702), K blocks with MLk MLk >= tau, sqrt|Faa Faa| >= tau/MaxRed
+ * and the Keep flags (:713-830).  It needs mjk_set_shells (or the call's
+ * own nbassh) and mjk_set_lk_diag; vectors must come reduced set by reduced
+ * set.  Screened builds compute in FP64. */
 int mjk_set_lk_screening(mjk_context *ctx, int32_t enable, double thrcom);
+
+/* Shell structure of the basis, nBasSh(1,1:nShell) of the Cholesky module
+ * (src/cholesky_util/cholesky.F90): shell s holds nbassh[s] consecutive AOs,
+ * shells in order, sum = nbas.  Used by the LK screening. */
+int mjk_set_shells(mjk_context *ctx, int32_t nshell, const int32_t *nbassh);
+
+/* The diagonal the LK screening starts every call from: the integral
+ * diagonal D(ab) that CHO_IODIAG reads (src/fock_util/cho_lk_scf.F90:180),
+ * nbas(nbas+1)/2 values in LT order, zero for pairs outside the first
+ * reduced set (host array, copied).  maxred is MaxRed (0: the number of
+ * declared reduced sets); estimate / update are the Estimate / Update
+ * switches of src/fock_util/fock_util_global.F90:56-59 (defaults .false. /
+ * .true.): with update the diagonal loses sum_J L^2 after every batch, with
+ * estimate it is rebuilt from each batch. */
+int mjk_set_lk_diag(mjk_context *ctx, const double *diag, int32_t maxred, int32_t estimate,
+                    int32_t update);
+
+/* Counters of the last screened LK call: stats[0] shells listed over all
+ * (screening batch, orbital), stats[1] half-transformed (batch, orbital,
+ * shell pair), stats[2] K blocks computed, stats[3] batches.  No counterpart
+ * in the reference (its timers count time, not work). */
+int mjk_lk_stats(mjk_context *ctx, int64_t *stats);
 
 /* w_dev[J] = ||L^J||_F^2 of the full symmetric matrix (device, nchol
  * doubles): the weights a caller uses to place the ladder split points.

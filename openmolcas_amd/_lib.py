@@ -21,6 +21,7 @@ MJK_ERR_SHORT_READ = 77
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16X3 = 2
+PREC_LADDER = 3
 
 PHASES = ("coulomb_v", "coulomb_j", "half_transform", "exchange", "reduce", "assembly", "pack")
 
@@ -43,7 +44,8 @@ SIGNATURES = {
     "mjk_set_x_budget": (_c.c_int, [_vp, _i64]),
     "mjk_set_reduced_set": (_c.c_int, [_vp, _i32, _i64, _vp]),
     "mjk_upload_vectors": (_c.c_int, [_vp, _i32, _i64, _i64, _vp, _i32]),
-    "mjk_upload_vectors_file": (_c.c_int, [_vp, _c.c_char_p, _i32, _i64, _i64, _vp, _i64]),
+    "mjk_upload_vectors_file": (_c.c_int, [_vp, _c.c_char_p, _i32, _i64, _i64, _vp, _i32, _i64]),
+    "mjk_set_ladder": (_c.c_int, [_vp, _i64, _i64]),
     "mjk_cho_fscf": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
     "mjk_cho_fscf_sym": (_c.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
     "mjk_cho_lk_scf_sym": (_c.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32,
@@ -66,6 +68,9 @@ SIGNATURES = {
                                          _c.c_double]),
     "mjk_screening_stats": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "mjk_set_lk_screening": (_c.c_int, [_vp, _i32, _c.c_double]),
+    "mjk_set_shells": (_c.c_int, [_vp, _i32, _vp]),
+    "mjk_set_lk_diag": (_c.c_int, [_vp, _vp, _i32, _i32, _i32]),
+    "mjk_lk_stats": (_c.c_int, [_vp, _vp]),
     "mjk_cd_incore": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _c.c_double, _vp]),
     "mjk_fock_update": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mjk_energy": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmolcas_jk.so")
 
 SOURCES = ["capi.hip", "coulomb.hip", "exchange.hip", "assembly.hip", "puvx.hip", "backward.hip", "peaks.hip", "tune.hip",
-           "deco.hip"]
+           "deco.hip", "lk.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -43,11 +43,13 @@ FORTRAN = os.path.join(HERE, "fortran")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FORTRAN_BIN = os.path.join(FORTRAN, "bin", "fscf_driver")
 FORTRAN_SYM_BIN = os.path.join(FORTRAN, "bin", "fscf_sym_driver")
+FORTRAN_UHF_CAS_BIN = os.path.join(FORTRAN, "bin", "uhf_cas_driver")
 
 
 def build_fortran(verbose: bool = False):
-    """ISO_C_BINDING interface module + the Fortran drivers of the CHO_FSCF
-    drop-in, C1 and with symmetry (amdflang).  Returns the C1 driver path, or
+    """ISO_C_BINDING interface module + the Fortran drivers of the drop-ins:
+    CHO_FSCF C1 and with symmetry, and the two-density UHF / CASSCF calls on
+    separately allocated arrays (amdflang).  Returns the C1 driver path, or
     None without a compiler."""
     if not os.path.exists(FLANG):
         if verbose:
@@ -63,6 +65,8 @@ def build_fortran(verbose: bool = False):
          "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_BIN],
         [FLANG, "-O2", "-J", obj, os.path.join(FORTRAN, "fscf_sym_driver.F90"), iface_o, "-L" + LIBDIR,
          "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_SYM_BIN],
+        [FLANG, "-O2", "-J", obj, os.path.join(FORTRAN, "uhf_cas_driver.F90"), iface_o, "-L" + LIBDIR,
+         "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_UHF_CAS_BIN],
     ]
     for cmd in cmds:
         r = subprocess.run(cmd, capture_output=True, text=True)

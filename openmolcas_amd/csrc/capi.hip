@@ -129,14 +129,15 @@ int resolve_phases(Context *c) {
 
 namespace {
 
-// L[(jvec + j)*ldl + map[r]] = src[r + j*nrs]
+// L[(jvec + j)*ldl + map[r]] = src[r + j*nrs]; grid-stride over vectors (y) and
+// reduced-set positions (x), so a launch of any size fits the grid limits.
 __global__ void k_scatter_rs(const double *__restrict__ src, int64_t nrs, int64_t jnum,
                              const int32_t *__restrict__ map, double *__restrict__ L, int64_t ldl,
                              int64_t jvec) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nrs * jnum) return;
-  const int64_t j = i / nrs, r = i % nrs;
-  L[(jvec + j) * ldl + map[r]] = src[i];
+  for (int64_t j = blockIdx.y; j < jnum; j += gridDim.y)
+    for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrs;
+         r += int64_t(gridDim.x) * blockDim.x)
+      L[(jvec + j) * ldl + map[r]] = src[r + j * nrs];
 }
 
 void free_buf(DevBuf &b) {
@@ -150,25 +151,39 @@ void free_buf(DevBuf &b) {
 
 using namespace mjk;
 
-// LK thresholds of the drop-ins: the basic threshold of
+// LK thresholds of the drop-ins, per density: the basic threshold of
 // Cho_LK_ScreeningThreshold (cho_lk_screeningthreshold.F90:34-39) and
-//   tau  = LKThr / max(1,nOcc) * dmpk               (cho_lk_scf.F90:155-156)
-//   thrv = sqrt(LKThr / (max(1,nOcc) * NumVT)) * dmpk (cho_lk_scf.F90:170-171)
-// with the smallest threshold over the densities.  tau bounds a skipped K
-// tile of one vector slice, thrv a skipped half-transform tile.
+//   tau  = LKThr / max(1,nOcc_d) * dmpk  (/ MaxRed with Estimate)  (cho_lk_scf.F90:155-165)
+//   thrv = sqrt(LKThr / (max(1,nOcc_d) * NumVT)) * dmpk            (cho_lk_scf.F90:170-171)
+// (cho_lk_casscf.F90:181-200 with the inactive / active orbital counts).
 static void lk_thresholds(const mjk_context *c, int nden, const int32_t *nocc, double dmpk,
                           double dfmat, double *tau, double *thrv) {
   double thr = std::min(c->lk_thrcom, 1.0e-6);
   if (dfmat >= 0.0 && dfmat <= 1.0) thr *= dfmat;
   const double lkthr = std::max(thr, 1.0e-15);
-  *tau = *thrv = 0.0;
+  const double maxred = double(c->lk.maxred > 0 ? c->lk.maxred
+                                                : int(std::max<size_t>(c->rsets.size(), 1)));
   for (int d = 0; d < nden; ++d) {
     const double no = std::max(1, nocc[d]);
-    const double t = lkthr / no * dmpk;
-    const double v = std::sqrt(lkthr / (no * double(c->nchol))) * dmpk;
-    *tau = d ? std::min(*tau, t) : t;
-    *thrv = d ? std::min(*thrv, v) : v;
+    tau[d] = (lkthr / no) * dmpk;
+    if (c->lk.estimate) tau[d] = tau[d] / maxred;
+    thrv[d] = std::sqrt(lkthr / (no * double(c->nchol))) * dmpk;
   }
+}
+
+// Shells for the LK screening of a call: the context's (mjk_set_shells), or
+// the call's own nBasSh when the context has none.
+static int lk_ready(mjk_context *c, int32_t nshell, const int32_t *nbassh, const char *who) {
+  if (c->lk.nshell < 1 && nshell > 0 && nbassh) {
+    const int rc = mjk_set_shells(c, nshell, nbassh);
+    if (rc) return rc;
+  }
+  if (c->lk.nshell < 1)
+    return set_error(c, MJK_ERR_STATE, std::string(who) + ": LK screening needs the shells (mjk_set_shells)");
+  if (!c->lk.diag_set)
+    return set_error(c, MJK_ERR_STATE,
+                     std::string(who) + ": LK screening needs the integral diagonal (mjk_set_lk_diag)");
+  return MJK_OK;
 }
 
 // Point-group symmetry on the C1 engine: irrep s holds the global AO indices
@@ -302,6 +317,15 @@ int mjk_set_stream(mjk_context *c, void *s) {
   return MJK_OK;
 }
 
+int mjk_set_ladder(mjk_context *c, int64_t j_fp32, int64_t j_bf16) {
+  if (!c) return MJK_ERR_ARG;
+  if (j_fp32 < 0 || j_fp32 > j_bf16 || j_bf16 > c->nchol)
+    return set_error(c, MJK_ERR_ARG, "set_ladder: need 0 <= j_fp32 <= j_bf16 <= nchol");
+  c->lad_fp32 = j_fp32;
+  c->lad_bf16 = j_bf16;
+  return MJK_OK;
+}
+
 int mjk_set_x_budget(mjk_context *c, int64_t bytes) {
   if (!c || bytes < 1) return MJK_ERR_ARG;
   c->x_budget = bytes;
@@ -365,6 +389,8 @@ int mjk_set_reduced_set(mjk_context *c, int32_t jred, int64_t nrs, const int64_t
   }
   rs.nrs = nrs;
   rs.identity = ident;
+  rs.h_map = ident ? std::vector<int32_t>() : m32;
+  if (size_t(jred) < c->lk.present_version.size()) c->lk.present_version[size_t(jred)] = -1;
   if (nrs > 0) {
     void *p = nullptr;
     if (hipMalloc(&p, sizeof(int32_t) * size_t(nrs)) != hipSuccess) {
@@ -422,8 +448,11 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
         } else {
           src = lrs + j0 * nrs;
         }
-        hipLaunchKernelGGL(k_scatter_rs, dim3(unsigned(ceil_div(nrs * jn, 256))), dim3(256), 0,
-                           c->stream, src, nrs, jn, rs.d_map, c->L, c->ldl, jvec + j0);
+        const unsigned gx = unsigned(std::min<int64_t>(ceil_div(nrs, 256), 1024));
+        const unsigned gy = unsigned(std::min<int64_t>(jn, 65535));
+        hipLaunchKernelGGL(k_scatter_rs, dim3(gx, gy), dim3(256), 0, c->stream, src, nrs, jn,
+                           rs.d_map, c->L, c->ldl, jvec + j0);
+        if ((rc = hip_check(c, hipGetLastError(), "scatter vectors"))) return rc;
         if (!on_device && (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))
           return rc;
       }
@@ -440,7 +469,7 @@ int mjk_upload_vectors(mjk_context *c, int32_t jred, int64_t jvec, int64_t jnum,
 }
 
 int mjk_upload_vectors_file(mjk_context *c, const char *path, int32_t jred, int64_t jvec,
-                            int64_t jnum, const int64_t *iadr, int64_t split_bytes) {
+                            int64_t jnum, const int64_t *iadr, int32_t mbl, int64_t split_bytes) {
   if (!c) return MJK_ERR_ARG;
   if (!path || (jnum > 0 && !iadr)) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: null pointer");
   if (jred < 0 || size_t(jred) >= c->rsets.size())
@@ -448,6 +477,9 @@ int mjk_upload_vectors_file(mjk_context *c, const char *path, int32_t jred, int6
   if (jvec < 0 || jnum < 0 || jvec + jnum > c->nchol)
     return set_error(c, MJK_ERR_ARG, "upload_vectors_file: vector range outside the context capacity");
   if (split_bytes < 0) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: negative split size");
+  if (mbl == 0) mbl = 8;
+  if (mbl != 8 && mbl != 512)
+    return set_error(c, MJK_ERR_ARG, "upload_vectors_file: mbl must be 8 (word) or 512 (block)");
   if (jnum == 0) return MJK_OK;
   for (int64_t i = 0; i < jnum; ++i)
     if (iadr[i] < 0) return set_error(c, MJK_ERR_ARG, "upload_vectors_file: negative disk address");
@@ -489,7 +521,7 @@ int mjk_upload_vectors_file(mjk_context *c, const char *path, int32_t jred, int6
     for (int64_t i = 0; i < jn; ++i) {
       char *dst = reinterpret_cast<char *>(buf.data() + i * nrs);
       int64_t left = nrs * int64_t(sizeof(double));
-      int64_t at = iadr[j0 + i] * int64_t(sizeof(double));  // word address, 8-byte words
+      int64_t at = iadr[j0 + i] * int64_t(mbl);  // disk address in units of mbl bytes
       while (left > 0) {
         const int k = split_bytes > 0 ? int(at / split_bytes) : 0;
         const int64_t in = split_bytes > 0 ? at - k * split_bytes : at;
@@ -594,6 +626,66 @@ int mjk_set_lk_screening(mjk_context *c, int32_t enable, double thrcom) {
   return MJK_OK;
 }
 
+int mjk_set_shells(mjk_context *c, int32_t nshell, const int32_t *nbassh) {
+  if (!c) return MJK_ERR_ARG;
+  if (nshell < 1 || !nbassh) return set_error(c, MJK_ERR_ARG, "set_shells: need nshell >= 1 shells");
+  int64_t sum = 0;
+  for (int s = 0; s < nshell; ++s) {
+    if (nbassh[s] < 0) return set_error(c, MJK_ERR_ARG, "set_shells: negative shell size");
+    sum += nbassh[s];
+  }
+  if (sum != c->n) return set_error(c, MJK_ERR_ARG, "set_shells: shell sizes do not add up to nbas");
+  LKState &K = c->lk;
+  std::vector<int32_t> shell(size_t(c->n)), koff(size_t(nshell) + 1);
+  for (int s = 0, a = 0; s < nshell; ++s) {
+    koff[size_t(s)] = a;
+    for (int i = 0; i < nbassh[s]; ++i) shell[size_t(a++)] = s;
+  }
+  koff[size_t(nshell)] = c->n;
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")) ||
+      (rc = ensure(c, K.shell, sizeof(int32_t) * size_t(c->n))) ||
+      (rc = ensure(c, K.koff, sizeof(int32_t) * (size_t(nshell) + 1))) ||
+      (rc = ensure(c, K.nbsh, sizeof(int32_t) * size_t(nshell))) ||
+      (rc = hip_check(c, hipMemcpy(K.shell.ptr, shell.data(), sizeof(int32_t) * size_t(c->n),
+                                   hipMemcpyHostToDevice), "copy shells")) ||
+      (rc = hip_check(c, hipMemcpy(K.koff.ptr, koff.data(), sizeof(int32_t) * (size_t(nshell) + 1),
+                                   hipMemcpyHostToDevice), "copy shell offsets")) ||
+      (rc = hip_check(c, hipMemcpy(K.nbsh.ptr, nbassh, sizeof(int32_t) * size_t(nshell),
+                                   hipMemcpyHostToDevice), "copy shell sizes")))
+    return rc;
+  K.nshell = nshell;
+  K.nbassh.assign(nbassh, nbassh + nshell);
+  K.version += 1;  // shell-pair occupancy of every reduced set is stale
+  return MJK_OK;
+}
+
+int mjk_set_lk_diag(mjk_context *c, const double *diag, int32_t maxred, int32_t estimate,
+                    int32_t update) {
+  if (!c) return MJK_ERR_ARG;
+  if (!diag) return set_error(c, MJK_ERR_ARG, "set_lk_diag: null diagonal");
+  if (maxred < 0) return set_error(c, MJK_ERR_ARG, "set_lk_diag: maxred < 0");
+  LKState &K = c->lk;
+  int rc;
+  if ((rc = ensure(c, K.diag0, sizeof(double) * size_t(c->np))) ||
+      (rc = hip_check(c, hipMemcpy(K.diag0.ptr, diag, sizeof(double) * size_t(c->np),
+                                   hipMemcpyHostToDevice), "copy diagonal")))
+    return rc;
+  K.diag_set = true;
+  K.maxred = maxred;
+  K.estimate = estimate != 0;
+  K.update = update != 0;
+  return MJK_OK;
+}
+
+int mjk_lk_stats(mjk_context *c, int64_t *stats) {
+  if (!c) return MJK_ERR_ARG;
+  if (!stats) return set_error(c, MJK_ERR_ARG, "lk_stats: null pointer");
+  for (int i = 0; i < 3; ++i) stats[i] = int64_t(c->lk.last_stats[i]);
+  stats[3] = c->lk.last_batches;
+  return MJK_OK;
+}
+
 int mjk_fock_update(mjk_context *c, int32_t nd, double *flt, int64_t ldf, const double *jlt,
                     int64_t ldj, const int32_t *jsrc, const double *fj, const double *klt,
                     int64_t ldk, const double *fk) {
@@ -640,8 +732,8 @@ int mjk_phase_times(mjk_context *c, double *ms, int64_t *calls) {
 
 // ---- reference-shaped drop-ins -------------------------------------------
 
-int mjk_cho_fscf(mjk_context *c, int32_t nden, double *flt, const int32_t *nforb,
-                 const int32_t *niorb, const double *porb, const double *dlt, double exfac,
+int mjk_cho_fscf(mjk_context *c, int32_t nden, double *const *flt, const int32_t *nforb,
+                 const int32_t *niorb, const double *const *porb, const double *dlt, double exfac,
                  int32_t precision) {
   if (!c) return MJK_ERR_ARG;
   if (nden != 1 && nden != 2)
@@ -652,26 +744,32 @@ int mjk_cho_fscf(mjk_context *c, int32_t nden, double *flt, const int32_t *nforb
   int64_t tot = 0;
   for (int d = 0; d < nden; ++d) {
     if (nforb[d] < 0 || niorb[d] < 0) return set_error(c, MJK_ERR_ARG, "cho_fscf: negative nOcc");
+    if (!flt[d]) return set_error(c, MJK_ERR_ARG, "cho_fscf: null FLT(d)");
     nocc[d] = nforb[d] + niorb[d];
+    if (nocc[d] > 0 && (!porb || !porb[d]))
+      return set_error(c, MJK_ERR_ARG, "cho_fscf: null Porb(d)");
     tot += int64_t(nocc[d]) * c->n;
   }
-  if (tot > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_fscf: null porb");
   // device scratch: D | J | K(nden) | F(nden) | orbitals
   const size_t need = sizeof(double) * size_t(np * (2 + 2 * nden) + tot);
   int rc;
   if ((rc = ensure(c, c->w->misc, need))) return rc;
   double *dD = static_cast<double *>(c->w->misc.ptr);
   double *dJ = dD + np, *dK = dJ + np, *dF = dK + nden * np, *dC = dF + nden * np;
-  if ((rc = hip_check(c, hipMemcpyAsync(dD, dlt, sizeof(double) * np, hipMemcpyHostToDevice,
-                                        c->stream), "copy DLT")) ||
-      (rc = hip_check(c, hipMemcpyAsync(dF, flt, sizeof(double) * np * nden,
-                                        hipMemcpyHostToDevice, c->stream), "copy FLT")))
-    return rc;
-  if (tot > 0 && (rc = hip_check(c, hipMemcpyAsync(dC, porb, sizeof(double) * tot,
-                                                   hipMemcpyHostToDevice, c->stream),
-                                 "copy Porb")))
-    return rc;
   const double *cptr[2] = {dC, dC + int64_t(nocc[0]) * c->n};
+  if ((rc = hip_check(c, hipMemcpyAsync(dD, dlt, sizeof(double) * np, hipMemcpyHostToDevice,
+                                        c->stream), "copy DLT")))
+    return rc;
+  for (int d = 0; d < nden; ++d) {  // FLT(d) and Porb(d) are separate arrays
+    if ((rc = hip_check(c, hipMemcpyAsync(dF + d * np, flt[d], sizeof(double) * np,
+                                          hipMemcpyHostToDevice, c->stream), "copy FLT")))
+      return rc;
+    if (nocc[d] > 0 &&
+        (rc = hip_check(c, hipMemcpyAsync(const_cast<double *>(cptr[d]), porb[d],
+                                          sizeof(double) * size_t(nocc[d]) * c->n,
+                                          hipMemcpyHostToDevice, c->stream), "copy Porb")))
+      return rc;
+  }
   if (precision == MJK_PREC_FP64) {  // V pass inside the half-transform
     if ((rc = coulomb_exchange(c, dD, np, dJ, np, nden, nocc, cptr, dK, np, nullptr))) return rc;
   } else {
@@ -682,15 +780,17 @@ int mjk_cho_fscf(mjk_context *c, int32_t nden, double *flt, const int32_t *nforb
   const double fj[2] = {1.0, 1.0};                 // FactCI, cho_fscf.F90:64
   const double fk[2] = {-exfac, -exfac};           // FactXI, cho_fscf.F90:65
   if ((rc = fock_update(c, nden, dF, np, dJ, np, jsrc, fj, dK, np, fk))) return rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(flt, dF, sizeof(double) * np * nden,
-                                        hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
-    return rc;
+  for (int d = 0; d < nden; ++d)
+    if ((rc = hip_check(c, hipMemcpyAsync(flt[d], dF + d * np, sizeof(double) * np,
+                                          hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
+      return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "cho_fscf sync");
 }
 
-int mjk_cho_fscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t nden, double *flt,
-                     const int32_t *nforb, const int32_t *niorb, const double *porb,
-                     const double *dlt, double exfac, int32_t precision) {
+int mjk_cho_fscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t nden,
+                     double *const *flt, const int32_t *nforb, const int32_t *niorb,
+                     const double *const *porb, const double *dlt, double exfac,
+                     int32_t precision) {
   if (!c) return MJK_ERR_ARG;
   if (nden != 1 && nden != 2)
     return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: invalid parameter nDen");  // cho_fscf.F90:75-78
@@ -701,19 +801,21 @@ int mjk_cho_fscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t 
   if ((rc = sym_map(c, nsym, nbas, &m, "cho_fscf_sym")) ||
       (rc = sym_counts(c, m, nden, nforb, niorb, nocc, "cho_fscf_sym")))
     return rc;
-  const int64_t n = c->n, np = c->np, ntri = m.ntri();
-  if (nocc[0] + nocc[1] > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: null porb");
+  const int64_t n = c->n, np = c->np;
+  for (int d = 0; d < nden; ++d)
+    if (!flt[d] || (nocc[d] > 0 && (!porb || !porb[d])))
+      return set_error(c, MJK_ERR_ARG, "cho_fscf_sym: null FLT(d) or Porb(d)");
   // DLT and FLT blocks into the full LT; the orbitals of irrep s into columns of
   // orbital-major A2(nocc_d, n) matrices that vanish outside the irrep
   std::vector<double> dfull(static_cast<size_t>(np), 0.0), ffull(static_cast<size_t>(np * nden), 0.0);
   std::vector<double> cfull(static_cast<size_t>((int64_t(nocc[0]) + nocc[1]) * n), 0.0);
   m.each([&](int64_t t, int64_t g) {
     dfull[size_t(g)] = dlt[t];
-    for (int d = 0; d < nden; ++d) ffull[size_t(d * np + g)] = flt[d * ntri + t];
+    for (int d = 0; d < nden; ++d) ffull[size_t(d * np + g)] = flt[d][t];
   });
-  const double *pd = porb;
   double *cd = cfull.data();
   for (int d = 0; d < nden; ++d) {
+    const double *pd = porb ? porb[d] : nullptr;
     int64_t k0 = 0;  // first orbital of irrep s within density d
     for (int s = 0; s < nsym; ++s) {
       const int64_t no = nforb[s + 8 * d] + niorb[s + 8 * d], nb = nbas[s];
@@ -725,19 +827,19 @@ int mjk_cho_fscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t 
     cd += int64_t(nocc[d]) * n;
   }
   const int32_t nf0[2] = {0, 0};
-  if ((rc = mjk_cho_fscf(c, nden, ffull.data(), nf0, nocc, cfull.data(), dfull.data(), exfac,
-                         precision)))
-    return rc;
+  double *fp[2] = {ffull.data(), ffull.data() + np};
+  const double *cp[2] = {cfull.data(), cfull.data() + int64_t(nocc[0]) * n};
+  if ((rc = mjk_cho_fscf(c, nden, fp, nf0, nocc, cp, dfull.data(), exfac, precision))) return rc;
   m.each([&](int64_t t, int64_t g) {
-    for (int d = 0; d < nden; ++d) flt[d * ntri + t] = ffull[size_t(d * np + g)];
+    for (int d = 0; d < nden; ++d) flt[d][t] = ffull[size_t(d * np + g)];
   });
   return MJK_OK;
 }
 
 int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t nden,
-                       double *flt, const int32_t *nforb, const int32_t *niorb, const double *porb,
-                       const double *plt, double factxi, int32_t nscreen, double dmpk, double dfmat,
-                       int32_t precision) {
+                       double *const *flt, const int32_t *nforb, const int32_t *niorb,
+                       const double *const *porb, const double *plt, double factxi,
+                       int32_t nscreen, double dmpk, double dfmat, int32_t precision) {
   if (!c) return MJK_ERR_ARG;
   if (nden != 1 && nden != 2)
     return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: invalid parameter nDen");  // cho_lk_scf.F90:93-96
@@ -748,18 +850,20 @@ int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_
   if ((rc = sym_map(c, nsym, nbas, &m, "cho_lk_scf_sym")) ||
       (rc = sym_counts(c, m, nden, nforb, niorb, nocc, "cho_lk_scf_sym")))
     return rc;
-  const int64_t n = c->n, np = c->np, ntri = m.ntri();
-  if (nocc[0] + nocc[1] > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: null porb");
+  const int64_t n = c->n, np = c->np;
+  for (int d = 0; d < nden; ++d)
+    if (!flt[d] || (nocc[d] > 0 && (!porb || !porb[d])))
+      return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: null FLT(d) or Porb(d)");
   // Porb(d)%SB(s)%A2(nBas(s),nBas(s)), AO index fastest, its first nForb+nIorb
   // columns the orbitals: they become columns k0(s).. of an n x n matrix
   std::vector<double> pfull(static_cast<size_t>(np), 0.0), ffull(static_cast<size_t>(np * nden), 0.0);
   std::vector<double> cfull(static_cast<size_t>(nden * n * n), 0.0);
   m.each([&](int64_t t, int64_t g) {
     pfull[size_t(g)] = plt[t];
-    for (int d = 0; d < nden; ++d) ffull[size_t(d * np + g)] = flt[d * ntri + t];
+    for (int d = 0; d < nden; ++d) ffull[size_t(d * np + g)] = flt[d][t];
   });
-  const double *pd = porb;
   for (int d = 0; d < nden; ++d) {
+    const double *pd = porb ? porb[d] : nullptr;
     double *cd = cfull.data() + d * n * n;
     int64_t k0 = 0;
     for (int s = 0; s < nsym; ++s) {
@@ -771,28 +875,31 @@ int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_
     }
   }
   const int32_t nf0[2] = {0, 0};
-  if ((rc = mjk_cho_lk_scf(c, nden, ffull.data(), nullptr, nf0, nocc, cfull.data(), pfull.data(),
-                           factxi, nscreen, dmpk, dfmat, 0, nullptr, precision)))
+  double *fp[2] = {ffull.data(), ffull.data() + np};
+  const double *cp[2] = {cfull.data(), cfull.data() + n * n};
+  if ((rc = mjk_cho_lk_scf(c, nden, fp, nullptr, nf0, nocc, cp, pfull.data(), factxi, nscreen, dmpk,
+                           dfmat, 0, nullptr, precision)))
     return rc;
   m.each([&](int64_t t, int64_t g) {
-    for (int d = 0; d < nden; ++d) flt[d * ntri + t] = ffull[size_t(d * np + g)];
+    for (int d = 0; d < nden; ++d) flt[d][t] = ffull[size_t(d * np + g)];
   });
   return MJK_OK;
 }
 
-int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const int32_t *nforb,
-                   const int32_t *niorb, const double *porb, const double *plt, double factxi,
+int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *const *flt, double *const *klt,
+                   const int32_t *nforb, const int32_t *niorb, const double *const *porb,
+                   const double *plt, double factxi,
                    int32_t nscreen, double dmpk, double dfmat, int32_t nshell,
                    const int32_t *nbassh, int32_t precision) {
-  // Without mjk_set_lk_screening every tile is computed (the reference's result
-  // for dmpk -> 0); with it, tiles are skipped below the LK thresholds.
-  (void)nscreen;
+  // Without mjk_set_lk_screening every shell pair and orbital is computed (the
+  // reference's result for dmpk -> 0); with it, the reference's LK screening.
   if (!c) return MJK_ERR_ARG;
   if (nden != 1 && nden != 2)
     return set_error(c, MJK_ERR_ARG, "cho_lk_scf: invalid parameter nDen");  // cho_lk_scf.F90:93-96
   if (!flt || !nforb || !niorb || !plt) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: null pointer");
   if (dmpk < 0.0) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: dmpk < 0");
   const int64_t np = c->np, n = c->n;
+  int rc0;
   int32_t nocc[2] = {0, 0};
   int64_t tot = 0;
   for (int d = 0; d < nden; ++d) {
@@ -800,8 +907,9 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
       return set_error(c, MJK_ERR_ARG, "cho_lk_scf: bad orbital count");
     nocc[d] = nforb[d] + niorb[d];
     tot += int64_t(nocc[d]) * n;
+    if (!flt[d] || (nocc[d] > 0 && (!porb || !porb[d])) || (klt && !klt[d]))
+      return set_error(c, MJK_ERR_ARG, "cho_lk_scf: null FLT(d), KLT(d) or Porb(d)");
   }
-  if (tot > 0 && !porb) return set_error(c, MJK_ERR_ARG, "cho_lk_scf: null porb");
   // KLT storage: shell-pair blocks (cho_lk_scf.F90:274-292); no shells = one shell
   std::vector<int64_t> kpos;
   if (klt) {
@@ -831,12 +939,13 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
       first_a += sh[sa];
     }
   }
+  if (c->lk_on && (rc0 = lk_ready(c, nshell, nbassh, "cho_lk_scf"))) return rc0;
   // orbitals to the orbital-major layout the exchange engine reads
   std::vector<double> corb(static_cast<size_t>(tot));
   for (int64_t d = 0, o0 = 0; d < nden; o0 += int64_t(nocc[d]) * n, ++d)
     for (int64_t k = 0; k < nocc[d]; ++k)
       for (int64_t b = 0; b < n; ++b)
-        corb[size_t(o0 + k + nocc[d] * b)] = porb[size_t(d) * n * n + b + n * k];
+        corb[size_t(o0 + k + nocc[d] * b)] = porb[d][b + n * k];
   // device scratch: D | J | K(nden) | F(nden) | orbitals
   const size_t need = sizeof(double) * size_t(np * (2 + 2 * nden) + tot);
   int rc;
@@ -844,30 +953,40 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
   double *dD = static_cast<double *>(c->w->misc.ptr);
   double *dJ = dD + np, *dK = dJ + np, *dF = dK + nden * np, *dC = dF + nden * np;
   if ((rc = hip_check(c, hipMemcpyAsync(dD, plt, sizeof(double) * np, hipMemcpyHostToDevice,
-                                        c->stream), "copy PLT")) ||
-      (rc = hip_check(c, hipMemcpyAsync(dF, flt, sizeof(double) * np * nden,
-                                        hipMemcpyHostToDevice, c->stream), "copy FLT")))
+                                        c->stream), "copy PLT")))
     return rc;
+  for (int d = 0; d < nden; ++d)
+    if ((rc = hip_check(c, hipMemcpyAsync(dF + d * np, flt[d], sizeof(double) * np,
+                                          hipMemcpyHostToDevice, c->stream), "copy FLT")))
+      return rc;
   if (tot > 0 && (rc = hip_check(c, hipMemcpyAsync(dC, corb.data(), sizeof(double) * tot,
                                                    hipMemcpyHostToDevice, c->stream),
                                  "copy Porb")))
     return rc;
   const double *cptr[2] = {dC, dC + int64_t(nocc[0]) * n};
-  double tau = 0.0, thrv = 0.0;
-  if (c->lk_on) lk_thresholds(c, nden, nocc, dmpk, dfmat, &tau, &thrv);
-  if (precision == MJK_PREC_FP64 && tau <= 0.0 && thrv <= 0.0) {  // V pass inside the half-transform
+  if (c->lk_on) {  // the reference's LK screening, FP64
+    if (precision != MJK_PREC_FP64)
+      return set_error(c, MJK_ERR_ARG, "cho_lk_scf: LK screening computes in FP64 only");
+    double tau[2], thrv[2];
+    lk_thresholds(c, nden, nocc, dmpk, dfmat, tau, thrv);
+    c->lk_nscreen = nscreen;
+    if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr)) ||
+        (rc = lk_exchange(c, nden, nocc, cptr, tau, thrv, false, dK, np)))
+      return rc;
+  } else if (precision == MJK_PREC_FP64) {  // V pass inside the half-transform
     if ((rc = coulomb_exchange(c, dD, np, dJ, np, nden, nocc, cptr, dK, np, nullptr))) return rc;
   } else {
     if ((rc = coulomb(c, 1, dD, np, dJ, np, nullptr))) return rc;
-    if ((rc = exchange_screened(c, nden, nocc, cptr, dK, np, precision, tau, thrv))) return rc;
+    if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) return rc;
   }
   const int32_t jsrc[2] = {0, 0};
   const double fj[2] = {1.0, 1.0};               // FactCI, cho_lk_scf.F90:102
   const double fk[2] = {-factxi, -factxi};       // -FactXI, cho_lk_scf.F90:803,830
   if ((rc = fock_update(c, nden, dF, np, dJ, np, jsrc, fj, dK, np, fk))) return rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(flt, dF, sizeof(double) * np * nden,
-                                        hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
-    return rc;
+  for (int d = 0; d < nden; ++d)
+    if ((rc = hip_check(c, hipMemcpyAsync(flt[d], dF + d * np, sizeof(double) * np,
+                                          hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
+      return rc;
   std::vector<double> kh;
   if (klt) {
     kh.resize(static_cast<size_t>(np * nden));
@@ -879,7 +998,7 @@ int mjk_cho_lk_scf(mjk_context *c, int32_t nden, double *flt, double *klt, const
   if (klt)
     for (int d = 0; d < nden; ++d)
       for (int64_t p = 0; p < np; ++p)
-        klt[d * np + kpos[size_t(p)]] += -factxi * kh[size_t(d * np + p)];
+        klt[d][kpos[size_t(p)]] += -factxi * kh[size_t(d * np + p)];
   return MJK_OK;
 }
 
@@ -909,15 +1028,19 @@ int mjk_cho_fock_dft_red(mjk_context *c, const double *dlt, double *flt) {
 // FI += J[DI] + fk_i * K[porb1], FA += J[DA] + fk_a * K[porb2] (if do_active),
 // W_PWXY from porb3 and cmo: the shared body of the CASSCF drop-ins.
 // porb1/porb2/porb3 are host arrays in the orbital-major layout A2(nk,nBas).
-static int fmcscf_core(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,
-                       double fk_i, double fk_a, const double *dlt, int32_t do_active,
+static int fmcscf_core(mjk_context *c, double *const *flt, int32_t nforb, int32_t niorb,
+                       int32_t naorb, double fk_i, double fk_a, const double *const *dlt,
+                       int32_t do_active,
                        const double *porb1, int32_t nchm, const double *porb2,
                        const double *porb3, double *w_pwxy, const double *cmo, int32_t nfro,
-                       int32_t norb, int32_t precision, double tau = 0.0, double thrv = 0.0) {
+                       int32_t norb, int32_t precision, const double *tau = nullptr,
+                       const double *thrv = nullptr) {
   if (!flt || !dlt) return set_error(c, MJK_ERR_ARG, "cho_fmcscf: null pointer");
   if (nforb < 0 || niorb < 0 || naorb < 0 || nchm < 0)
     return set_error(c, MJK_ERR_ARG, "cho_fmcscf: negative orbital count");
   const int nden = do_active ? 2 : 1;
+  for (int d = 0; d < nden; ++d)
+    if (!flt[d] || !dlt[d]) return set_error(c, MJK_ERR_ARG, "cho_fmcscf: null FLT(d) or DLT(d)");
   const int64_t np = c->np, n = c->n;
   const int32_t nk1 = nforb + niorb;
   const int32_t nk2 = do_active ? nchm : 0;
@@ -940,15 +1063,22 @@ static int fmcscf_core(mjk_context *c, double *flt, int32_t nforb, int32_t niorb
     return hip_check(c, hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice,
                                        c->stream), what);
   };
-  if ((rc = h2d(dD, dlt, size_t(np * nden), "copy DLT")) ||
-      (rc = h2d(dF, flt, size_t(np * nden), "copy FLT")))
-    return rc;
+  for (int d = 0; d < nden; ++d)  // FI/FA and DI/DA are separate arrays
+    if ((rc = h2d(dD + d * np, dlt[d], size_t(np), "copy DLT")) ||
+        (rc = h2d(dF + d * np, flt[d], size_t(np), "copy FLT")))
+      return rc;
   if (nk1 > 0 && (rc = h2d(dC1, porb1, size_t(nk1 * n), "copy POrb(1)"))) return rc;
   if (nk2 > 0 && (rc = h2d(dC2, porb2, size_t(nk2 * n), "copy POrb(2)"))) return rc;
   if ((rc = coulomb(c, nden, dD, np, dJ, np, nullptr))) return rc;
   const int32_t nocc[2] = {nk1, nk2};
   const double *cptr[2] = {dC1, dC2};
-  if ((rc = exchange_screened(c, nden, nocc, cptr, dK, np, precision, tau, thrv))) return rc;
+  if (tau) {  // the reference's LK screening (CASSCF list cut), FP64
+    if (precision != MJK_PREC_FP64)
+      return set_error(c, MJK_ERR_ARG, "cho_lk_casscf: LK screening computes in FP64 only");
+    if ((rc = lk_exchange(c, nden, nocc, cptr, tau, thrv, true, dK, np))) return rc;
+  } else if ((rc = exchange(c, nden, nocc, cptr, dK, np, precision))) {
+    return rc;
+  }
   const int32_t jsrc[2] = {0, 1};
   const double fj[2] = {1.0, 1.0};        // FactCI, FactCA (cho_fmcscf.F90:71)
   const double fk[2] = {fk_i, fk_a};      // FactXI, FactXA
@@ -962,16 +1092,17 @@ static int fmcscf_core(mjk_context *c, double *flt, int32_t nforb, int32_t niorb
                                           hipMemcpyDeviceToHost, c->stream), "copy W_PWXY")))
       return rc;
   }
-  if ((rc = hip_check(c, hipMemcpyAsync(flt, dF, sizeof(double) * np * nden,
-                                        hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
-    return rc;
+  for (int d = 0; d < nden; ++d)
+    if ((rc = hip_check(c, hipMemcpyAsync(flt[d], dF + d * np, sizeof(double) * np,
+                                          hipMemcpyDeviceToHost, c->stream), "copy FLT back")))
+      return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "cho_fmcscf sync");
 }
 
 extern "C" {
 
-int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, int32_t naorb,
-                   double factxi, const double *dlt, int32_t do_active, const double *porb1,
+int mjk_cho_fmcscf(mjk_context *c, double *const *flt, int32_t nforb, int32_t niorb, int32_t naorb,
+                   double factxi, const double *const *dlt, int32_t do_active, const double *porb1,
                    int32_t nchm, const double *porb2, const double *porb3, double *w_pwxy,
                    const double *cmo, int32_t nfro, int32_t norb, double exfac,
                    int32_t precision) {
@@ -981,16 +1112,19 @@ int mjk_cho_fmcscf(mjk_context *c, double *flt, int32_t nforb, int32_t niorb, in
                      porb3, w_pwxy, cmo, nfro, norb, precision);  // FactXA = -1/2, cho_fmcscf.F90:74
 }
 
-int mjk_cho_lk_casscf(mjk_context *c, const double *dlt, double *flt, const double *msq,
+int mjk_cho_lk_casscf(mjk_context *c, const double *const *dlt, double *const *flt, const double *msq,
                       double *w_pwxy, double factxi, int32_t nfiorb, int32_t naorb, int32_t nchm,
                       const double *ash1, const double *ash2, int32_t do_active, int32_t nscreen,
                       double dmpk, double dfmat, const double *cmo, int32_t nfro, int32_t norb,
                       double exfac, int32_t precision) {
-  // Without mjk_set_lk_screening every tile is computed (the reference's result
-  // for dmpk -> 0); with it, tiles are skipped below the LK thresholds.
-  (void)nscreen;
+  // Without mjk_set_lk_screening every shell pair and orbital is computed (the
+  // reference's result for dmpk -> 0); with it, the reference's LK screening.
   if (!c) return MJK_ERR_ARG;
   if (dmpk < 0.0) return set_error(c, MJK_ERR_ARG, "cho_lk_casscf: dmpk < 0");
+  if (c->lk_on) {
+    const int rc = lk_ready(c, 0, nullptr, "cho_lk_casscf");
+    if (rc) return rc;
+  }
   const int64_t n = c->n;
   if (nfiorb < 0 || nfiorb > n || nchm < 0 || nchm > n)
     return set_error(c, MJK_ERR_ARG, "cho_lk_casscf: bad orbital count");
@@ -1004,15 +1138,16 @@ int mjk_cho_lk_casscf(mjk_context *c, const double *dlt, double *flt, const doub
     for (int64_t b = 0; b < n; ++b) p1[size_t(k + nfiorb * b)] = msq[b + n * k];
   for (int64_t k = 0; k < nk2; ++k)
     for (int64_t b = 0; b < n; ++b) p2[size_t(k + nk2 * b)] = ash2[b + n * k];
-  double tau = 0.0, thrv = 0.0;
+  double tau[2] = {0.0, 0.0}, thrv[2] = {0.0, 0.0};
   if (c->lk_on) {
     const int32_t no[2] = {nfiorb, nk2};
-    lk_thresholds(c, do_active ? 2 : 1, no, dmpk, dfmat, &tau, &thrv);
+    lk_thresholds(c, do_active ? 2 : 1, no, dmpk, dfmat, tau, thrv);
+    c->lk_nscreen = nscreen;
   }
   // FactX = [FactXI*ExFac, -1/2*ExFac], cho_lk_casscf.F90:113
   return fmcscf_core(c, flt, 0, nfiorb, naorb, factxi * exfac, -0.5 * exfac, dlt, do_active,
-                     p1.data(), nchm, p2.data(), ash1, w_pwxy, cmo, nfro, norb, precision, tau,
-                     thrv);
+                     p1.data(), nchm, p2.data(), ash1, w_pwxy, cmo, nfro, norb, precision,
+                     c->lk_on ? tau : nullptr, c->lk_on ? thrv : nullptr);
 }
 
 }  // extern "C"

@@ -1340,9 +1340,12 @@ void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, dou
 
 int exchange(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
              int64_t ldk, int32_t precision) {
+  const int64_t M = c->nchol;
+  if (precision == MJK_PREC_LADDER)  // split points of mjk_set_ladder; FP64 until set
+    return exchange_mixed(c, nk, nocc, corb, klt, ldk, c->lad_fp32 < 0 ? M : c->lad_fp32,
+                          c->lad_bf16 < 0 ? M : c->lad_bf16);
   if (precision < MJK_PREC_FP64 || precision > MJK_PREC_BF16X3)
     return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
-  const int64_t M = c->nchol;
   return exchange_mixed(c, nk, nocc, corb, klt, ldk, precision == MJK_PREC_FP64 ? M : 0,
                         precision == MJK_PREC_BF16X3 ? 0 : M);
 }
@@ -1353,10 +1356,15 @@ int exchange_screened(Context *c, int32_t nk, const int32_t *nocc, const double 
   if (ldk < c->np) return set_error(c, MJK_ERR_ARG, "exchange: ldk < nbas(nbas+1)/2");
   if (c->nloaded != c->nchol)
     return set_error(c, MJK_ERR_SHORT_READ, "exchange: not all Cholesky vectors were uploaded");
-  if (precision < MJK_PREC_FP64 || precision > MJK_PREC_BF16X3)
-    return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
   if (!(tau >= 0.0) || !(thr_x >= 0.0))
     return set_error(c, MJK_ERR_ARG, "exchange: screening thresholds must be >= 0");
+  if (precision == MJK_PREC_LADDER) {
+    if (tau > 0.0 || thr_x > 0.0)
+      return set_error(c, MJK_ERR_ARG, "exchange: the precision ladder cannot be screened");
+    return exchange(c, nk, nocc, corb, klt, ldk, precision);
+  }
+  if (precision < MJK_PREC_FP64 || precision > MJK_PREC_BF16X3)
+    return set_error(c, MJK_ERR_ARG, "exchange: precision mode not available");
   std::vector<Segment> segs = {{0, c->nchol, precision}};
   return exchange_segments(c, nk, nocc, corb, klt, ldk, segs, tau, thr_x);
 }

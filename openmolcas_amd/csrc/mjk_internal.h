@@ -67,6 +67,35 @@ struct ReducedSet {
   int64_t nrs = 0;
   bool identity = false;      // rs position r maps to LT index r for all r
   int32_t *d_map = nullptr;   // device copy of rs -> LT map (0-based)
+  std::vector<int32_t> h_map; // host copy (shell-pair occupancy of the LK screening)
+};
+
+// State of the reference-form LK screening that the reference keeps in module
+// globals (use Cholesky: nShell, nBasSh, the diagonal CHO_IODIAG reads, MaxRed;
+// Fock_util_global: Estimate, Update) and the device work buffers of one call.
+struct LKState {
+  int32_t nshell = 0;
+  std::vector<int32_t> nbassh;            // host
+  int64_t version = 0;                    // bumped when the shells change
+  DevBuf shell, koff, nbsh;               // device: shell of each AO, first AO, size
+  bool diag_set = false;
+  DevBuf diag0;                           // [np] the diagonal at the start of a call
+  int32_t maxred = 0, estimate = 0, update = 1;
+  std::vector<DevBuf> present;            // per reduced set: [nshp] shell pair occupied
+  std::vector<int64_t> present_version;
+  DevBuf present_all;                     // every shell pair (vectors written in place)
+  int64_t present_all_version = -1;
+  // per call
+  DevBuf diag, dh, ylk, svshp, sumac, mlk, listed, keep, faa, x, kacc, stats;
+  long long last_stats[3] = {0, 0, 0};    // listed shells, half-transformed shell pairs, K blocks
+  int64_t last_batches = 0;
+  ~LKState() {
+    for (DevBuf *b : {&shell, &koff, &nbsh, &diag0, &present_all, &diag, &dh, &ylk, &svshp, &sumac,
+                      &mlk, &listed, &keep, &faa, &x, &kacc, &stats})
+      if (b->ptr) (void)hipFree(b->ptr);
+    for (DevBuf &b : present)
+      if (b.ptr) (void)hipFree(b.ptr);
+  }
 };
 
 struct Context {
@@ -89,6 +118,9 @@ struct Context {
 
   int64_t x_budget = int64_t(16) << 30;  // bytes for one chunk of X
 
+  // precision ladder of MJK_PREC_LADDER (mjk_set_ladder); -1 = unset (FP64)
+  int64_t lad_fp32 = -1, lad_bf16 = -1;
+
   // workspaces
   DevBuf ws_l32;          // FP32 shadow of L (mixed-precision exchange)
   bool l32_valid = false;
@@ -98,6 +130,8 @@ struct Context {
   // LK screening of the LK drop-ins (mjk_set_lk_screening) and its counters
   bool lk_on = false;
   double lk_thrcom = 1.0e-6;
+  int32_t lk_nscreen = 10;        // nScreen of the LK call in progress
+  LKState lk;
   DevBuf scount;                  // device: [K items skipped, X tiles skipped]
   Screen scr;                     // screening of the exchange launch in progress
   FuseV fv;                       // Coulomb V fused into the exchange in progress
@@ -179,6 +213,13 @@ int vector_norms(Context *c, double *w);
 // [j_bf16, nchol) in split-BF16; all partial sums reduced in FP64.
 int exchange_mixed(Context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
                    double *klt, int64_t ldk, int64_t j_fp32, int64_t j_bf16);
+
+// Reference-form LK exchange (CHO_LK_SCF / CHO_LK_CASSCF screening) of nd <= 2
+// densities: klt(d) (plain LT, row stride ldk) = the screened sum_{J,k} X X^T,
+// unscaled.  corb[d] is orbital-major (nocc[d] x n) on the device; tau/thrv per
+// density; cas_cut selects the CASSCF shell-list cut MLk >= sqrt(tau).
+int lk_exchange(Context *c, int32_t nd, const int32_t *nocc, const double *const *corb,
+                const double *tau, const double *thrv, bool cas_cut, double *klt, int64_t ldk);
 
 // Fock assembly: flt(d) += fj[d]*jlt(jsrc[d]) + fk[d]*klt(d).
 int fock_update(Context *c, int32_t nd, double *flt, int64_t ldf, const double *jlt,

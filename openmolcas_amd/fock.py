@@ -24,7 +24,7 @@ import torch
 from . import _lib
 from ._lib import MjkError, PREC_FP64
 
-_PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+_PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "ladder": 3}
 
 
 def precision_code(p) -> int:
@@ -59,6 +59,27 @@ def _ld(t: torch.Tensor) -> int:
 
 def _host_f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _rows_inout(a, n: int, npair: int, what: str):
+    """Per-density host arrays updated in place: a (n, npair) C-contiguous
+    float64 array (its rows) or a list of n separately allocated 1-D float64
+    arrays.  Returns the list of writable arrays."""
+    rows = list(a) if isinstance(a, (list, tuple)) else [a[d] for d in range(n)]
+    if len(rows) < n:
+        raise ValueError(f"{what}: expected {n} per-density arrays")
+    for r in rows[:n]:
+        if not isinstance(r, np.ndarray) or r.dtype != np.float64 or not r.flags.c_contiguous \
+                or r.size < npair:
+            raise TypeError(f"{what} must be C-contiguous float64 arrays of >= {npair} values "
+                            "(updated in place)")
+    return rows[:n]
+
+
+def _ptr_array(arrs):
+    """C array of pointers to the given host arrays (None -> NULL)."""
+    return (ctypes.c_void_p * max(1, len(arrs)))(
+        *[a.ctypes.data if a is not None else None for a in arrs])
 
 
 class JKContext:
@@ -109,6 +130,10 @@ class JKContext:
         except Exception:
             pass
 
+    def set_ladder(self, j_fp32: int, j_bf16: int) -> None:
+        """Split points of the "ladder" precision (FP64 / FP32 / split-BF16)."""
+        self._check(self.lib.mjk_set_ladder(self._h, int(j_fp32), int(j_bf16)))
+
     def set_x_budget(self, nbytes: int):
         self._check(self.lib.mjk_set_x_budget(self._h, int(nbytes)))
 
@@ -132,14 +157,16 @@ class JKContext:
             a = _host_f64(lrs)
             self._check(self.lib.mjk_upload_vectors(self._h, int(jred), int(jvec), a.shape[0], _ptr(a), 0))
 
-    def upload_vectors_file(self, path: str, jred: int, jvec: int, iadr, split_bytes: int = 0) -> None:
+    def upload_vectors_file(self, path: str, jred: int, jvec: int, iadr, split_bytes: int = 0,
+                            mbl: int = 8) -> None:
         """Vectors jvec.. of reduced set jred read from the reference's vector
         file CHVEC<iSym>: iadr[i] is the word address (InfVec(J,3,iSym)) of
         vector jvec+i, each holding the reduced set's nrs doubles.  split_bytes:
-        piece size of a unit split by MOLCAS_MAXFILESIZE (0: one file)."""
+        piece size of a unit split by MOLCAS_MAXFILESIZE (0: one file); mbl:
+        bytes per disk address unit (8 word-addressed, 512 block-addressed)."""
         a = np.ascontiguousarray(np.asarray(iadr, dtype=np.int64))
         self._check(self.lib.mjk_upload_vectors_file(self._h, os.fsencode(path), int(jred), int(jvec),
-                                                     a.shape[0], _ptr(a), int(split_bytes)))
+                                                     a.shape[0], _ptr(a), int(mbl), int(split_bytes)))
 
     def upload_packed(self, L) -> None:
         """All vectors on one reduced set covering every LT pair: L is (nchol, npair)."""
@@ -149,25 +176,22 @@ class JKContext:
         self.upload_vectors(0, 0, L)
 
     # ------------------------------------------------------------ drop-ins (host)
-    def cho_fscf(self, nden: int, flt: np.ndarray, nforb, niorb, porb, dlt: np.ndarray,
+    def cho_fscf(self, nden: int, flt, nforb, niorb, porb, dlt: np.ndarray,
                  exfac: float, precision="fp64") -> int:
         """CHO_FSCF(rc,nDen,FLT,nForb,nIorb,Porb,DLT,ExFac), nSym = 1.
 
-        flt: (nden, npair) float64, updated in place.  porb: list of nden arrays
-        (nbas, nforb[d]+niorb[d]) = Porb(d)%SB(1)%A2(nOcc,nBas) memory.  dlt:
-        folded LT density (only DLT(1) enters Coulomb, as in the reference)."""
-        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
-            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        flt: (nden, npair) float64 array or a list of nden separate arrays,
+        updated in place (FLT(d)%A0).  porb: list of nden arrays (nbas,
+        nforb[d]+niorb[d]) = Porb(d)%SB(1)%A2(nOcc,nBas) memory.  dlt: folded
+        LT density (only DLT(1) enters Coulomb, as in the reference)."""
+        nd = max(1, min(int(nden), 2))
+        fl = _rows_inout(flt, nd, self.npair, "flt")
         nf = np.ascontiguousarray(nforb, dtype=np.int32)
         ni = np.ascontiguousarray(niorb, dtype=np.int32)
         d = _host_f64(dlt).reshape(-1)[: self.npair].copy()
-        if nden in (1, 2):
-            parts = [_host_f64(porb[k]).reshape(-1) for k in range(nden)]
-            pc = np.concatenate(parts) if parts else np.zeros(1)
-        else:
-            pc = np.zeros(1)
-        rc = self.lib.mjk_cho_fscf(self._h, int(nden), _ptr(flt), _ptr(nf), _ptr(ni), _ptr(pc),
-                                   _ptr(d), float(exfac), precision_code(precision))
+        pc = [_host_f64(porb[k]) for k in range(nd)] if nden in (1, 2) else [None] * nd
+        rc = self.lib.mjk_cho_fscf(self._h, int(nden), _ptr_array(fl), _ptr(nf), _ptr(ni),
+                                   _ptr_array(pc), _ptr(d), float(exfac), precision_code(precision))
         self._check(rc)
         return rc
 
@@ -181,19 +205,21 @@ class JKContext:
         niorb: (nden, nsym) counts.  porb[d][s]: (nbas[s], nocc(s)) =
         Porb(d)%SB(s)%A2(nOcc,nBas) memory.  The vectors (every JSYM) must be
         in the context already, with reduced-set maps into the C1 triangle."""
-        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
-            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
         nsym = len(nbas)
+        ntri = sum(int(b) * (int(b) + 1) // 2 for b in nbas)
+        fl = _rows_inout(flt, nden, ntri, "flt")
         nb = np.ascontiguousarray(nbas, dtype=np.int32)
         nf = np.zeros((max(nden, 1), 8), dtype=np.int32)
         ni = np.zeros((max(nden, 1), 8), dtype=np.int32)
         nf[:nden, :nsym] = np.asarray(nforb, dtype=np.int32).reshape(nden, nsym)
         ni[:nden, :nsym] = np.asarray(niorb, dtype=np.int32).reshape(nden, nsym)
-        parts = [_host_f64(porb[d][k]).reshape(-1) for d in range(nden) for k in range(nsym)]
-        pc = np.concatenate(parts) if parts else np.zeros(1)
+        pc = []  # Porb(d)%A0: the irrep blocks of density d back to back
+        for dd in range(nden):
+            parts = [_host_f64(porb[dd][k]).reshape(-1) for k in range(nsym)]
+            pc.append(np.concatenate(parts) if parts else np.zeros(1))
         d = _host_f64(dlt).reshape(-1)
-        rc = self.lib.mjk_cho_fscf_sym(self._h, int(nsym), _ptr(nb), int(nden), _ptr(flt), _ptr(nf),
-                                       _ptr(ni), _ptr(pc), _ptr(d), float(exfac),
+        rc = self.lib.mjk_cho_fscf_sym(self._h, int(nsym), _ptr(nb), int(nden), _ptr_array(fl),
+                                       _ptr(nf), _ptr(ni), _ptr_array(pc), _ptr(d), float(exfac),
                                        precision_code(precision))
         self._check(rc)
         return rc
@@ -206,26 +232,28 @@ class JKContext:
         porb[d][s]: (nbas[s], >= nocc(s)) AO x orbital matrices (the columns of
         Porb(d)%SB(s)%A2(nBas,nBas)); plt: folded total density in irrep LT
         blocks; flt updated in place.  No KLT output."""
-        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
-            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
         nsym = len(nbas)
+        ntri = sum(int(b) * (int(b) + 1) // 2 for b in nbas)
+        fl = _rows_inout(flt, nden, ntri, "flt")
         nb = np.ascontiguousarray(nbas, dtype=np.int32)
         nf = np.zeros((max(nden, 1), 8), dtype=np.int32)
         ni = np.zeros((max(nden, 1), 8), dtype=np.int32)
         nf[:nden, :nsym] = np.asarray(nforb, dtype=np.int32).reshape(nden, nsym)
         ni[:nden, :nsym] = np.asarray(niorb, dtype=np.int32).reshape(nden, nsym)
-        parts = []
+        pc = []  # Porb(d)%A0: square irrep blocks, AO index fastest
         for d in range(nden):
+            parts = []
             for k in range(nsym):
                 c = _host_f64(porb[d][k])
                 sq = np.zeros((nbas[k], nbas[k]))
                 sq[:, :c.shape[1]] = c
-                parts.append(sq.T.reshape(-1))  # AO index fastest
-        pc = np.concatenate(parts) if parts else np.zeros(1)
+                parts.append(sq.T.reshape(-1))
+            pc.append(np.concatenate(parts) if parts else np.zeros(1))
         p = _host_f64(plt).reshape(-1)
-        rc = self.lib.mjk_cho_lk_scf_sym(self._h, int(nsym), _ptr(nb), int(nden), _ptr(flt), _ptr(nf),
-                                         _ptr(ni), _ptr(pc), _ptr(p), float(factxi), int(nscreen),
-                                         float(dmpk), float(dfmat), precision_code(precision))
+        rc = self.lib.mjk_cho_lk_scf_sym(self._h, int(nsym), _ptr(nb), int(nden), _ptr_array(fl),
+                                         _ptr(nf), _ptr(ni), _ptr_array(pc), _ptr(p), float(factxi),
+                                         int(nscreen), float(dmpk), float(dfmat),
+                                         precision_code(precision))
         self._check(rc)
         return rc
 
@@ -239,34 +267,37 @@ class JKContext:
         self._check(rc)
         return rc
 
-    def cho_lk_scf(self, nden: int, flt: np.ndarray, klt, nforb, niorb, porb, plt: np.ndarray,
+    def cho_lk_scf(self, nden: int, flt, klt, nforb, niorb, porb, plt: np.ndarray,
                    factxi: float, nscreen: int = 10, dmpk: float = 0.1, dfmat: float = 0.0,
                    nbassh=None, precision="fp64") -> int:
         """CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,
-        dFmat), nSym = 1, without LK screening (the dmpk -> 0 result).
+        dFmat), nSym = 1 (screened after set_lk_screening, else the dmpk -> 0
+        result).
 
-        flt: (nden, npair) updated in place.  klt: None or (nden, npair),
-        accumulated in the shell-pair-blocked storage of the shells nbassh.
-        porb: list of nden (nbas, >= nforb[d]+niorb[d]) AO x orbital matrices
-        C[b, k] (the MSQ columns).  plt: folded total density."""
-        for a in (flt,) + ((klt,) if klt is not None else ()):
-            if a.dtype != np.float64 or not a.flags.c_contiguous:
-                raise TypeError("flt/klt must be C-contiguous float64 arrays (updated in place)")
+        flt: (nden, npair) array or list of nden arrays, updated in place.
+        klt: None, or the same form, accumulated in the shell-pair-blocked
+        storage of the shells nbassh.  porb: list of nden (nbas, >=
+        nforb[d]+niorb[d]) AO x orbital matrices C[b, k] (the MSQ columns).
+        plt: folded total density."""
+        nd = max(1, min(int(nden), 2))
+        fl = _rows_inout(flt, nd, self.npair, "flt")
+        kl = _rows_inout(klt, nd, self.npair, "klt") if klt is not None else None
         nf = np.ascontiguousarray(nforb, dtype=np.int32)
         ni = np.ascontiguousarray(niorb, dtype=np.int32)
         n = self.nbas
-        pc = np.zeros(max(1, nden * n * n)) if nden in (1, 2) else np.zeros(1)
+        pc = [None] * nd
         if nden in (1, 2):
             for d in range(nden):
                 c = _host_f64(porb[d])
                 sq = np.zeros((n, n))
                 sq[:, :c.shape[1]] = c
-                pc[d * n * n:(d + 1) * n * n] = sq.T.reshape(-1)  # AO index fastest
+                pc[d] = np.ascontiguousarray(sq.T)  # A2(nBas,nBas), AO index fastest
         p = _host_f64(plt).reshape(-1)[: self.npair].copy()
         sh = np.ascontiguousarray(nbassh if nbassh is not None else [], dtype=np.int32)
-        rc = self.lib.mjk_cho_lk_scf(self._h, int(nden), _ptr(flt), _ptr(klt), _ptr(nf), _ptr(ni),
-                                     _ptr(pc), _ptr(p), float(factxi), int(nscreen), float(dmpk),
-                                     float(dfmat), len(sh), _ptr(sh) if len(sh) else None,
+        rc = self.lib.mjk_cho_lk_scf(self._h, int(nden), _ptr_array(fl),
+                                     _ptr_array(kl) if kl is not None else None, _ptr(nf), _ptr(ni),
+                                     _ptr_array(pc), _ptr(p), float(factxi), int(nscreen),
+                                     float(dmpk), float(dfmat), len(sh), _ptr(sh) if len(sh) else None,
                                      precision_code(precision))
         self._check(rc)
         return rc
@@ -278,9 +309,9 @@ class JKContext:
         """CHO_LK_CASSCF(DLT,FLT,MSQ,W_PWXY,FactXI,nFIorb,nAorb,nChM,Ash,DoActive,
         nScreen,dmpk,dFmat,CMO,ExFac), nSym = 1, without screening.  flt (2, npair)
         updated in place; msq (nbas, >= nfiorb) and ash2 (nbas, >= nchm) are AO x
-        vector matrices; ash1 (nbas, naorb) the active MOs; cmo (nbas, nbas)."""
-        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
-            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        vector matrices; ash1 (nbas, naorb) the active MOs; cmo (nbas, nbas).
+        flt: (2, npair) or a list [FI, FA]; dlt: (2, npair) or [DI, DA]."""
+        fl = _rows_inout(flt, 2 if do_active else 1, self.npair, "flt")
         n = self.nbas
 
         def square(a, k):  # A2(nBas,nBas), AO index fastest
@@ -293,8 +324,9 @@ class JKContext:
         a2 = square(ash2, nchm)
         a1 = _host_f64(ash1) if ash1 is not None and naorb > 0 else np.zeros(1)
         cm = _host_f64(np.asarray(cmo).T) if cmo is not None else np.zeros(1)
-        d = _host_f64(dlt)
-        rc = self.lib.mjk_cho_lk_casscf(self._h, _ptr(d), _ptr(flt), _ptr(m), _ptr(w_pwxy),
+        dl = [_host_f64(x).reshape(-1) for x in (list(dlt) if isinstance(dlt, (list, tuple))
+                                                 else np.asarray(dlt).reshape(-1, self.npair))]
+        rc = self.lib.mjk_cho_lk_casscf(self._h, _ptr_array(dl), _ptr_array(fl), _ptr(m), _ptr(w_pwxy),
                                         float(factxi), int(nfiorb), int(naorb), int(nchm),
                                         _ptr(a1), _ptr(a2), int(bool(do_active)), int(nscreen),
                                         float(dmpk), float(dfmat), _ptr(cm), int(nfro), int(norb),
@@ -309,18 +341,19 @@ class JKContext:
         """CHO_FMCSCF(rc,FLT,nForb,nIorb,nAorb,FactXI,DLT,DoActive,POrb,nChM,
         W_PWXY,CMO,ExFac), nSym = 1.  flt (2, npair) updated in place; w_pwxy
         (norb*naorb*naorb(naorb+1)/2,) filled; cmo (nbas, nbas) with cmo[a, p] =
-        CMO(a,p)."""
-        if flt.dtype != np.float64 or not flt.flags.c_contiguous:
-            raise TypeError("flt must be a C-contiguous float64 array (updated in place)")
+        CMO(a,p).  flt: (2, npair) or a list [FI, FA] of separate arrays;
+        dlt: (2, npair) or [DI, DA]."""
+        fl = _rows_inout(flt, 2 if do_active else 1, self.npair, "flt")
         p1 = _host_f64(porb1) if porb1 is not None else np.zeros(1)
         p2 = _host_f64(porb2) if porb2 is not None else np.zeros(1)
         p3 = _host_f64(porb3) if porb3 is not None else np.zeros(1)
         cm = _host_f64(np.asarray(cmo).T) if cmo is not None else np.zeros(1)
-        d = _host_f64(dlt)
+        dl = [_host_f64(x).reshape(-1) for x in (list(dlt) if isinstance(dlt, (list, tuple))
+                                                 else np.asarray(dlt).reshape(-1, self.npair))]
         if w_pwxy is not None and (w_pwxy.dtype != np.float64 or not w_pwxy.flags.c_contiguous):
             raise TypeError("w_pwxy must be a C-contiguous float64 array")
-        rc = self.lib.mjk_cho_fmcscf(self._h, _ptr(flt), int(nforb), int(niorb), int(naorb),
-                                     float(factxi), _ptr(d), int(bool(do_active)), _ptr(p1),
+        rc = self.lib.mjk_cho_fmcscf(self._h, _ptr_array(fl), int(nforb), int(niorb), int(naorb),
+                                     float(factxi), _ptr_array(dl), int(bool(do_active)), _ptr(p1),
                                      int(nchm), _ptr(p2), _ptr(p3), _ptr(w_pwxy), _ptr(cm),
                                      int(nfro), int(norb), float(exfac), precision_code(precision))
         self._check(rc)
@@ -415,8 +448,30 @@ class JKContext:
                 "x_skipped": v[3].value}
 
     def set_lk_screening(self, enable: bool, thrcom: float = 1e-6) -> None:
-        """LK screening of the LK drop-ins; thrcom is the decomposition threshold."""
+        """LK screening of the LK drop-ins (the reference's); thrcom is the
+        decomposition threshold ThrCom."""
         self._check(self.lib.mjk_set_lk_screening(self._h, int(bool(enable)), float(thrcom)))
+
+    def set_shells(self, nbassh) -> None:
+        """Shell sizes nBasSh (consecutive AOs per shell, in order)."""
+        sh = np.ascontiguousarray(nbassh, dtype=np.int32)
+        self._check(self.lib.mjk_set_shells(self._h, len(sh), _ptr(sh)))
+
+    def set_lk_diag(self, diag, maxred: int = 0, estimate: bool = False, update: bool = True) -> None:
+        """The integral diagonal (LT order) each LK call starts from, MaxRed and
+        the Estimate / Update switches."""
+        d = _host_f64(diag).reshape(-1)
+        if d.size != self.npair:
+            raise ValueError("diag must hold nbas(nbas+1)/2 values")
+        self._check(self.lib.mjk_set_lk_diag(self._h, _ptr(d), int(maxred), int(bool(estimate)),
+                                             int(bool(update))))
+
+    def lk_stats(self) -> np.ndarray:
+        """[listed shells, half-transformed shell pairs, K blocks, batches] of
+        the last screened LK call."""
+        st = np.zeros(4, dtype=np.int64)
+        self._check(self.lib.mjk_lk_stats(self._h, _ptr(st)))
+        return st
 
     def vector_norms(self) -> torch.Tensor:
         """(nchol,) squared Frobenius norms of the symmetric vectors L^J."""

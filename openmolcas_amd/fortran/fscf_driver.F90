@@ -14,7 +14,7 @@ program fscf_driver
   real(8) :: hdr(4), ExFac
   real(8), allocatable :: Lrs(:), Porb(:), DLT(:), FLT(:)
   integer(c_int64_t), allocatable :: map(:)
-  integer :: nForb(1), nIorb(1)
+  integer :: nForb(8, 1), nIorb(8, 1)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -38,9 +38,11 @@ program fscf_driver
   if (rc /= 0) stop 12
   rc = int(mjk_upload_vectors(jk_ctx, 0_c_int32_t, 0_c_int64_t, int(NumCho, c_int64_t), Lrs, 0_c_int32_t))
   if (rc /= 0) stop 13
-  nForb(1) = 0
-  nIorb(1) = nOcc
-  call cho_fscf_gpu(rc, 1, FLT, nForb, nIorb, Porb, DLT, ExFac)
+  nForb = 0
+  nIorb = 0
+  nIorb(1, 1) = nOcc
+  ! nDen = 1: the second-density arguments are not referenced
+  call cho_fscf_gpu(rc, 1, FLT, FLT, nForb, nIorb, Porb, Porb, DLT, ExFac)
   if (rc /= 0) stop 14
   call jk_gpu_final(rc)
 

@@ -59,7 +59,7 @@ program fscf_sym_driver
   nForb = 0
   nIorb = 0
   nIorb(1:2, 1) = nOcc
-  call cho_fscf_sym_gpu(rc, 2, nBas, 1, FLT, nForb, nIorb, Porb, DLT, ExFac)
+  call cho_fscf_sym_gpu(rc, 2, nBas, 1, FLT, FLT, nForb, nIorb, Porb, Porb, DLT, ExFac)
   if (rc /= 0) stop 16
   call jk_gpu_final(rc)
 

@@ -3,10 +3,17 @@
 ! Fortran drop-ins with the argument lists of OpenMolcas's
 !   CHO_FSCF   (src/fock_util/cho_fscf.F90:14)
 !   CHO_FMCSCF (src/fock_util/cho_fmcscf.F90:14)
-! for nSym = 1.  The reference passes DSBA_Type arrays; a caller passes
-! their %A0 storage (LT packed per irrep, orbital matrices A2(nOcc,nBas)).
-! rc keeps the reference's meaning (0 ok, 33 memory, 77 short read); the
-! GPU context plays the role of the vector buffer of Cho_X_Init.
+! for nSym = 1.  The reference passes arrays of DSBA_Type, one separately
+! allocated object per density (FLT(1:nDen), Porb(1:nDen), KLT(1:nDen),
+! src/scf/choscf_drv.F90:285-291,401-402; FI/FA and DI/DA,
+! src/fock_util/cho_cas_drv.F90:42-43,101-102).  The shims take the %A0
+! storage of each density as a dummy argument of its own (FLT1, FLT2, ...)
+! and hand the library one pointer per density; nothing is assumed about
+! where the arrays of two densities sit.  For nDen = 1 the second-density
+! dummies are not referenced (pass the first array again).  rc keeps the
+! reference's meaning (0 ok, 33 memory, 77 short read); the GPU context
+! plays the role of the vector buffer of Cho_X_Init.  Every shim takes an
+! optional precision (MJK_PREC_*; default FP64).
 !-----------------------------------------------------------------------
 module molcas_jk_iface
   use, intrinsic :: iso_c_binding, only: c_ptr, c_int, c_int32_t, c_int64_t, c_double, c_null_ptr, &
@@ -16,9 +23,11 @@ module molcas_jk_iface
   public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_upload_vectors_file, mjk_cho_fscf, &
             mjk_cho_fmcscf, mjk_cho_lk_scf, mjk_cho_fock_dft_red, cho_fscf_gpu, cho_lk_scf_gpu, &
             mjk_cho_fscf_sym, cho_fscf_sym_gpu, mjk_cho_lk_scf_sym, cho_lk_scf_sym_gpu, &
-            cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_lk_casscf, cho_lk_casscf_gpu, jk_gpu_init, jk_gpu_final
+            cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_lk_casscf, cho_lk_casscf_gpu, jk_gpu_init, jk_gpu_final, &
+            mjk_set_ladder
 
-  integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1
+  integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1, MJK_PREC_BF16X3 = 2, &
+                                           MJK_PREC_LADDER = 3
   type(c_ptr), save, public :: jk_ctx = c_null_ptr
 
   interface
@@ -48,24 +57,31 @@ module molcas_jk_iface
       real(c_double), intent(in) :: lrs(*)
     end function mjk_upload_vectors
     ! path: file name terminated by c_null_char; iadr(i) = InfVec(jvec+i,3,iSym);
+    ! mbl = MBL(Lu): 8 (CHO_ADRVEC=1, word addresses) or 512 (CHO_ADRVEC=2);
     ! split_bytes = 0, or the piece size of a unit split by MOLCAS_MAXFILESIZE
-    integer(c_int) function mjk_upload_vectors_file(ctx, path, jred, jvec, jnum, iadr, split_bytes) &
+    integer(c_int) function mjk_upload_vectors_file(ctx, path, jred, jvec, jnum, iadr, mbl, split_bytes) &
         bind(C, name='mjk_upload_vectors_file')
       import :: c_ptr, c_int, c_int32_t, c_int64_t, c_char
       type(c_ptr), value :: ctx
       character(kind=c_char), intent(in) :: path(*)
-      integer(c_int32_t), value :: jred
+      integer(c_int32_t), value :: jred, mbl
       integer(c_int64_t), value :: jvec, jnum, split_bytes
       integer(c_int64_t), intent(in) :: iadr(*)
     end function mjk_upload_vectors_file
+    integer(c_int) function mjk_set_ladder(ctx, j_fp32, j_bf16) bind(C, name='mjk_set_ladder')
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: j_fp32, j_bf16
+    end function mjk_set_ladder
+    ! flt(d), porb(d): c_loc of FLT(d)%A0 and Porb(d)%A0
     integer(c_int) function mjk_cho_fscf(ctx, nden, flt, nforb, niorb, porb, dlt, exfac, prec) &
         bind(C, name='mjk_cho_fscf')
       import :: c_ptr, c_int, c_int32_t, c_double
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nden, prec
-      real(c_double), intent(inout) :: flt(*)
+      type(c_ptr), intent(in) :: flt(*), porb(*)
       integer(c_int32_t), intent(in) :: nforb(*), niorb(*)
-      real(c_double), intent(in) :: porb(*), dlt(*)
+      real(c_double), intent(in) :: dlt(*)
       real(c_double), value :: exfac
     end function mjk_cho_fscf
     integer(c_int) function mjk_cho_fscf_sym(ctx, nsym, nbas, nden, flt, nforb, niorb, porb, dlt, &
@@ -74,8 +90,8 @@ module molcas_jk_iface
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsym, nden, prec
       integer(c_int32_t), intent(in) :: nbas(*), nforb(*), niorb(*)
-      real(c_double), intent(inout) :: flt(*)
-      real(c_double), intent(in) :: porb(*), dlt(*)
+      type(c_ptr), intent(in) :: flt(*), porb(*)
+      real(c_double), intent(in) :: dlt(*)
       real(c_double), value :: exfac
     end function mjk_cho_fscf_sym
     integer(c_int) function mjk_cho_lk_scf_sym(ctx, nsym, nbas, nden, flt, nforb, niorb, porb, plt, &
@@ -84,8 +100,8 @@ module molcas_jk_iface
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsym, nden, nscreen, prec
       integer(c_int32_t), intent(in) :: nbas(*), nforb(*), niorb(*)
-      real(c_double), intent(inout) :: flt(*)
-      real(c_double), intent(in) :: porb(*), plt(*)
+      type(c_ptr), intent(in) :: flt(*), porb(*)
+      real(c_double), intent(in) :: plt(*)
       real(c_double), value :: factxi, dmpk, dfmat
     end function mjk_cho_lk_scf_sym
     integer(c_int) function mjk_cho_lk_casscf(ctx, dlt, flt, msq, w_pwxy, factxi, nfiorb, naorb, nchm, &
@@ -93,8 +109,9 @@ module molcas_jk_iface
         bind(C, name='mjk_cho_lk_casscf')
       import :: c_ptr, c_int, c_int32_t, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(in) :: dlt(*), msq(*), ash1(*), ash2(*), cmo(*)
-      real(c_double), intent(inout) :: flt(*), w_pwxy(*)
+      type(c_ptr), intent(in) :: dlt(*), flt(*)
+      real(c_double), intent(in) :: msq(*), ash1(*), ash2(*), cmo(*)
+      real(c_double), intent(inout) :: w_pwxy(*)
       real(c_double), value :: factxi, dmpk, dfmat, exfac
       integer(c_int32_t), value :: nfiorb, naorb, nchm, do_active, nscreen, nfro, norb, prec
     end function mjk_cho_lk_casscf
@@ -109,19 +126,19 @@ module molcas_jk_iface
       import :: c_ptr, c_int, c_int32_t, c_double
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nden, nscreen, nshell, prec
-      real(c_double), intent(inout) :: flt(*), klt(*)
+      type(c_ptr), intent(in) :: flt(*), klt(*), porb(*)
       integer(c_int32_t), intent(in) :: nforb(*), niorb(*), nbassh(*)
-      real(c_double), intent(in) :: porb(*), plt(*)
+      real(c_double), intent(in) :: plt(*)
       real(c_double), value :: factxi, dmpk, dfmat
     end function mjk_cho_lk_scf
     integer(c_int) function mjk_cho_fmcscf(ctx, flt, nforb, niorb, naorb, factxi, dlt, do_active, &
         porb1, nchm, porb2, porb3, w_pwxy, cmo, nfro, norb, exfac, prec) bind(C, name='mjk_cho_fmcscf')
       import :: c_ptr, c_int, c_int32_t, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(inout) :: flt(*)
+      type(c_ptr), intent(in) :: flt(*), dlt(*)
       integer(c_int32_t), value :: nforb, niorb, naorb, do_active, nchm, nfro, norb, prec
       real(c_double), value :: factxi, exfac
-      real(c_double), intent(in) :: dlt(*), porb1(*), porb2(*), porb3(*), cmo(*)
+      real(c_double), intent(in) :: porb1(*), porb2(*), porb3(*), cmo(*)
       real(c_double), intent(inout) :: w_pwxy(*)
     end function mjk_cho_fmcscf
   end interface
@@ -142,60 +159,106 @@ contains
     jk_ctx = c_null_ptr
   end subroutine jk_gpu_final
 
-  ! CHO_FSCF(rc,nDen,FLT,nForb,nIorb,Porb,DLT,ExFac), nSym = 1:
-  ! FLT(:,d) += J[DLT(:,1)] - ExFac * sum_Jk X X^T with Porb(d) = A2(nOcc,nBas).
-  subroutine cho_fscf_gpu(rc, nDen, FLT, nForb, nIorb, Porb, DLT, ExFac)
+  integer(c_int32_t) function prec_of(prec)
+    integer, intent(in), optional :: prec
+    prec_of = MJK_PREC_FP64
+    if (present(prec)) prec_of = int(prec, c_int32_t)
+  end function prec_of
+
+  ! CHO_FSCF(rc,nDen,FLT,nForb,nIorb,Porb,DLT,ExFac), nSym = 1, called as
+  !   call cho_fscf_gpu(rc,nDen,FLT(1)%A0,FLT(2)%A0,nForb,nIorb,Porb(1)%A0,Porb(2)%A0,DLT(1)%A0,ExFac)
+  ! FLTd += J[DLT] - ExFac * sum_Jk X X^T with Porbd = A2(nOcc,nBas); the
+  ! second-density arrays are referenced only when nDen = 2.
+  subroutine cho_fscf_gpu(rc, nDen, FLT1, FLT2, nForb, nIorb, Porb1, Porb2, DLT, ExFac, prec)
     integer, intent(out) :: rc
-    integer, intent(in) :: nDen, nForb(nDen), nIorb(nDen)
-    real(8), intent(inout) :: FLT(*)
-    real(8), intent(in) :: Porb(*), DLT(*), ExFac
+    integer, intent(in) :: nDen, nForb(8, *), nIorb(8, *)
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
+    real(c_double), intent(in), target :: Porb1(*), Porb2(*)
+    real(c_double), intent(in) :: DLT(*)
+    real(8), intent(in) :: ExFac
+    integer, intent(in), optional :: prec
     integer(c_int32_t) :: nf(2), ni(2)
+    type(c_ptr) :: pf(2), pp(2)
+    integer :: d
     nf = 0
     ni = 0
-    nf(1:min(nDen, 2)) = int(nForb(1:min(nDen, 2)), c_int32_t)
-    ni(1:min(nDen, 2)) = int(nIorb(1:min(nDen, 2)), c_int32_t)
-    rc = int(mjk_cho_fscf(jk_ctx, int(nDen, c_int32_t), FLT, nf, ni, Porb, DLT, real(ExFac, c_double), &
-                          MJK_PREC_FP64))
+    do d = 1, min(nDen, 2)
+      nf(d) = int(nForb(1, d), c_int32_t)
+      ni(d) = int(nIorb(1, d), c_int32_t)
+    end do
+    pf(1) = c_loc(FLT1(1))
+    pp(1) = c_loc(Porb1(1))
+    pf(2) = pf(1)
+    pp(2) = pp(1)
+    if (nDen == 2) then
+      pf(2) = c_loc(FLT2(1))
+      pp(2) = c_loc(Porb2(1))
+    end if
+    rc = int(mjk_cho_fscf(jk_ctx, int(nDen, c_int32_t), pf, nf, ni, pp, DLT, real(ExFac, c_double), &
+                          prec_of(prec)))
   end subroutine cho_fscf_gpu
 
-  ! CHO_FSCF(rc,nDen,FLT,nForb,nIorb,Porb,DLT,ExFac) with nSym irreps: FLT, DLT and
-  ! Porb are the %A0 memory of the DSBA types (irrep blocks back to back), nForb
-  ! and nIorb the reference's (8,nDen) arrays.
-  subroutine cho_fscf_sym_gpu(rc, nSym, nBas, nDen, FLT, nForb, nIorb, Porb, DLT, ExFac)
+  ! CHO_FSCF with nSym irreps: FLTd, DLT and Porbd are the %A0 memory of the
+  ! DSBA types (irrep blocks back to back), nForb and nIorb the reference's
+  ! (8,nDen) arrays.
+  subroutine cho_fscf_sym_gpu(rc, nSym, nBas, nDen, FLT1, FLT2, nForb, nIorb, Porb1, Porb2, DLT, ExFac, prec)
     integer, intent(out) :: rc
-    integer, intent(in) :: nSym, nBas(nSym), nDen, nForb(8, nDen), nIorb(8, nDen)
-    real(8), intent(inout) :: FLT(*)
-    real(8), intent(in) :: Porb(*), DLT(*), ExFac
+    integer, intent(in) :: nSym, nBas(nSym), nDen, nForb(8, *), nIorb(8, *)
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
+    real(c_double), intent(in), target :: Porb1(*), Porb2(*)
+    real(c_double), intent(in) :: DLT(*)
+    real(8), intent(in) :: ExFac
+    integer, intent(in), optional :: prec
     integer(c_int32_t) :: nb(8), nf(8, 2), ni(8, 2)
+    type(c_ptr) :: pf(2), pp(2)
     nb = 0
     nf = 0
     ni = 0
     nb(1:nSym) = int(nBas(1:nSym), c_int32_t)
     nf(:, 1:min(nDen, 2)) = int(nForb(:, 1:min(nDen, 2)), c_int32_t)
     ni(:, 1:min(nDen, 2)) = int(nIorb(:, 1:min(nDen, 2)), c_int32_t)
-    rc = int(mjk_cho_fscf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), FLT, nf, ni, &
-                              Porb, DLT, real(ExFac, c_double), MJK_PREC_FP64))
+    pf(1) = c_loc(FLT1(1))
+    pp(1) = c_loc(Porb1(1))
+    pf(2) = pf(1)
+    pp(2) = pp(1)
+    if (nDen == 2) then
+      pf(2) = c_loc(FLT2(1))
+      pp(2) = c_loc(Porb2(1))
+    end if
+    rc = int(mjk_cho_fscf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), pf, nf, ni, &
+                              pp, DLT, real(ExFac, c_double), prec_of(prec)))
   end subroutine cho_fscf_sym_gpu
 
-  ! CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,dFmat) with nSym
-  ! irreps: FLT, PLT and Porb (MSQ A2(nBas(s),nBas(s)) blocks) as the %A0 memory of the
-  ! DSBA types; KLT is not returned.
-  subroutine cho_lk_scf_sym_gpu(rc, nSym, nBas, nDen, FLT, nForb, nIorb, Porb, PLT, FactXI, nScreen, &
-                                dmpk, dFmat)
+  ! CHO_LK_SCF with nSym irreps: FLTd, PLT and Porbd (MSQ A2(nBas(s),nBas(s))
+  ! blocks) as the %A0 memory of the DSBA types; KLT is not returned.
+  subroutine cho_lk_scf_sym_gpu(rc, nSym, nBas, nDen, FLT1, FLT2, nForb, nIorb, Porb1, Porb2, PLT, FactXI, &
+                                nScreen, dmpk, dFmat, prec)
     integer, intent(out) :: rc
-    integer, intent(in) :: nSym, nBas(nSym), nDen, nForb(8, nDen), nIorb(8, nDen), nScreen
-    real(8), intent(inout) :: FLT(*)
-    real(8), intent(in) :: Porb(*), PLT(*), FactXI, dmpk, dFmat
+    integer, intent(in) :: nSym, nBas(nSym), nDen, nForb(8, *), nIorb(8, *), nScreen
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
+    real(c_double), intent(in), target :: Porb1(*), Porb2(*)
+    real(c_double), intent(in) :: PLT(*)
+    real(8), intent(in) :: FactXI, dmpk, dFmat
+    integer, intent(in), optional :: prec
     integer(c_int32_t) :: nb(8), nf(8, 2), ni(8, 2)
+    type(c_ptr) :: pf(2), pp(2)
     nb = 0
     nf = 0
     ni = 0
     nb(1:nSym) = int(nBas(1:nSym), c_int32_t)
     nf(:, 1:min(nDen, 2)) = int(nForb(:, 1:min(nDen, 2)), c_int32_t)
     ni(:, 1:min(nDen, 2)) = int(nIorb(:, 1:min(nDen, 2)), c_int32_t)
-    rc = int(mjk_cho_lk_scf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), FLT, nf, ni, &
-                                Porb, PLT, real(FactXI, c_double), int(nScreen, c_int32_t), &
-                                real(dmpk, c_double), real(dFmat, c_double), MJK_PREC_FP64))
+    pf(1) = c_loc(FLT1(1))
+    pp(1) = c_loc(Porb1(1))
+    pf(2) = pf(1)
+    pp(2) = pp(1)
+    if (nDen == 2) then
+      pf(2) = c_loc(FLT2(1))
+      pp(2) = c_loc(Porb2(1))
+    end if
+    rc = int(mjk_cho_lk_scf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), pf, nf, ni, &
+                                pp, PLT, real(FactXI, c_double), int(nScreen, c_int32_t), &
+                                real(dmpk, c_double), real(dFmat, c_double), prec_of(prec)))
   end subroutine cho_lk_scf_sym_gpu
 
   ! CHO_FOCK_DFT_RED(irc,DLT,FLT), nSym = 1: FLT = J[DLT] (pure DFT, no exchange).
@@ -206,62 +269,106 @@ contains
     irc = int(mjk_cho_fock_dft_red(jk_ctx, DLT, FLT))
   end subroutine cho_fock_dft_red_gpu
 
-  ! CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,dFmat), nSym = 1,
-  ! unscreened: FLT(:,d) += J[PLT(:,1)] - FactXI * K[Porb(d)], Porb(d) = MSQ A2(nBas,nBas);
-  ! KLT(:,d) accumulates -FactXI*K in shell-pair blocks of the nShell shells nBasSh.
-  subroutine cho_lk_scf_gpu(rc, nDen, FLT, KLT, nForb, nIorb, Porb, PLT, FactXI, nScreen, dmpk, dFmat, &
-                            nShell, nBasSh)
+  ! CHO_LK_SCF(rc,nDen,FLT,KLT,nForb,nIorb,Porb,PLT,FactXI,nScreen,dmpk,dFmat), nSym = 1:
+  ! FLTd += J[PLT] - FactXI * K[Porbd], Porbd = MSQ A2(nBas,nBas); KLTd accumulates
+  ! -FactXI*K in shell-pair blocks of the nShell shells nBasSh.
+  subroutine cho_lk_scf_gpu(rc, nDen, FLT1, FLT2, KLT1, KLT2, nForb, nIorb, Porb1, Porb2, PLT, FactXI, &
+                            nScreen, dmpk, dFmat, nShell, nBasSh, prec)
     integer, intent(out) :: rc
-    integer, intent(in) :: nDen, nForb(nDen), nIorb(nDen), nScreen, nShell, nBasSh(*)
-    real(8), intent(inout) :: FLT(*), KLT(*)
-    real(8), intent(in) :: Porb(*), PLT(*), FactXI, dmpk, dFmat
+    integer, intent(in) :: nDen, nForb(8, *), nIorb(8, *), nScreen, nShell, nBasSh(*)
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*), KLT1(*), KLT2(*)
+    real(c_double), intent(in), target :: Porb1(*), Porb2(*)
+    real(c_double), intent(in) :: PLT(*)
+    real(8), intent(in) :: FactXI, dmpk, dFmat
+    integer, intent(in), optional :: prec
     integer(c_int32_t) :: nf(2), ni(2)
     integer(c_int32_t), allocatable :: sh(:)
+    type(c_ptr) :: pf(2), pk(2), pp(2)
+    integer :: d
     nf = 0
     ni = 0
-    nf(1:min(nDen, 2)) = int(nForb(1:min(nDen, 2)), c_int32_t)
-    ni(1:min(nDen, 2)) = int(nIorb(1:min(nDen, 2)), c_int32_t)
+    do d = 1, min(nDen, 2)
+      nf(d) = int(nForb(1, d), c_int32_t)
+      ni(d) = int(nIorb(1, d), c_int32_t)
+    end do
+    pf(1) = c_loc(FLT1(1))
+    pk(1) = c_loc(KLT1(1))
+    pp(1) = c_loc(Porb1(1))
+    pf(2) = pf(1)
+    pk(2) = pk(1)
+    pp(2) = pp(1)
+    if (nDen == 2) then
+      pf(2) = c_loc(FLT2(1))
+      pk(2) = c_loc(KLT2(1))
+      pp(2) = c_loc(Porb2(1))
+    end if
     allocate(sh(max(nShell, 1)))
     sh = 0
     if (nShell > 0) sh(1:nShell) = int(nBasSh(1:nShell), c_int32_t)
-    rc = int(mjk_cho_lk_scf(jk_ctx, int(nDen, c_int32_t), FLT, KLT, nf, ni, Porb, PLT, real(FactXI, c_double), &
+    rc = int(mjk_cho_lk_scf(jk_ctx, int(nDen, c_int32_t), pf, pk, nf, ni, pp, PLT, real(FactXI, c_double), &
                             int(nScreen, c_int32_t), real(dmpk, c_double), real(dFmat, c_double), &
-                            int(nShell, c_int32_t), sh, MJK_PREC_FP64))
+                            int(nShell, c_int32_t), sh, prec_of(prec)))
     deallocate(sh)
   end subroutine cho_lk_scf_gpu
 
   ! CHO_LK_CASSCF(DLT,FLT,MSQ,W_PWXY,FactXI,nFIorb,nAorb,nChM,Ash,DoActive,nScreen,dmpk,dFmat,CMO,ExFac),
-  ! nSym = 1, unscreened; Ash1 = active MOs A2(nAorb,nBas), Ash2 = DA vectors A2(nBas,nBas).
-  subroutine cho_lk_casscf_gpu(rc, DLT, FLT, MSQ, W_PWXY, FactXI, nFIorb, nAorb, nChM, Ash1, Ash2, DoActive, &
-                               nScreen, dmpk, dFmat, CMO, nFro, nOrb, ExFac)
+  ! nSym = 1, called with DLT(1)%A0, DLT(2)%A0, FLT(1)%A0, FLT(2)%A0 (DI, DA, FI, FA);
+  ! Ash1 = active MOs A2(nAorb,nBas), Ash2 = DA vectors A2(nBas,nBas).
+  subroutine cho_lk_casscf_gpu(rc, DLT1, DLT2, FLT1, FLT2, MSQ, W_PWXY, FactXI, nFIorb, nAorb, nChM, Ash1, &
+                               Ash2, DoActive, nScreen, dmpk, dFmat, CMO, nFro, nOrb, ExFac, prec)
     integer, intent(out) :: rc
     integer, intent(in) :: nFIorb, nAorb, nChM, nScreen, nFro, nOrb
     logical, intent(in) :: DoActive
-    real(8), intent(in) :: DLT(*), MSQ(*), Ash1(*), Ash2(*), CMO(*), FactXI, dmpk, dFmat, ExFac
-    real(8), intent(inout) :: FLT(*), W_PWXY(*)
+    real(c_double), intent(in), target :: DLT1(*), DLT2(*)
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
+    real(8), intent(in) :: MSQ(*), Ash1(*), Ash2(*), CMO(*), FactXI, dmpk, dFmat, ExFac
+    real(8), intent(inout) :: W_PWXY(*)
+    integer, intent(in), optional :: prec
     integer(c_int32_t) :: act
+    type(c_ptr) :: pd(2), pf(2)
     act = 0
-    if (DoActive) act = 1
-    rc = int(mjk_cho_lk_casscf(jk_ctx, DLT, FLT, MSQ, W_PWXY, real(FactXI, c_double), int(nFIorb, c_int32_t), &
+    pd(1) = c_loc(DLT1(1))
+    pf(1) = c_loc(FLT1(1))
+    pd(2) = pd(1)
+    pf(2) = pf(1)
+    if (DoActive) then
+      act = 1
+      pd(2) = c_loc(DLT2(1))
+      pf(2) = c_loc(FLT2(1))
+    end if
+    rc = int(mjk_cho_lk_casscf(jk_ctx, pd, pf, MSQ, W_PWXY, real(FactXI, c_double), int(nFIorb, c_int32_t), &
                                int(nAorb, c_int32_t), int(nChM, c_int32_t), Ash1, Ash2, act, int(nScreen, c_int32_t), &
                                real(dmpk, c_double), real(dFmat, c_double), CMO, int(nFro, c_int32_t), &
-                               int(nOrb, c_int32_t), real(ExFac, c_double), MJK_PREC_FP64))
+                               int(nOrb, c_int32_t), real(ExFac, c_double), prec_of(prec)))
   end subroutine cho_lk_casscf_gpu
 
-  ! CHO_FMCSCF(rc,FLT,nForb,nIorb,nAorb,FactXI,DLT,DoActive,POrb,nChM,W_PWXY,CMO,ExFac), nSym = 1.
-  subroutine cho_fmcscf_gpu(rc, FLT, nForb, nIorb, nAorb, FactXI, DLT, DoActive, POrb1, nChM, POrb2, &
-                            POrb3, W_PWXY, CMO, nFro, nOrb, ExFac)
+  ! CHO_FMCSCF(rc,FLT,nForb,nIorb,nAorb,FactXI,DLT,DoActive,POrb,nChM,W_PWXY,CMO,ExFac), nSym = 1,
+  ! called with FLT(1)%A0, FLT(2)%A0 (FI, FA) and DLT(1)%A0, DLT(2)%A0 (DI, DA).
+  subroutine cho_fmcscf_gpu(rc, FLT1, FLT2, nForb, nIorb, nAorb, FactXI, DLT1, DLT2, DoActive, POrb1, nChM, &
+                            POrb2, POrb3, W_PWXY, CMO, nFro, nOrb, ExFac, prec)
     integer, intent(out) :: rc
     integer, intent(in) :: nForb, nIorb, nAorb, nChM, nFro, nOrb
     logical, intent(in) :: DoActive
-    real(8), intent(inout) :: FLT(*), W_PWXY(*)
-    real(8), intent(in) :: FactXI, DLT(*), POrb1(*), POrb2(*), POrb3(*), CMO(*), ExFac
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
+    real(c_double), intent(in), target :: DLT1(*), DLT2(*)
+    real(8), intent(inout) :: W_PWXY(*)
+    real(8), intent(in) :: FactXI, POrb1(*), POrb2(*), POrb3(*), CMO(*), ExFac
+    integer, intent(in), optional :: prec
     integer(c_int32_t) :: act
+    type(c_ptr) :: pd(2), pf(2)
     act = 0
-    if (DoActive) act = 1
-    rc = int(mjk_cho_fmcscf(jk_ctx, FLT, int(nForb, c_int32_t), int(nIorb, c_int32_t), int(nAorb, c_int32_t), &
-                            real(FactXI, c_double), DLT, act, POrb1, int(nChM, c_int32_t), POrb2, POrb3, W_PWXY, &
-                            CMO, int(nFro, c_int32_t), int(nOrb, c_int32_t), real(ExFac, c_double), MJK_PREC_FP64))
+    pd(1) = c_loc(DLT1(1))
+    pf(1) = c_loc(FLT1(1))
+    pd(2) = pd(1)
+    pf(2) = pf(1)
+    if (DoActive) then
+      act = 1
+      pd(2) = c_loc(DLT2(1))
+      pf(2) = c_loc(FLT2(1))
+    end if
+    rc = int(mjk_cho_fmcscf(jk_ctx, pf, int(nForb, c_int32_t), int(nIorb, c_int32_t), int(nAorb, c_int32_t), &
+                            real(FactXI, c_double), pd, act, POrb1, int(nChM, c_int32_t), POrb2, POrb3, W_PWXY, &
+                            CMO, int(nFro, c_int32_t), int(nOrb, c_int32_t), real(ExFac, c_double), prec_of(prec)))
   end subroutine cho_fmcscf_gpu
 
 end module molcas_jk_iface

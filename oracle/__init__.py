@@ -21,6 +21,13 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
                            rules, equality with cho_fscf (unscreened)
   cho_lk_casscf         -- dense brute force FI/FA with ExFac-scaled factors,
                            (pw|xy), equality with cho_fmcscf at ExFac = 1
+  cho_lk_scf_screened / cho_lk_casscf_screened (mjk_oracle_lk.c)
+                        -- dmpk = 0 equals the unscreened routines; the
+                           kept shell lists, half-transformed shell pairs
+                           and K blocks equal an independent per-batch
+                           evaluation of the reference's inequalities
+                           (tests/_lk_brute.py); skipped blocks are
+                           bounded by the thresholds on localized inputs
   cd_incore             -- ||D - V V^T||_max <= thr, exact rank, numpy cholesky
   energy                -- closed form tr(hP) + 1/2 tr(PJ) - 1/4 tr(PK)
 """
@@ -39,6 +46,19 @@ _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class _LK(ctypes.Structure):
+    _fields_ = [
+        ("nshell", ctypes.c_int32),
+        ("nbassh", _i32p),
+        ("diag", _dp),
+        ("thrcom", ctypes.c_double),
+        ("maxred", ctypes.c_int32),
+        ("estimate", ctypes.c_int32),
+        ("update", ctypes.c_int32),
+        ("nscreen", ctypes.c_int32),
+    ]
 
 
 class _Chol(ctypes.Structure):
@@ -99,6 +119,20 @@ def lib():
                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
                                         ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_double, ctypes.c_int64]
+        L.ora_lk_thresholds.restype = None
+        L.ora_lk_thresholds.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                        _i32p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
+        L.ora_cho_lk_scf_scr.restype = ctypes.c_int32
+        L.ora_cho_lk_scf_scr.argtypes = [ctypes.POINTER(_Chol), ctypes.c_int32, _dp, _dp, _i32p, _i32p,
+                                         ctypes.POINTER(_dp), _dp, ctypes.c_double, ctypes.c_int32,
+                                         ctypes.c_double, ctypes.c_double, ctypes.POINTER(_LK),
+                                         ctypes.c_int64, _i64p]
+        L.ora_cho_lk_casscf_scr.restype = ctypes.c_int32
+        L.ora_cho_lk_casscf_scr.argtypes = [ctypes.POINTER(_Chol), _dp, _dp, _dp, _dp, ctypes.c_double,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                            ctypes.c_double, _dp, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_double, ctypes.POINTER(_LK), ctypes.c_int64, _i64p]
         L.ora_cd_incore.restype = ctypes.c_int32
         L.ora_cd_incore.argtypes = [_dp, ctypes.c_int32, _dp, ctypes.c_int32, _i32p, ctypes.c_double]
         L.ora_energy.restype = ctypes.c_double
@@ -331,6 +365,90 @@ def cho_lk_casscf(store: CholStore, dlt: np.ndarray, flt: np.ndarray, msq, factx
     if rc != 0:
         raise RuntimeError(f"ora_cho_lk_casscf rc={rc}")
     return f, w[:norb * naorb * nxy]
+
+
+def lk_thresholds(thrcom: float, dfmat: float, dmpk: float, nocc, numvt: int, estimate: bool = False,
+                  maxred: int = 1):
+    """tau and thrv per density (cho_lk_scf.F90:143-171)."""
+    nd = len(nocc)
+    no = np.array(nocc, dtype=np.int32)
+    tau, thrv = np.zeros(nd), np.zeros(nd)
+    lib().ora_lk_thresholds(float(thrcom), float(dfmat), float(dmpk), nd, _p(no, _i32p), int(numvt),
+                            int(bool(estimate)), int(maxred), _p(tau), _p(thrv))
+    return tau, thrv
+
+
+def _lk_struct(store, nbassh, diag, thrcom, maxred, estimate, update, keep):
+    sh = np.ascontiguousarray(nbassh, dtype=np.int32)
+    dg = _f64(diag).reshape(-1)
+    keep += [sh, dg]
+    return _LK(len(sh), _p(sh, _i32p), _p(dg), float(thrcom),
+               int(store._c.nred if maxred is None else maxred), int(bool(estimate)), int(bool(update)), 0)
+
+
+def cho_lk_scf_screened(store: CholStore, flt: np.ndarray, nocc, porbs, plt: np.ndarray, factxi: float,
+                        nbassh, diag, thrcom: float, nscreen: int = 10, dmpk: float = 0.1,
+                        dfmat: float = 1.0, maxred=None, estimate: bool = False, update: bool = True,
+                        batch: int = 25):
+    """CHO_LK_SCF with the reference's LK screening (nforb = 0, niorb = nocc).
+    porbs[d]: (n, nocc[d]) AO x orbital matrix; diag: the integral diagonal
+    on the LT (CHO_IODIAG); batch: Cho_LK_MaxVecPerBatch.  Returns (flt, klt,
+    stats) with stats = [listed (batch, orbital, shell), half-transformed
+    shell pairs, K blocks]."""
+    nden = len(nocc)
+    n = store.n
+    f = _f64(flt).reshape(nden, -1).copy()
+    k = np.zeros_like(f)
+    nf = np.zeros(nden, dtype=np.int32)
+    ni = np.array(nocc, dtype=np.int32)
+    cs = []
+    for p in porbs:
+        sq = np.zeros((n, n))
+        sq[:, :p.shape[1]] = p
+        cs.append(np.ascontiguousarray(sq.T))
+    arr = (_dp * nden)(*[_p(c) for c in cs])
+    keep = []
+    lk = _lk_struct(store, nbassh, diag, thrcom, maxred, estimate, update, keep)
+    stats = np.zeros(3, dtype=np.int64)
+    rc = lib().ora_cho_lk_scf_scr(ctypes.byref(store._c), nden, _p(f), _p(k), _p(nf, _i32p), _p(ni, _i32p),
+                                  arr, _p(_f64(plt)), float(factxi), int(nscreen), float(dmpk), float(dfmat),
+                                  ctypes.byref(lk), int(batch), _p(stats, _i64p))
+    if rc != 0:
+        raise RuntimeError(f"ora_cho_lk_scf_scr rc={rc}")
+    return f, k, stats
+
+
+def cho_lk_casscf_screened(store: CholStore, dlt: np.ndarray, flt: np.ndarray, msq, factxi: float,
+                           nfiorb: int, naorb: int, nchm: int, ash1, ash2, do_active: bool, cmo,
+                           nfro: int, norb: int, exfac: float, nbassh, diag, thrcom: float,
+                           nscreen: int = 10, dmpk: float = 0.1, dfmat: float = 1.0, maxred=None,
+                           estimate: bool = False, update: bool = True, batch: int = 25):
+    """CHO_LK_CASSCF with the reference's LK screening.  Arguments as
+    cho_lk_casscf plus the LK state.  Returns (flt, w, stats)."""
+    n = store.n
+
+    def square(a, k):
+        sq = np.zeros((n, n))
+        if a is not None and k > 0:
+            sq[:, :k] = np.asarray(a)[:, :k]
+        return np.ascontiguousarray(sq.T)
+
+    f = _f64(flt).reshape(2, -1).copy()
+    d = _f64(dlt).reshape(2, -1)
+    a1 = _f64(ash1) if naorb else np.zeros(1)
+    cm = _f64(np.asarray(cmo).T) if cmo is not None else np.zeros(1)
+    nxy = naorb * (naorb + 1) // 2
+    w = np.zeros(max(norb * naorb * nxy, 1))
+    keep = []
+    lk = _lk_struct(store, nbassh, diag, thrcom, maxred, estimate, update, keep)
+    stats = np.zeros(3, dtype=np.int64)
+    rc = lib().ora_cho_lk_casscf_scr(ctypes.byref(store._c), _p(d), _p(f), _p(square(msq, nfiorb)), _p(w),
+                                     float(factxi), nfiorb, naorb, nchm, _p(a1), _p(square(ash2, nchm)),
+                                     int(do_active), int(nscreen), float(dmpk), float(dfmat), _p(cm), nfro,
+                                     norb, float(exfac), ctypes.byref(lk), int(batch), _p(stats, _i64p))
+    if rc != 0:
+        raise RuntimeError(f"ora_cho_lk_casscf_scr rc={rc}")
+    return f, w[:norb * naorb * nxy], stats
 
 
 def cd_incore(x: np.ndarray, thr: float, mxvec=None):

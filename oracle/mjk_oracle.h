@@ -95,6 +95,45 @@ int32_t ora_cho_lk_casscf(const ora_chol *ch, const double *dlt, double *flt,
                           const double *cmo, int32_t nfro, int32_t norb, double exfac,
                           int64_t nvec_batch);
 
+/* LK screening state of CHO_LK_SCF / CHO_LK_CASSCF that the reference reads
+ * from module globals: the shells (nShell, nBasSh; the AOs of a shell are
+ * contiguous, shells in order), the diagonal CHO_IODIAG reads (LT storage,
+ * zero outside the first reduced set), ThrCom, MaxRed and the
+ * Fock_util_global switches Estimate / Update (src/fock_util/
+ * fock_util_global.F90:56-59).  nscreen is set from the call's argument. */
+typedef struct {
+  int32_t nshell;
+  const int32_t *nbassh;
+  const double *diag;
+  double thrcom;
+  int32_t maxred;
+  int32_t estimate, update;
+  int32_t nscreen;
+} ora_lk;
+
+void ora_lk_thresholds(double thrcom, double dfmat, double dmpk, int32_t nden,
+                       const int32_t *nocc, int64_t numvt, int32_t estimate,
+                       int32_t maxred, double *tau, double *thrv);
+
+int32_t ora_lk_exchange(const ora_chol *ch, int32_t nden, const int32_t *nocc,
+                        const double *const *csq, const double *fact, const double *tau,
+                        const double *thrv, int32_t cas_cut, const ora_lk *lk,
+                        int64_t nvec_batch, double *klt, int64_t *stats);
+
+int32_t ora_cho_lk_scf_scr(const ora_chol *ch, int32_t nden, double *flt, double *klt,
+                           const int32_t *nforb, const int32_t *niorb,
+                           const double *const *porb, const double *plt, double factxi,
+                           int32_t nscreen, double dmpk, double dfmat, const ora_lk *lk,
+                           int64_t nvec_batch, int64_t *stats);
+
+int32_t ora_cho_lk_casscf_scr(const ora_chol *ch, const double *dlt, double *flt,
+                              const double *msq, double *w_pwxy, double factxi,
+                              int32_t nfiorb, int32_t naorb, int32_t nchm,
+                              const double *ash1, const double *ash2, int32_t do_active,
+                              int32_t nscreen, double dmpk, double dfmat, const double *cmo,
+                              int32_t nfro, int32_t norb, double exfac, const ora_lk *lk,
+                              int64_t nvec_batch, int64_t *stats);
+
 int32_t ora_cd_incore(double *x, int32_t n, double *vec, int32_t mxvec,
                       int32_t *numcho, double thr);
 

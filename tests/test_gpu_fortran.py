@@ -1,6 +1,7 @@
-"""The CHO_FSCF drop-in called from Fortran through the ISO_C_BINDING module
-(openmolcas_amd/fortran), C1 and with two irreps, checked against the oracle
-(FP64, relative 1e-11)."""
+"""The drop-ins called from Fortran through the ISO_C_BINDING module
+(openmolcas_amd/fortran): CHO_FSCF C1 and with two irreps, and the
+two-density calls (UHF, CASSCF FI/FA) on separately allocated arrays,
+checked against the oracle (FP64, relative 1e-11)."""
 import os
 import subprocess
 
@@ -8,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from openmolcas_amd.build import FORTRAN_BIN, FORTRAN_SYM_BIN
+from openmolcas_amd.build import FORTRAN_BIN, FORTRAN_SYM_BIN, FORTRAN_UHF_CAS_BIN
 from openmolcas_amd.synthetic import chol_vectors_np, fold, npair, orbitals_np
 
 pytestmark = pytest.mark.gpu
@@ -55,3 +56,59 @@ def test_fortran_cho_fscf_symmetry_dropin(gpu, tmp_path):
     flt = np.fromfile(fout, dtype=np.float64)
     want = p["want"][0]
     assert np.max(np.abs(flt - want)) / np.max(np.abs(want)) < 1e-11
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_UHF_CAS_BIN), reason="Fortran driver not built (no amdflang)")
+def test_fortran_two_density_separate_arrays(gpu, tmp_path):
+    """UHF (nDen = 2) CHO_FSCF / CHO_LK_SCF and CHO_FMCSCF / CHO_LK_CASSCF with
+    FI/FA called from Fortran with every per-density array allocated on its
+    own and followed by canary words (src/scf/choscf_drv.F90:420,433,
+    src/fock_util/cho_cas_drv.F90:277,285): results against the oracle and
+    no canary touched."""
+    n, M = 37, 29
+    oa, ob, ni, na = 6, 4, 5, 4
+    nfro, norb = 1, n - 1
+    exfac, factxi = 0.8, -0.5
+    L = chol_vectors_np(n, M, seed=93)
+    rng = np.random.default_rng(94)
+    Ca, Cb = orbitals_np(n, oa, seed=95), orbitals_np(n, ob, seed=96)
+    dlt = fold(Ca @ Ca.T + Cb @ Cb.T)
+    flta0, fltb0 = rng.standard_normal(npair(n)), rng.standard_normal(npair(n))
+    cmo, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Ci, Cact = cmo[:, :ni], cmo[:, ni:ni + na]
+    occ = rng.uniform(0.1, 1.9, na)
+    DI, DA = fold(2.0 * Ci @ Ci.T), fold((Cact * occ) @ Cact.T)
+    fi0, fa0 = rng.standard_normal(npair(n)), rng.standard_normal(npair(n))
+    p1, p2 = np.sqrt(2.0) * Ci, Cact * np.sqrt(occ)
+
+    def square(c):  # A2(nBas,nBas), AO index fastest
+        sq = np.zeros((n, n))
+        sq[:, :c.shape[1]] = c
+        return np.ascontiguousarray(sq.T)
+
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([n, M, oa, ob, ni, na, na, nfro, norb, exfac, factxi], dtype=np.float64).tofile(f)
+        for a in (L, Ca, Cb, square(Ca), square(Cb), dlt, flta0, fltb0, DI, DA, fi0, fa0, p1, p2, Cact,
+                  cmo.T, square(p1), square(p2)):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+    r = subprocess.run([FORTRAN_UHF_CAS_BIN, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    out = np.fromfile(fout, dtype=np.float64)
+    npr, nw = npair(n), norb * na * (na * (na + 1) // 2)
+    sizes = [npr] * 8 + [nw] + [npr] * 2 + [nw]
+    parts = np.split(out[:-1], np.cumsum(sizes)[:-1])
+    assert out[-1] == 0.0, f"{int(out[-1])} canary words after the per-density arrays were written"
+    st = oracle.CholStore.full(L)
+    want = oracle.cho_fscf(st, np.stack([flta0, fltb0]), [oa, ob], [Ca, Cb], dlt, exfac)
+    lk_f, lk_k = oracle.cho_lk_scf(st, np.stack([flta0, fltb0]), [oa, ob], [Ca, Cb], dlt, exfac)
+    fm_f, fm_w = oracle.cho_fmcscf(st, np.stack([fi0, fa0]), ni, na, factxi, np.stack([DI, DA]), True,
+                                   p1, na, p2, Cact, cmo, nfro, norb)
+    lc_f, lc_w = oracle.cho_lk_casscf(st, np.stack([DI, DA]), np.stack([fi0, fa0]), p1, factxi, ni, na, na,
+                                      Cact, p2, True, cmo, nfro, norb, 1.0)
+    expect = [want[0], want[1], lk_f[0], lk_f[1], lk_k[0], lk_k[1], fm_f[0], fm_f[1], fm_w,
+              lc_f[0], lc_f[1], lc_w]
+    for got, exp in zip(parts, expect):
+        assert np.max(np.abs(got - exp)) / max(np.max(np.abs(exp)), 1e-300) < 1e-11

@@ -1,4 +1,5 @@
-"""Tile screening of the exchange (the MI355X form of LK screening).
+"""Tile screening of the device-engine exchange (mjk_exchange_screened).  The
+drop-ins' screening is the reference's LK, tested in test_gpu_lk.py.
 
 * Localized system (vectors and orbitals with Gaussian AO envelopes): the
   screened exchange skips K tiles and half-transform tiles, and its error
@@ -9,8 +10,6 @@
                ||X^J||_F <= ||C||_F ||L^J||_F.
 * Dense synthetic vectors (no locality): nothing is skipped and the result is
   bit-identical to the unscreened build.
-* LK drop-in with screening on: thresholds from Cho_LK_ScreeningThreshold and
-  dmpk, FLT within the same bound of the unscreened drop-in.
 """
 import numpy as np
 import pytest
@@ -92,34 +91,4 @@ def test_screening_dense_vectors_skip_nothing(gpu):
     assert torch.equal(K0, K1)
     K32 = jk.exchange(C, "fp32")
     assert torch.equal(jk.exchange_screened(C, 1e-10, 1e-10, "fp32"), K32)
-    jk.close()
-
-
-def test_lk_scf_dropin_with_screening(gpu):
-    n, M, o = 320, 260, 40
-    L, C = localized_system(n, M, o, width=9.0, seed=11)
-    ii, jj = np.tril_indices(n)
-    P = C @ C.T
-    plt = np.where(ii == jj, 1.0, 2.0) * P[ii, jj]
-    jk = JKContext(n, M, device=gpu)
-    jk.upload_packed(L)
-    f0 = np.zeros((1, npair(n)))
-    jk.cho_lk_scf(1, f0, None, [0], [o], [C], plt, 0.5, dmpk=0.1, dfmat=1.0)
-    jk.set_lk_screening(True, thrcom=1e-6)
-    f1 = np.zeros((1, npair(n)))
-    jk.cho_lk_scf(1, f1, None, [0], [o], [C], plt, 0.5, dmpk=0.1, dfmat=1.0)
-    st = jk.screening_stats()
-    print("LK drop-in:", st)
-    assert st["k_skipped"] > 0
-    lkthr = 1e-6
-    tau = lkthr / o * 0.1
-    thrv = np.sqrt(lkthr / (o * M)) * 0.1
-    err = np.max(np.abs(f1 - f0))
-    assert err <= 0.5 * error_bound(st, tau, thrv, L, C, n, 64), (err, st)
-    jk.set_lk_screening(False)
-    f2 = np.zeros((1, npair(n)))
-    jk.cho_lk_scf(1, f2, None, [0], [o], [C], plt, 0.5, dmpk=0.1, dfmat=1.0)
-    assert np.array_equal(f2, f0)
-    with pytest.raises(Exception):
-        jk.exchange_screened([torch.from_numpy(C)], 1e-9, 0.0, "bf16x3")
     jk.close()

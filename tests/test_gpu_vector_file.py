@@ -141,3 +141,48 @@ def test_vector_file_errors(gpu, tmp_path):
     assert torch.equal(K, ref.exchange([C]))
     jk.close()
     ref.close()
+
+
+def write_block_file(path, blocks, mbl=512):
+    """CHO_ADRVEC = 2 layout: the file is opened with DANAME_MF, so a disk
+    address counts blocks of MBL = 512 bytes (src/io_util/ddafile.F90:36,
+    src/io_util/fast_io.F90:81) and every write ends on the next block
+    boundary (ddafile.F90:42).  Returns the block address per vector."""
+    iadr, chunks, at = [], [], 2  # the first two blocks hold something else
+    chunks.append(np.full(at * mbl // 8, np.nan))
+    for blk in blocks:
+        for v in blk:
+            iadr.append(at)
+            nblk = -(-v.size * 8 // mbl)
+            pad = np.full(nblk * mbl // 8 - v.size, np.nan)  # never read
+            chunks += [v, pad]
+            at += nblk
+    np.concatenate(chunks).astype("<f8").tofile(path)
+    return np.array(iadr, dtype=np.int64)
+
+
+def test_block_addressed_vector_file(gpu, tmp_path):
+    n, M0, M1 = 33, 6, 5
+    npr = npair(n)
+    rs1, L0, L1 = layout(n, M0, M1, seed=13)
+    path = str(tmp_path / "CHVEC1")
+    iadr = write_block_file(path, [L0, L1])
+    C = torch.from_numpy(orbitals_np(n, 5, seed=14))
+    dlt = torch.from_numpy(fold(np.eye(n)))
+    mem = JKContext(n, M0 + M1, device=gpu)
+    fil = JKContext(n, M0 + M1, device=gpu)
+    for jk in (mem, fil):
+        jk.set_reduced_set(0, np.arange(npr))
+        jk.set_reduced_set(1, rs1)
+    mem.upload_vectors(0, 0, L0)
+    mem.upload_vectors(1, M0, L1)
+    fil.upload_vectors_file(path, 0, 0, iadr[:M0], mbl=512)
+    fil.upload_vectors_file(path, 1, M0, iadr[M0:], mbl=512)
+    J0, K0 = mem.coulomb_exchange(dlt, [C])
+    J1, K1 = fil.coulomb_exchange(dlt, [C])
+    assert torch.equal(J0, J1) and torch.equal(K0, K1)
+    with pytest.raises(MjkError) as e:  # only word (8) and block (512) addressing exist
+        fil.upload_vectors_file(path, 0, 0, iadr[:M0], mbl=7)
+    assert e.value.rc == 1
+    mem.close()
+    fil.close()
