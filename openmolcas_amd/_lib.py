@@ -71,6 +71,13 @@ SIGNATURES = {
     "mjk_set_shells": (_c.c_int, [_vp, _i32, _vp]),
     "mjk_set_lk_diag": (_c.c_int, [_vp, _vp, _i32, _i32, _i32]),
     "mjk_lk_stats": (_c.c_int, [_vp, _vp]),
+    "mjk_choinfo_read": (_c.c_int, [_c.POINTER(_vp), _c.c_char_p, _c.c_char_p, _vp]),
+    "mjk_choinfo_error": (_c.c_char_p, [_vp]),
+    "mjk_choinfo_free": (_c.c_int, [_vp]),
+    "mjk_choinfo_dims": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mjk_choinfo_reduced_set": (_c.c_int, [_vp, _i32, _vp, _vp]),
+    "mjk_choinfo_infvec": (_c.c_int, [_vp, _i32, _vp]),
+    "mjk_load_choinfo": (_c.c_int, [_vp, _vp, _c.c_char_p, _c.c_char_p, _i64]),
     "mjk_cd_incore": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _c.c_double, _vp]),
     "mjk_fock_update": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mjk_energy": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),

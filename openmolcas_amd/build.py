@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmolcas_jk.so")
 
 SOURCES = ["capi.hip", "coulomb.hip", "exchange.hip", "assembly.hip", "puvx.hip", "backward.hip", "peaks.hip", "tune.hip",
-           "deco.hip", "lk.hip"]
+           "deco.hip", "lk.hip", "chofiles.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -844,6 +844,8 @@ int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_
   if (nden != 1 && nden != 2)
     return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: invalid parameter nDen");  // cho_lk_scf.F90:93-96
   if (!flt || !nforb || !niorb || !plt) return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: null pointer");
+  if (c->lk_on && nsym > 1)  // the reference screens per irrep shell blocks (nBasSh(iSym,iSh))
+    return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: LK screening with nSym > 1 is not supported");
   SymMap m;
   int32_t nocc[2] = {0, 0};
   int rc;

@@ -587,3 +587,58 @@ class JKContext:
         calls = (ctypes.c_int64 * len(_lib.PHASES))()
         self._check(self.lib.mjk_phase_times(self._h, ms, calls))
         return {p: (ms[i], calls[i]) for i, p in enumerate(_lib.PHASES)}
+
+
+class CholeskyFiles:
+    """The reference's Cholesky index files read on the host (Cho_X_Init,
+    nSym = 1): CHORST (restart info, InfVec) and CHRED (reduced sets), the
+    element maps rebuilt from isoshl (the runfile's ISOSHL, 1-based shell of
+    each AO).  No device is needed to read; ``load`` feeds a JKContext."""
+
+    def __init__(self, chorst: str, chred: str, isoshl):
+        self.lib = _lib.load()
+        sh = np.ascontiguousarray(isoshl, dtype=np.int32)
+        self._keep = sh
+        h = ctypes.c_void_p()
+        rc = self.lib.mjk_choinfo_read(ctypes.byref(h), os.fsencode(chorst), os.fsencode(chred), _ptr(sh))
+        self._h = h
+        if rc != 0:
+            msg = self.lib.mjk_choinfo_error(h).decode()
+            self.close()
+            raise MjkError(rc, msg)
+        nb, mr, nsh, av = (ctypes.c_int32() for _ in range(4))
+        nc, thr = ctypes.c_int64(), ctypes.c_double()
+        self.lib.mjk_choinfo_dims(h, ctypes.byref(nb), ctypes.byref(nc), ctypes.byref(mr), ctypes.byref(nsh),
+                                  ctypes.byref(av), ctypes.byref(thr))
+        self.nbas, self.numcho, self.maxred = nb.value, nc.value, mr.value
+        self.nshell, self.adrvec, self.thrcom = nsh.value, av.value, thr.value
+
+    def reduced_set(self, jred: int) -> np.ndarray:
+        """0-based LT index of each element of reduced set jred (0-based)."""
+        nrs = ctypes.c_int64()
+        if self.lib.mjk_choinfo_reduced_set(self._h, int(jred), ctypes.byref(nrs), None) != 0:
+            raise MjkError(1, f"no reduced set {jred}")
+        m = np.zeros(nrs.value, dtype=np.int64)
+        self.lib.mjk_choinfo_reduced_set(self._h, int(jred), ctypes.byref(nrs), _ptr(m))
+        return m
+
+    def infvec(self, col: int) -> np.ndarray:
+        v = np.zeros(self.numcho, dtype=np.int64)
+        if self.numcho and self.lib.mjk_choinfo_infvec(self._h, int(col), _ptr(v)) != 0:
+            raise MjkError(1, f"bad InfVec column {col}")
+        return v
+
+    def load(self, jk: "JKContext", chvec: str, chodiag: str = None, split_bytes: int = 0) -> None:
+        jk._check(self.lib.mjk_load_choinfo(jk._h, self._h, os.fsencode(chvec),
+                                            os.fsencode(chodiag) if chodiag else None, int(split_bytes)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.mjk_choinfo_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
