@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the Cholesky J/K Fock build on one MI355X.
 
-Workload (one "step" = one J+K build): synthetic single FP64 J/K build,
-nbas=512, nchol=2048, nocc=64, D = C C^T -- Cholesky vectors resident in HBM
-before the timed region, as in an SCF iteration.  A step runs
-    J = two streaming passes over L      (HBM-bound)
-    K = half-transform + SYRK on MFMA     (MFMA-bound)
-    F = h + J - 1/2 K                     (FP64 assembly)
-through the C ABI.  Prints ONE JSON line.
+Default workload (the largest single-GPU configuration): 64 independent
+systems, nbas ~ U[100,600], nchol = round(k nbas) with k ~ U[4,6], nocc =
+10-20% of nbas (seeded), Cholesky vectors resident in HBM.  One "step" is one
+replay of the captured HIP graph that runs, for every system, the closed-shell
+energy build (density, J, K, Fock, E) and the hand-written backward (dE/dC,
+dE/dL).  value = systems built (forward + backward) per second in FP64.  The
+same line reports the forward-only replay, the precision ladder (FP64 / FP32 /
+split-BF16 split points tuned per system for |dE| <= 1e-6 Eh) with its rate
+and errors, the roofline of the dominant kernel from profiled eager steps of
+the same batch, the CPU oracle on a sample of the same workload, and, as
+secondary objects, the single n=512 build and the n=1200 build.
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -27,8 +32,12 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="batch64", choices=["batch64", "single512"],
+                   help="headline workload: the 64-system batch (default) or the single n=512 build")
+    p.add_argument("--no-secondary", action="store_true",
+                   help="skip the secondary n=512 and n=1200 objects")
     p.add_argument("--nbas", type=int, default=512)
     p.add_argument("--nchol", type=int, default=2048)
     p.add_argument("--nocc", type=int, default=64)
@@ -39,8 +48,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--extra", action="store_true",
-                   help="also time the other workloads (n=1200 build + backward, 9-density batch, "
-                        "64-system graph batch); adds an 'extra' object")
+                   help="also time the 9-density state-averaged batch; adds an 'extra' object")
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed region, copy 1 GiB device to device (calibrates HBM counters)")
     p.add_argument("--cpu-sample-vectors", type=int, default=1 << 30)
@@ -211,26 +219,22 @@ def _timeit(fn, reps):
     return (time.perf_counter() - t0) / reps
 
 
-def extra_workloads(dev):
-    """Secondary measurements: the n=1200/nchol=6000/nocc=150 build with its
-    backward, the 9-density state-averaged batch (n=800, nchol=3600) and the
-    64-system uneven batch as one graph replay.  FP64 and mixed."""
-    import numpy as np
+def build1200(dev):
+    """Secondary: the n=1200/nchol=6000/nocc=150 build (P = 2CC^T) with its
+    backward, FP64, FP32, split-BF16 and the tuned ladder (|dE| <= 1e-6 Eh)."""
     import torch
 
-    from openmolcas_amd.batch import MoleculeBatch, uneven_shapes
+    from openmolcas_amd import ladder
     from openmolcas_amd.fock import JKContext
     from openmolcas_amd.synthetic import chol_vectors_torch, npair, orbitals_torch
 
-    out = {}
-    # -- n=1200, nchol=6000, nocc=150: forward J+K and backward
     n, M, o = 1200, 6000, 150
     jk = JKContext(n, M, device=dev)
     L = jk.device_vectors()
     chol_vectors_torch(n, M, 3, device=dev, out=L)
     C = orbitals_torch(n, o, 4, device=dev)
     dlt = jk.density(C, 2.0)
-    res = {"workload": "nbas=1200 nchol=6000 nocc=150, P = 2CC^T"}
+    res = {"workload": "nbas=1200 nchol=6000 nocc=150, P = 2CC^T (secondary)"}
     J, V = jk.coulomb(dlt, want_v=True)
     K64 = jk.exchange([C])
     G = torch.zeros_like(J)
@@ -238,46 +242,47 @@ def extra_workloads(dev):
     h = torch.zeros_like(J)
     e64 = jk.energy(h, dlt, G).item()
     res["coulomb_ms"] = round(1e3 * _timeit(lambda: jk.coulomb(dlt), 3), 3)
-    bytes_j = 2.0 * M * npair(n) * 8
-    res["coulomb_GBs"] = round(bytes_j / (res["coulomb_ms"] * 1e-3) / 1e9, 1)
+    res["coulomb_GBs"] = round(2.0 * M * npair(n) * 8 / (res["coulomb_ms"] * 1e-3) / 1e9, 1)
+    res["builds_per_s_fp64"] = round(1.0 / _timeit(lambda: jk.coulomb_exchange(dlt, [C]), 3), 3)
     fk = M * o * (2.0 * n * n + n * (n + 1))
+    dC64 = jk.grad_orbitals(C, G, 4.0)
     for prec in ("fp64", "fp32", "bf16x3"):
         t = _timeit(lambda: jk.exchange([C], precision=prec), 2)
         K = jk.exchange([C], precision=prec)
         Gp = torch.zeros_like(J)
         jk.fock_update(Gp, J, [0], [1.0], K, [-1.0])
-        e = jk.energy(h, dlt, Gp).item()
         res[f"exchange_{prec}_ms"] = round(1e3 * t, 3)
         res[f"exchange_{prec}_TFs"] = round(fk / t / 1e12, 2)
-        res[f"dE_{prec}"] = abs(e - e64)
+        res[f"dE_{prec}"] = abs(jk.energy(h, dlt, Gp).item() - e64)
         if prec != "fp64":
-            F = Gp
-            dC_mixed = jk.grad_orbitals(C, F, 4.0)
-            dC64 = jk.grad_orbitals(C, G, 4.0)
-            res[f"dEdC_maxabs_err_{prec}"] = float(torch.max(torch.abs(dC_mixed - dC64)).item())
-    # precision ladder planned from the measured full-mode errors (|dE| <= 1e-6 Eh)
-    from openmolcas_amd import ladder
+            res[f"dEdC_maxabs_err_{prec}"] = float(torch.max(torch.abs(jk.grad_orbitals(C, Gp, 4.0) - dC64)).item())
     plan = ladder.tune(jk, [C], dlt, factor=0.5, tol=1e-6)  # G = J - K[C]: dE = 1/2 dK.D
     t = _timeit(lambda: jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16), 2)
     K = jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
     Gp = torch.zeros_like(J)
     jk.fock_update(Gp, J, [0], [1.0], K, [-1.0])
-    dC64 = jk.grad_orbitals(C, G, 4.0)
-    dC_ladder = jk.grad_orbitals(C, Gp, 4.0)
     res["ladder"] = {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "exchange_ms": round(1e3 * t, 3),
                      "dE": abs(jk.energy(h, dlt, Gp).item() - e64),
-                     "dEdC_maxabs_err": float(torch.max(torch.abs(dC_ladder - dC64)).item()),
+                     "dEdC_maxabs_err": float(torch.max(torch.abs(jk.grad_orbitals(C, Gp, 4.0) - dC64)).item()),
                      "speedup_vs_fp64": round(res["exchange_fp64_ms"] / (1e3 * t), 3)}
     dl = torch.empty((M, npair(n)), dtype=torch.float64, device=dev)
     res["backward_dC_ms"] = round(1e3 * _timeit(lambda: jk.grad_orbitals(C, G, 4.0), 3), 3)
     res["backward_dL_ms"] = round(1e3 * _timeit(
         lambda: jk.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=dl), 2), 3)
-    out["nbas1200"] = res
     del dl, L
     jk.close()
     torch.cuda.empty_cache()
+    return res
 
-    # -- state-averaged batch: n=800, nchol=3600, 1 x rank 120 + 8 x U{4..16}
+
+def state_averaged(dev):
+    """Extra: the 9-density state-averaged batch (n=800, nchol=3600)."""
+    import numpy as np
+    import torch
+
+    from openmolcas_amd.fock import JKContext
+    from openmolcas_amd.synthetic import chol_vectors_torch, npair, orbitals_torch
+
     n, M = 800, 3600
     rng = np.random.default_rng(11)
     ranks = [120] + [int(r) for r in rng.integers(4, 17, size=8)]
@@ -306,44 +311,177 @@ def extra_workloads(dev):
         res[f"exchange_{prec}_ms"] = round(1e3 * t, 3)
         res[f"exchange_{prec}_TFs"] = round(fk / t / 1e12, 2)
     res["builds_per_s_fp64"] = round(1.0 / (tj + res["exchange_fp64_ms"] * 1e-3), 3)
-    out["state_averaged_9"] = res
     del L
     jk.close()
     torch.cuda.empty_cache()
+    return res
 
-    # -- 64 uneven systems, forward + backward, one graph replay
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def batch_cpu_baseline(shapes, seed, frac=32):
+    """The CPU oracle (C restatement of CHO_FSCF) on the batch workload: every
+    system's forward J/K build with 1/frac of its vectors (at least 8), on
+    all host cores, then the first 16 systems on one core; both scaled to the
+    full vector counts (the build is linear in the vector count).  The
+    reference has no backward, so the CPU rate counts forward builds."""
+    import numpy as np
+
+    import oracle
+    from openmolcas_amd.synthetic import chol_vectors_np, fold, npair, orbitals_np
+
+    def run(systems):
+        tot = 0.0
+        for i, (n, M, o) in systems:
+            ms = max(8, M // frac)
+            L = chol_vectors_np(n, ms, seed + 7 * i)
+            C = orbitals_np(n, o, seed + 7 * i + 1)
+            dlt = fold(2.0 * C @ C.T)
+            st = oracle.CholStore.full(L)
+            flt = np.zeros((1, npair(n)))
+            t0 = time.perf_counter()
+            oracle.cho_fscf(st, flt, [o], [np.sqrt(2.0) * C], dlt, 0.5, batch=256)
+            tot += (time.perf_counter() - t0) * M / ms
+        return tot
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    oracle.set_threads(threads)
+    t_all = run(list(enumerate(shapes)))
+    oracle.set_threads(1)
+    sub = list(enumerate(shapes))[:16]
+    t_one = run(sub) * sum(M * o * n * n for n, M, o in shapes) / sum(M * o * n * n for _, (n, M, o) in sub)
+    oracle.set_threads(threads)
+    return {
+        "value": round(len(shapes) / t_all, 4),
+        "unit": "builds/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle CHO_FSCF restatement (C, OpenMP) on every one of the {len(shapes)} systems with "
+                  f"1/{frac} of its vectors (>= 8), forward J/K only (the reference has no backward), "
+                  f"scaled to the full vector counts: {t_all:.1f} s per batch on {threads} threads",
+        "single_core": {"value": round(len(shapes) / t_one, 5), "unit": "builds/s", "cores": 1,
+                        "sample": "the first 16 systems at 1/{frac} of their vectors, scaled by the work "
+                                  "M o n^2 of all 64".replace("{frac}", str(frac))},
+        "cpu_model": cpu_model(),
+    }
+
+
+def batch_workload(args, dev):
+    """The 64-system uneven batch: FP64 forward + backward graph replays (the
+    timed region), the forward-only replay, the tuned ladder, and a profiled
+    eager step for the per-kernel roofline."""
+    import torch
+
+    from openmolcas_amd.batch import MoleculeBatch, uneven_shapes
+
     shapes = uneven_shapes(64, seed=5)
     batch = MoleculeBatch(shapes, device=dev, seed=3)
     batch.autotune("fp64")
-    res = {"workload": "64 systems nbas U[100,600], nchol U[4,6] nbas, nocc U[0.1,0.2] nbas; "
-                       "forward + backward per graph replay"}
-    for prec in ("fp64", "fp32"):
-        batch.capture(prec)
-        t = _timeit(batch.replay, 3)
-        res[f"graph_replay_{prec}_ms"] = round(1e3 * t, 2)
-        res[f"E_{prec}"] = batch.E.clone()
-    res["fwd_exchange_TFs_fp64"] = round(batch.flops() / (res["graph_replay_fp64_ms"] * 1e-3) / 1e12, 2)
-    res["maxabs_dE_fp32"] = float(torch.max(torch.abs(res.pop("E_fp32") - res.pop("E_fp64"))).item())
-    res["speedup_fp32"] = round(res["graph_replay_fp64_ms"] / res["graph_replay_fp32_ms"], 3)
-    out["molecule_batch_64"] = res
+    nsys = len(shapes)
+
+    def timed(precision, backward, steps, warmup):
+        batch.capture(precision, backward)
+        for _ in range(warmup):
+            batch.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.replay()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t64 = timed("fp64", True, args.steps, args.warmup)
+    E64 = batch.E.clone()
+    dC64 = [d.clone() for d in batch.dC]
+    t_fwd = timed("fp64", False, args.steps, 1)
+    fk = batch.flops()
+    # profiled eager step: per-phase kernel time summed over the 64 systems
+    batch.set_profiling(True)
+    batch.step("fp64")
+    torch.cuda.synchronize()
+    phases = batch.phase_times()
+    batch.set_profiling(False)
+    # algorithmic work per phase of one step
+    ht_f = sum(2.0 * M * o * n * n for n, M, o in shapes)                # forward half-transform
+    sy_f = sum(1.0 * M * o * n * (n + 1) for n, M, o in shapes)          # SYRK, lower triangle
+    bw_f = sum(M * (2.0 * n * o * o + 2.0 * n * o * o + 1.0 * n * (n + 1) * o) for n, M, o in shapes)
+    # the half-transform phase holds the forward pass and the backward's recompute of X
+    ph = {
+        "half_transform": ("k_ht<F64> (X = C^T L: forward with the Coulomb V, and the backward's recompute)",
+                           2.0 * ht_f, phases["half_transform"]),
+        "exchange": ("k_syrk<F64> (K = sum X^T X)", sy_f, phases["exchange"]),
+        "backward": ("k_gemm (dE/dL: Z = XC, W = CZ, W C^T packed)", bw_f, phases["backward"]),
+    }
+    name, (label, flops, (ms, calls)) = max(ph.items(), key=lambda kv: kv[1][2][0])
+    achieved = flops / (ms * 1e-3) / 1e12
+    # ladder: per-system split points for |dE| <= 1e-6 Eh, timed the same way
+    plans = batch.tune_ladder(tol=1e-6)
+    t_lad = timed("ladder", True, args.steps, 1)
+    Elad = batch.E.clone()
+    dClad = [d.clone() for d in batch.dC]
+    dE = torch.abs(Elad - E64)
+    dCerr = max(torch.max(torch.abs(a - b)).item() for a, b in zip(dClad, dC64))
+    m_tot = sum(M for _, M, _ in shapes)
+    out = {
+        "value": round(nsys / t64, 4),
+        "ms_per_step": round(1e3 * t64, 3),
+        "config": {
+            "workload": "64 uneven systems (nbas U[100,600], nchol U[4,6] nbas, nocc U[0.1,0.2] nbas, "
+                        "seed 5), forward J+K+E and backward dE/dC, dE/dL per system, one HIP graph replay",
+            "systems": nsys, "nbas_total": sum(n for n, _, _ in shapes), "nchol_total": m_tot,
+            "precision": "fp64", "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "mfma", "kernel": label, "phase": name,
+            "achieved": round(achieved, 3), "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_TFLOPS["fp64"], 4),
+            "kernel_ms_per_step": round(ms, 3), "launches_per_step": calls,
+            "timing": "HIP events around every launch of one eager step of the same batch (the graph "
+                      "replays the same launches)",
+            "traffic": None,
+        },
+        "batch": {
+            "fwd_bwd_replay_ms_fp64": round(1e3 * t64, 3),
+            "fwd_replay_ms_fp64": round(1e3 * t_fwd, 3),
+            "fwd_exchange_TFs_over_fwd_replay": round(fk / t_fwd / 1e12, 2),
+            "fwd_exchange_gflop": round(fk / 1e9, 1),
+            "phase_ms_per_step_eager": {k: round(v[0], 3) for k, v in phases.items()},
+            "ladder": {
+                "fwd_bwd_replay_ms": round(1e3 * t_lad, 3),
+                "builds_per_s": round(nsys / t_lad, 4),
+                "speedup_vs_fp64": round(t64 / t_lad, 3),
+                "max_abs_dE_Eh": float(torch.max(dE).item()),
+                "max_abs_dEdC_err": dCerr,
+                "fp64_vector_fraction": round(sum(p.j_fp32 for p in plans) / m_tot, 4),
+                "fp32_vector_fraction": round(sum(p.j_bf16 - p.j_fp32 for p in plans) / m_tot, 4),
+                "tol_Eh": 1e-6,
+            },
+        },
+    }
     batch.close()
-    return out
+    del batch
+    torch.cuda.empty_cache()
+    return out, shapes
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); the J/K build would "
-              "shard over Cholesky vectors with one all-reduce of the LT Fock matrix, which this "
-              "library does not implement.", file=sys.stderr)
-        sys.exit(2)
+def single_build(args, dev, steps):
+    """The synthetic single J/K build (nbas=512, nchol=2048, nocc=64 by
+    default): FP64 fused build timed over `steps`, per-kernel phase times, the
+    mixed modes and the ladder, the CPU oracle on the same build and the
+    drop-in's parity against it."""
     import torch
 
     from openmolcas_amd.fock import JKContext
     from openmolcas_amd.synthetic import chol_vectors_torch, npair, orbitals_torch
 
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     n, M, o = args.nbas, args.nchol, args.nocc
     npr = npair(n)
 
@@ -384,14 +522,14 @@ def main():
     ctx.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     phases = ctx.phase_times()
     ctx.set_profiling(False)
     elapsed = t1 - t0
-    ms_step = 1000.0 * elapsed / args.steps
+    ms_step = 1000.0 * elapsed / steps
 
     # algorithmic work (per build)
     # the Coulomb passes of their own: V and J (two reads of L), or J alone when
@@ -411,12 +549,12 @@ def main():
     if not jv_calls:
         jv_ms = 0.0
     jj_ms, _ = avg("coulomb_j")
-    ht_per_launch = flops_ht / (ht_calls / args.steps)
-    sy_per_launch = flops_syrk / (sy_calls / args.steps)
+    ht_per_launch = flops_ht / (ht_calls / steps)
+    sy_per_launch = flops_syrk / (sy_calls / steps)
     ht_tf = ht_per_launch / (ht_ms * 1e-3) / 1e12
     sy_tf = sy_per_launch / (sy_ms * 1e-3) / 1e12
     j_gbs = bytes_j / ((jv_ms + jj_ms) * 1e-3) / 1e9
-    k_ms = phases["half_transform"][0] / args.steps + phases["exchange"][0] / args.steps
+    k_ms = phases["half_transform"][0] / steps + phases["exchange"][0] / steps
     k_tf = (flops_ht + flops_syrk) / (k_ms * 1e-3) / 1e12
     # dominant kernel: the larger of the two exchange kernels by time
     ptag = {"fp64": "F64", "fp32": "F32", "bf16x3": "BF16"}[args.precision]
@@ -430,18 +568,8 @@ def main():
               ("copy_TBs", 4))}
 
     out = {
-        "metric": "J+K builds/sec (HBM GB/s on J, MFMA utilisation on K)",
-        "value": round(args.steps / elapsed, 4),
-        "unit": "builds/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "value": round(steps / elapsed, 4),
         "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else args.precision,
-        "data": "synthetic (seeded N(0,1) symmetric vectors, norms 1..1e-6; QR orbitals)",
         "config": {
             "workload": f"synthetic single J/K build nbas={n} nchol={M} nocc={o}",
             "nbas": n, "nchol": M, "nocc": o, "precision": args.precision,
@@ -468,9 +596,56 @@ def main():
             "syrk_ms": round(sy_ms, 4),
             "exchange_TFs": round(k_tf, 2),
             "exchange_frac_peak": round(k_tf / peak, 4),
-            "phase_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in phases.items()},
+            "phase_ms_per_step": {k: round(v[0] / steps, 4) for k, v in phases.items()},
         },
     }
+    if args.precision == "fp64" and not args.no_mixed:
+        out["mixed"] = mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_step, args)
+    ctx.close()
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"], ref = cpu_baseline(n, M, o, args.seed, args.cpu_sample_vectors)
+        out["parity"] = gpu_parity(ref, dev)
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); the J/K build would "
+              "shard over Cholesky vectors with one all-reduce of the LT Fock matrix, which this "
+              "library does not implement.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    line = {
+        "metric": "J+K builds/sec + HBM GB/s (J), MFMA util (K): mixed vs FP64 at |dE|<=1e-6 Eh",
+        "unit": "builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded N(0,1) symmetric vectors, norms 1..1e-6; QR orbitals; h = sym N(0,1)/nbas)",
+    }
+    if args.workload == "batch64":
+        head, shapes = batch_workload(args, dev)
+        line.update(head)
+        if not args.no_cpu_baseline:
+            line["cpu_baseline"] = batch_cpu_baseline(shapes, seed=3)
+        if not args.no_secondary:
+            single = single_build(args, dev, steps=20)
+            line["secondary"] = {"single_build_n512_m2048_o64": single, "build_n1200_m6000_o150": build1200(dev)}
+    else:
+        single = single_build(args, dev, steps=args.steps)
+        line["dtype"] = "f64" if args.precision == "fp64" else args.precision
+        line.update(single)
+    if args.extra:
+        line["extra"] = {"state_averaged_9": state_averaged(dev)}
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
         src.fill_(1.0)
@@ -479,18 +654,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.extra:
-        ctx.close()
-        torch.cuda.empty_cache()
-        out["extra"] = extra_workloads(dev)
-    if args.precision == "fp64" and not args.no_mixed and not args.extra:
-        out["mixed"] = mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_step, args)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"], ref = cpu_baseline(n, M, o, args.seed, args.cpu_sample_vectors)
-        out["parity"] = gpu_parity(ref, dev)
-    print(json.dumps(out), flush=True)
-    if not args.extra:
-        ctx.close()
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -59,7 +59,8 @@ enum {
   MJK_PHASE_REDUCE = 4,    /* partial-sum reductions            */
   MJK_PHASE_ASSEMBLY = 5,  /* Fock update / energy               */
   MJK_PHASE_PACK = 6,      /* density / orbital staging          */
-  MJK_NUM_PHASES = 7
+  MJK_PHASE_BACKWARD = 7,  /* dE/dL batched GEMMs (Z, W, dL)     */
+  MJK_NUM_PHASES = 8
 };
 
 /* ---- lifecycle --------------------------------------------------------- */

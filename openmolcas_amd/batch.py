@@ -6,9 +6,12 @@ streams (largest first, each to the lane with the least work so far); the
 contexts of one lane share one set of device workspaces and that lane's stream
 (mjk_create_shared), so the lanes run concurrently and small systems fill the
 GPU together.  ``step()`` runs, for every system, the closed-shell energy build
-(density, Coulomb, exchange, Fock and energy steps) followed by the
-hand-written backward (dE/dC = 4FC and the packed dE/dL); in FP64 the Coulomb
-V pass runs inside the half-transform (mjk_coulomb_exchange).  ``capture()``
+(density, Coulomb, exchange, Fock and energy steps) followed, unless
+``backward=False``, by the hand-written backward (dE/dC = 4FC and the packed
+dE/dL); in FP64 the Coulomb V pass runs inside the half-transform
+(mjk_coulomb_exchange).  ``tune_ladder`` places each system's FP64 / FP32 /
+split-BF16 split points for an energy tolerance (openmolcas_amd.ladder.tune,
+verified per system) and precision "ladder" runs them.  ``capture()``
 records one step -- the fork to the lane streams and the join back included --
 into a torch CUDAGraph (a hipGraph on ROCm) after a warm-up step has sized
 every workspace, so a replay is a single graph launch with no allocation, no
@@ -76,7 +79,25 @@ class MoleculeBatch:
         for st in self.lane_streams:
             st.synchronize()
 
-    def _system(self, i, precision):
+    def tune_ladder(self, tol: float = 1e-6, accept: float = 0.5):
+        """Per system, the precision ladder whose measured |dE| against the
+        FP64 build is at most accept * tol (E = h.D + 1/2 (J - K[C C^T]).D with
+        D = fold(2 C C^T), so dE = 1/2 dK.D).  Sets each context's split points
+        for precision "ladder"; returns the plans."""
+        from . import ladder
+
+        self.ladders = []
+        for i, (c, C) in enumerate(zip(self.ctx, self.C)):
+            with torch.cuda.stream(self.lane_streams[self.lane_of[i]]):
+                dlt = c.density(C, 2.0)
+                plan = ladder.tune(c, [C], dlt, factor=0.5, tol=tol, accept=accept)
+                c.set_ladder(plan.j_fp32, plan.j_bf16)
+                self.ladders.append(plan)
+        for st in self.lane_streams:
+            st.synchronize()
+        return self.ladders
+
+    def _system(self, i, precision, backward=True):
         c, C, h = self.ctx[i], self.C[i], self.h[i]
         dlt = c.density(C, 2.0)                      # fold(P), P = 2 C C^T
         if precision == "fp64":                      # V pass inside the half-transform
@@ -87,30 +108,32 @@ class MoleculeBatch:
         G = torch.zeros_like(J)
         c.fock_update(G, J, [0], [1.0], K, [-1.0])   # G = J - 1/2 K[P]
         self.E[i:i + 1].copy_(c.energy(h, dlt, G))
+        if not backward:
+            return
         F = h + G
         self.dC[i].copy_(c.grad_orbitals(C, F, 4.0))
         if self.dL[i] is not None:
             c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i])
 
-    def step(self, precision="fp64"):
-        """Forward + backward of every system: forks from the batch stream to
+    def step(self, precision="fp64", backward=True):
+        """Forward (+ backward) of every system: forks from the batch stream to
         the lane streams and joins back (all queued, nothing synchronised)."""
         with torch.cuda.stream(self.stream):
             for st in self.lane_streams:
                 st.wait_stream(self.stream)
             for i in range(len(self.shapes)):
                 with torch.cuda.stream(self.lane_streams[self.lane_of[i]]):
-                    self._system(i, precision)
+                    self._system(i, precision, backward)
             for st in self.lane_streams:
                 self.stream.wait_stream(st)
 
-    def capture(self, precision="fp64"):
+    def capture(self, precision="fp64", backward=True):
         """Warm-up step (sizes the shared workspaces), then one captured step."""
-        self.step(precision)
+        self.step(precision, backward)
         self.stream.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph, stream=self.stream):
-            self.step(precision)
+            self.step(precision, backward)
         self.stream.synchronize()
         return self.graph
 
@@ -120,6 +143,19 @@ class MoleculeBatch:
     def flops(self):
         """Algorithmic exchange flop of one forward pass: sum M o (2n^2 + n(n+1))."""
         return float(sum(M * o * (2 * n * n + n * (n + 1)) for n, M, o in self.shapes))
+
+    def set_profiling(self, on: bool):
+        for c in self.ctx:
+            c.set_profiling(on)
+
+    def phase_times(self):
+        """Per-phase milliseconds and launches summed over every system."""
+        tot = {}
+        for c in self.ctx:
+            for k, (ms, calls) in c.phase_times().items():
+                a, b = tot.get(k, (0.0, 0))
+                tot[k] = (a + ms, b + calls)
+        return tot
 
     def close(self):
         for c in reversed(self.ctx):

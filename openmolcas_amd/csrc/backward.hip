@@ -287,7 +287,7 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
     g.m = n; g.n = n; g.k = nocc; g.alpha = beta;
     g.G = dl + j0 * ldd; g.ldg = ldd; g.V = v + j0; g.Pl = P; g.gv = alpha;
     gemm<true, 1>(c, g, int(jn));
-    phase_end(c, MJK_PHASE_EXCHANGE);
+    phase_end(c, MJK_PHASE_BACKWARD);
   }
   return hip_check(c, hipGetLastError(), "grad_vectors launch");
 }

@@ -87,6 +87,8 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         L = ctypes.CDLL(_LIB_PATH)
+        L.ora_set_threads.argtypes = [ctypes.c_int32]
+        L.ora_get_threads.restype = ctypes.c_int32
         L.ora_itri.restype = ctypes.c_int64
         L.ora_itri.argtypes = [ctypes.c_int64, ctypes.c_int64]
         L.ora_fold.argtypes = [ctypes.c_int32, _dp, _dp]
@@ -139,6 +141,15 @@ def lib():
         L.ora_energy.argtypes = [ctypes.c_int64, _dp, _dp, _dp, ctypes.c_double]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the restatement (1: a single core)."""
+    lib().ora_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().ora_get_threads())
 
 
 def _p(a: np.ndarray, ct=_dp):

@@ -14,8 +14,14 @@
 #include "mjk_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Threads of the OpenMP loops (the threaded BLAS of the reference); the CPU
+ * baseline times the restatement on all host cores and on one. */
+void ora_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int32_t ora_get_threads(void) { return (int32_t)omp_get_max_threads(); }
 
 /* src/misc_util/index_functions.F90:138-146 (1-based, symmetric in i,j). */
 int64_t ora_itri(int64_t i, int64_t j) {

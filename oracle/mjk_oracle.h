@@ -40,6 +40,9 @@ typedef struct {
   const double *lvec;    /* concatenated vectors, nrs[vred[J]] values each */
 } ora_chol;
 
+void ora_set_threads(int32_t n);
+int32_t ora_get_threads(void);
+
 int64_t ora_itri(int64_t i, int64_t j); /* 1-based, either order */
 void ora_fold(int32_t n, const double *a_sq, double *b_lt);
 void ora_unfold(int32_t n, const double *a_lt, double *b_sq);
