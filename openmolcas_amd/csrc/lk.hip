@@ -540,6 +540,7 @@ int lk_exchange(Context *c, int32_t nd, const int32_t *nocc, const double *const
   K.last_batches = 0;
   if (ktot == 0 || runs.empty()) return MJK_OK;
   const int64_t kt = std::max(ktot, 1);
+  const size_t flag_bytes = size_t(round_up(kt * ns, 4));  // fill_zero works in 4-byte words
   if ((rc = ensure(c, K.diag, sizeof(double) * size_t(np))) ||
       (rc = ensure(c, K.dh, sizeof(double) * size_t(n) * n)) ||
       (rc = ensure(c, K.ylk, sizeof(double) * size_t(kt) * n)) ||
@@ -547,7 +548,7 @@ int lk_exchange(Context *c, int32_t nd, const int32_t *nocc, const double *const
       (rc = ensure(c, K.sumac, sizeof(double) * size_t(kt) * ns)) ||
       (rc = ensure(c, K.mlk, sizeof(double) * size_t(kt) * ns)) ||
       (rc = ensure(c, K.faa, sizeof(double) * size_t(kt) * ns)) ||
-      (rc = ensure(c, K.listed, size_t(kt) * ns)) || (rc = ensure(c, K.keep, size_t(kt) * ns)) ||
+      (rc = ensure(c, K.listed, flag_bytes)) || (rc = ensure(c, K.keep, flag_bytes)) ||
       (rc = ensure(c, K.x, sizeof(double) * size_t(kt) * kLkBatch * n_pad)) ||
       (rc = ensure(c, K.stats, 3 * sizeof(unsigned long long))))
     return rc;
@@ -556,7 +557,7 @@ int lk_exchange(Context *c, int32_t nd, const int32_t *nocc, const double *const
                                         hipMemcpyDeviceToDevice, c->stream), "LK diagonal")) ||
       (rc = fill_zero(c, K.stats.ptr, 3 * sizeof(unsigned long long))) ||
       (rc = fill_zero(c, K.faa.ptr, sizeof(double) * size_t(kt) * ns)) ||
-      (rc = fill_zero(c, K.listed.ptr, size_t(kt) * ns)) ||
+      (rc = fill_zero(c, K.listed.ptr, flag_bytes)) ||
       (rc = fill_zero(c, K.mlk.ptr, sizeof(double) * size_t(kt) * ns)))
     return rc;
   const int32_t *shell = static_cast<const int32_t *>(K.shell.ptr);

@@ -184,7 +184,7 @@ class JKContext:
         updated in place (FLT(d)%A0).  porb: list of nden arrays (nbas,
         nforb[d]+niorb[d]) = Porb(d)%SB(1)%A2(nOcc,nBas) memory.  dlt: folded
         LT density (only DLT(1) enters Coulomb, as in the reference)."""
-        nd = max(1, min(int(nden), 2))
+        nd = int(nden) if nden in (1, 2) else 1  # an invalid nDen is rejected by the library
         fl = _rows_inout(flt, nd, self.npair, "flt")
         nf = np.ascontiguousarray(nforb, dtype=np.int32)
         ni = np.ascontiguousarray(niorb, dtype=np.int32)
@@ -279,7 +279,7 @@ class JKContext:
         storage of the shells nbassh.  porb: list of nden (nbas, >=
         nforb[d]+niorb[d]) AO x orbital matrices C[b, k] (the MSQ columns).
         plt: folded total density."""
-        nd = max(1, min(int(nden), 2))
+        nd = int(nden) if nden in (1, 2) else 1  # an invalid nDen is rejected by the library
         fl = _rows_inout(flt, nd, self.npair, "flt")
         kl = _rows_inout(klt, nd, self.npair, "klt") if klt is not None else None
         nf = np.ascontiguousarray(nforb, dtype=np.int32)
