@@ -337,6 +337,20 @@ int mjk_exchange(mjk_context *ctx, int32_t nk, const int32_t *nocc,
                  const double *const *corb, double *klt, int64_t ldk,
                  int32_t precision);
 
+/* mjk_coulomb without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it).  No counterpart in the reference.  Same kernels
+ * and results as mjk_coulomb; every scratch buffer (the padded density, the
+ * partial sums of V and J, and V when v_out is NULL) lives in one fixed
+ * 64 MiB device array per GPU, set up with the library and held until the
+ * process ends.  Builds that need more scratch than 64 MiB return
+ * MJK_ERR_ARG (the scratch grows with nd, nchol and nbas(nbas+1)/2: about
+ * nd * nbas(nbas+1)/2 * 8 bytes times the number of vector slices of the J
+ * pass).  Calls that overlap on one GPU share the array and race; calls on
+ * one context are serialised by its stream. */
+int mjk_coulomb_noalloc(mjk_context *ctx, int32_t nd, const double *dlt, int64_t ldd,
+                        double *jlt, int64_t ldj, double *v_out);
+
 /* Coulomb of one folded density and FP64 exchange of nk orbital sets in one
  * pass over the vectors: jlt as mjk_coulomb (nd = 1), klt as mjk_exchange in
  * FP64.  The V_J = sum_p L^J_p D_p pass runs inside the half-transform, which

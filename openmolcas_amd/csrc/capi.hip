@@ -555,6 +555,13 @@ int mjk_coulomb(mjk_context *c, int32_t nd, const double *dlt, int64_t ldd, doub
   return coulomb(c, nd, dlt, ldd, jlt, ldj, v_out);
 }
 
+int mjk_coulomb_noalloc(mjk_context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
+                        int64_t ldj, double *v_out) {
+  if (!c) return MJK_ERR_ARG;
+  if (!dlt || !jlt) return set_error(c, MJK_ERR_ARG, "coulomb_noalloc: null pointer");
+  return coulomb_noalloc(c, nd, dlt, ldd, jlt, ldj, v_out);
+}
+
 int mjk_exchange(mjk_context *c, int32_t nk, const int32_t *nocc, const double *const *corb,
                  double *klt, int64_t ldk, int32_t precision) {
   if (!c) return MJK_ERR_ARG;

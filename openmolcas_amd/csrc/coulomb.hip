@@ -7,6 +7,9 @@
 // Both passes stream L once; they are HBM-bound.  L is stored vector-major
 // (row J holds the packed LT of L^J, stride ldl, a multiple of 512 doubles),
 // so each pass reads whole 4 KiB row segments per workgroup iteration.
+#include <mutex>
+#include <string>
+
 #include "mjk_internal.h"
 
 namespace mjk {
@@ -376,119 +379,230 @@ int vector_norms(Context *c, double *w) {
   return hip_check(c, hipGetLastError(), "vector_norms launch");
 }
 
-// Pass 2 for one or two densities (ndt = 1, 2) or their padded counts:
-// J = L V (about 1024 workgroups of two segments, 2048 of one), then the
-// fixed-order reduction of the vector slices into jlt.
-static int j_pass(Context *c, int nd, int ndt, const double *V, double *jlt, int64_t ldj) {
-  const int64_t M = c->nchol, ldl = c->ldl, nseg = ldl / kSeg;
+namespace {
+
+// Work split and scratch of one Coulomb build (both passes).
+struct CoulombPlan {
+  int nd = 0, ndt = 0;
+  bool tile = false;                   // nd >= 3: the MFMA passes
+  int nsplit1 = 0, nsplit2 = 0;
+  int64_t nblk = 0, pchunk = 0, jchunk = 0;
+  size_t bytes_d = 0, bytes_vpart = 0, bytes_v = 0, bytes_jpart = 0;
+};
+
+constexpr int kTileVM = 64, kTileVK = 64, kTileU = 4;  // MFMA passes (nd >= 3)
+
+inline size_t piece(size_t bytes) { return size_t(round_up(int64_t(bytes), 256)); }
+
+// Pass 2 split of the streaming J pass for ndt densities: about 1024
+// workgroups of two segments, 2048 of one.
+void plan_j_stream(const Context *c, int nd, int ndt, CoulombPlan *p) {
+  const int64_t M = c->nchol, nseg = c->ldl / kSeg;
   const int64_t nxj = ceil_div(nseg, j_width(ndt));
   int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048 / j_width(ndt), nxj), 1), M));
-  const int64_t jchunk = ceil_div(M, nsplit2);
-  nsplit2 = int(ceil_div(M, jchunk));
+  p->jchunk = ceil_div(M, nsplit2);
+  p->nsplit2 = int(ceil_div(M, p->jchunk));
+  p->bytes_jpart = sizeof(double) * size_t(p->nsplit2) * nd * c->ldl;
+}
+
+CoulombPlan plan_coulomb(const Context *c, int nd) {
+  CoulombPlan p;
+  const int64_t M = c->nchol, ldl = c->ldl;
+  p.nd = nd;
+  p.bytes_d = sizeof(double) * size_t(nd) * ldl;
+  p.bytes_v = sizeof(double) * size_t(M) * nd;
+  if (nd >= 3) {
+    p.tile = true;
+    p.nblk = ceil_div(M, kTileVM);
+    const int64_t nk = ldl / kTileVK;
+    int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, p.nblk), 1), nk));
+    p.pchunk = ceil_div(nk, nsplit1) * kTileVK;  // whole chunks: ldl % VK == 0
+    p.nsplit1 = int(ceil_div(ldl, p.pchunk));
+    const int64_t npc = ldl / 512;
+    int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, npc), 1),
+                                        ceil_div(M, 4 * kTileU)));
+    p.jchunk = round_up(ceil_div(M, nsplit2), 4 * kTileU);
+    p.nsplit2 = int(ceil_div(M, p.jchunk));
+    p.bytes_jpart = sizeof(double) * size_t(p.nsplit2) * nd * ldl;
+  } else {
+    p.ndt = nd <= 1 ? 1 : 2;
+    const int jb = 8;
+    p.nblk = ceil_div(M, jb);
+    const int64_t nseg = ldl / kSeg;
+    int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, p.nblk), 1), nseg));
+    p.pchunk = ceil_div(nseg, nsplit1) * kSeg;
+    p.nsplit1 = int(ceil_div(ldl, p.pchunk));
+    plan_j_stream(c, nd, p.ndt, &p);
+  }
+  p.bytes_vpart = sizeof(double) * size_t(p.nsplit1) * M * nd;
+  return p;
+}
+
+// Where the passes put their scratch: the context's grow-only workspaces, or
+// (arena set) consecutive 256-byte pieces of a fixed device buffer that is
+// neither allocated nor freed here.
+struct Scratch {
+  double *arena = nullptr;
+  size_t cap = 0, used = 0;  // bytes
+};
+
+int take(Context *c, Scratch &s, DevBuf &ws, size_t bytes, double **out) {
+  if (!s.arena) {
+    const int rc = ensure(c, ws, bytes);
+    if (rc) return rc;
+    *out = static_cast<double *>(ws.ptr);
+    return MJK_OK;
+  }
+  if (s.used + piece(bytes) > s.cap)
+    return set_error(c, MJK_ERR_ARG, "coulomb: scratch exceeds the fixed buffer");
+  *out = s.arena + s.used / sizeof(double);
+  s.used += piece(bytes);
+  return MJK_OK;
+}
+
+// Pass 2 for one or two densities (ndt = 1, 2) or their padded counts:
+// J = L V, then the fixed-order reduction of the vector slices into jlt.
+int j_pass(Context *c, Scratch &s, const CoulombPlan &p, int nd, const double *V, double *jlt,
+           int64_t ldj) {
   int rc;
-  if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
-  double *jpart = static_cast<double *>(c->w->jpart.ptr);
+  double *jpart;
+  if ((rc = take(c, s, c->w->jpart, p.bytes_jpart, &jpart))) return rc;
   phase_begin(c);
-  switch (ndt) {
-    case 1: launch_j<1>(c, V, nd, jpart, nsplit2, jchunk); break;
-    case 2: launch_j<2>(c, V, nd, jpart, nsplit2, jchunk); break;
-    case 4: launch_j<4>(c, V, nd, jpart, nsplit2, jchunk); break;
-    case 8: launch_j<8>(c, V, nd, jpart, nsplit2, jchunk); break;
-    default: launch_j<16>(c, V, nd, jpart, nsplit2, jchunk); break;
+  switch (p.ndt) {
+    case 1: launch_j<1>(c, V, nd, jpart, p.nsplit2, p.jchunk); break;
+    case 2: launch_j<2>(c, V, nd, jpart, p.nsplit2, p.jchunk); break;
+    case 4: launch_j<4>(c, V, nd, jpart, p.nsplit2, p.jchunk); break;
+    case 8: launch_j<8>(c, V, nd, jpart, p.nsplit2, p.jchunk); break;
+    default: launch_j<16>(c, V, nd, jpart, p.nsplit2, p.jchunk); break;
   }
   phase_end(c, MJK_PHASE_COULOMB_J);
   phase_begin(c);
   hipLaunchKernelGGL(k_reduce_j, dim3(unsigned(ceil_div(c->np, 256)), unsigned(nd)), dim3(256), 0,
-                     c->stream, jpart, ldl, c->np, nd, nsplit2, jlt, ldj);
+                     c->stream, jpart, c->ldl, c->np, nd, p.nsplit2, jlt, ldj);
   phase_end(c, MJK_PHASE_REDUCE);
   return hip_check(c, hipGetLastError(), "coulomb launch");
 }
 
-int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
-            double *v_out) {
+int coulomb_run(Context *c, Scratch &s, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
+                int64_t ldj, double *v_out) {
   if (nd < 1 || nd > 16) return set_error(c, MJK_ERR_ARG, "coulomb: nd must be in [1,16]");
   if (ldd < c->np || ldj < c->np) return set_error(c, MJK_ERR_ARG, "coulomb: ld < nbas(nbas+1)/2");
   if (c->nloaded != c->nchol)
     return set_error(c, MJK_ERR_SHORT_READ, "coulomb: not all Cholesky vectors were uploaded");
   const int64_t M = c->nchol, ldl = c->ldl;
+  const CoulombPlan p = plan_coulomb(c, nd);
   int rc;
-
+  double *Dp, *vpart, *V;
   // staging copy of the densities (zero padded to ldl)
-  if ((rc = ensure(c, c->w->d, sizeof(double) * nd * ldl))) return rc;
-  double *Dp = static_cast<double *>(c->w->d.ptr);
+  if ((rc = take(c, s, c->w->d, p.bytes_d, &Dp)) || (rc = take(c, s, c->w->vpart, p.bytes_vpart, &vpart)))
+    return rc;
+  if (v_out) V = v_out;
+  else if ((rc = take(c, s, c->w->v, p.bytes_v, &V))) return rc;
   phase_begin(c);
   hipLaunchKernelGGL(k_pack_dens, dim3(unsigned(ceil_div(ldl, 256)), unsigned(nd)), dim3(256), 0,
                      c->stream, dlt, ldd, c->np, ldl, Dp);
   phase_end(c, MJK_PHASE_PACK);
 
-  const int64_t nseg = ldl / kSeg;
-  if (nd >= 3) {
+  if (p.tile) {
     // ---- pass 1: V = L D^T, split over pairs to fill the device
-    constexpr int VM = 64, VK = 64, U = 4;
-    const int64_t nblk = ceil_div(M, VM), nk = ldl / VK;
-    int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nblk), 1), nk));
-    const int64_t pchunk = ceil_div(nk, nsplit1) * VK;  // whole chunks: ldl % VK == 0
-    nsplit1 = int(ceil_div(ldl, pchunk));
-    if ((rc = ensure(c, c->w->vpart, sizeof(double) * nsplit1 * M * nd))) return rc;
-    if ((rc = ensure(c, c->w->v, sizeof(double) * M * nd))) return rc;
-    double *vpart = static_cast<double *>(c->w->vpart.ptr);
-    double *V = v_out ? v_out : static_cast<double *>(c->w->v.ptr);
     phase_begin(c);
-    hipLaunchKernelGGL((k_coulomb_v_tile<VM, VK>), dim3(unsigned(nblk), unsigned(nsplit1)),
-                       dim3(kThreads), 0, c->stream, c->L, ldl, M, Dp, nd, pchunk, vpart);
+    hipLaunchKernelGGL((k_coulomb_v_tile<kTileVM, kTileVK>), dim3(unsigned(p.nblk), unsigned(p.nsplit1)),
+                       dim3(kThreads), 0, c->stream, c->L, ldl, M, Dp, nd, p.pchunk, vpart);
     phase_end(c, MJK_PHASE_COULOMB_V);
     phase_begin(c);
     hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M * nd, 256))), dim3(256), 0, c->stream,
-                       vpart, M, nd, nsplit1, V);
+                       vpart, M, nd, p.nsplit1, V);
     phase_end(c, MJK_PHASE_REDUCE);
     // ---- pass 2: J = L^T V, split over vectors
-    const int64_t npc = ldl / 512;
-    int nsplit2 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, npc), 1), ceil_div(M, 4 * U)));
-    const int64_t jchunk = round_up(ceil_div(M, nsplit2), 4 * U);
-    nsplit2 = int(ceil_div(M, jchunk));
-    if ((rc = ensure(c, c->w->jpart, sizeof(double) * nsplit2 * nd * ldl))) return rc;
-    double *jpart = static_cast<double *>(c->w->jpart.ptr);
+    double *jpart;
+    if ((rc = take(c, s, c->w->jpart, p.bytes_jpart, &jpart))) return rc;
     phase_begin(c);
-    hipLaunchKernelGGL(k_coulomb_j_stream<U>, dim3(unsigned(npc), unsigned(nsplit2)), dim3(kThreads),
-                       0, c->stream, c->L, ldl, M, V, nd, jchunk, jpart);
+    hipLaunchKernelGGL(k_coulomb_j_stream<kTileU>, dim3(unsigned(ldl / 512), unsigned(p.nsplit2)),
+                       dim3(kThreads), 0, c->stream, c->L, ldl, M, V, nd, p.jchunk, jpart);
     phase_end(c, MJK_PHASE_COULOMB_J);
     phase_begin(c);
     hipLaunchKernelGGL(k_reduce_j, dim3(unsigned(ceil_div(c->np, 256)), unsigned(nd)), dim3(256), 0,
-                       c->stream, jpart, ldl, c->np, nd, nsplit2, jlt, ldj);
+                       c->stream, jpart, ldl, c->np, nd, p.nsplit2, jlt, ldj);
     phase_end(c, MJK_PHASE_REDUCE);
     return hip_check(c, hipGetLastError(), "coulomb launch");
   }
 
   // ---- pass 1: V = L^T D
-  const int ndt = nd <= 1 ? 1 : nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : 16;
-  const int jb = ndt <= 2 ? 8 : ndt == 4 ? 4 : ndt == 8 ? 2 : 1;
-  const int64_t nblk = ceil_div(M, jb);
-  int nsplit1 = int(std::min<int64_t>(std::max<int64_t>(ceil_div(2048, nblk), 1), nseg));
-  const int64_t pchunk = ceil_div(nseg, nsplit1) * kSeg;
-  nsplit1 = int(ceil_div(ldl, pchunk));
-  if ((rc = ensure(c, c->w->vpart, sizeof(double) * nsplit1 * M * nd))) return rc;
-  if ((rc = ensure(c, c->w->v, sizeof(double) * M * nd))) return rc;
-  double *vpart = static_cast<double *>(c->w->vpart.ptr);
-  double *V = v_out ? v_out : static_cast<double *>(c->w->v.ptr);
   phase_begin(c);
-  switch (ndt) {
-    case 1: launch_v<8, 1>(c, Dp, nd, vpart, nsplit1, pchunk); break;
-    case 2: launch_v<8, 2>(c, Dp, nd, vpart, nsplit1, pchunk); break;
-    case 4: launch_v<4, 4>(c, Dp, nd, vpart, nsplit1, pchunk); break;
-    case 8: launch_v<2, 8>(c, Dp, nd, vpart, nsplit1, pchunk); break;
-    default: launch_v<1, 16>(c, Dp, nd, vpart, nsplit1, pchunk); break;
+  switch (p.ndt) {
+    case 1: launch_v<8, 1>(c, Dp, nd, vpart, p.nsplit1, p.pchunk); break;
+    default: launch_v<8, 2>(c, Dp, nd, vpart, p.nsplit1, p.pchunk); break;
   }
   phase_end(c, MJK_PHASE_COULOMB_V);
   phase_begin(c);
   hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M * nd, 256))), dim3(256), 0, c->stream,
-                     vpart, M, nd, nsplit1, V);
+                     vpart, M, nd, p.nsplit1, V);
   phase_end(c, MJK_PHASE_REDUCE);
-  return j_pass(c, nd, ndt, V, jlt, ldj);
+  return j_pass(c, s, p, nd, V, jlt, ldj);
 }
 
-int coulomb_exchange(Context *c, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
-                     int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
-                     int64_t ldk, double *v_out) {
+// ---- the allocation-free Coulomb build ----------------------------------
+// Its scratch is one fixed device array per GPU, allocated with the code
+// object and held until the process ends.  Host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the array's name
+// (that is a host-side stand-in, not a device address).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+}  // namespace
+
+// Device address of this GPU's scratch array, looked up once per device (with
+// that device current) and cached.
+static int scratch_device_ptr(Context *c, double **out) {
+  static double *ptr[kMaxDevices] = {};
+  static std::mutex mu;
+  if (c->device < 0 || c->device >= kMaxDevices)
+    return set_error(c, MJK_ERR_ARG, "coulomb_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[c->device]) {
+    int cur = -1;
+    int rc;
+    if ((rc = hip_check(c, hipGetDevice(&cur), "hipGetDevice"))) return rc;
+    if (cur != c->device)
+      return set_error(c, MJK_ERR_STATE, "coulomb_noalloc: the context's device is not current");
+    void *p = nullptr;
+    if ((rc = hip_check(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")))
+      return rc;
+    ptr[c->device] = static_cast<double *>(p);
+  }
+  *out = ptr[c->device];
+  return MJK_OK;
+}
+
+int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+            double *v_out) {
+  Scratch s;  // the context's workspaces
+  return coulomb_run(c, s, nd, dlt, ldd, jlt, ldj, v_out);
+}
+
+int coulomb_noalloc(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+                    double *v_out) {
+  if (nd < 1 || nd > 16) return set_error(c, MJK_ERR_ARG, "coulomb: nd must be in [1,16]");
+  const CoulombPlan p = plan_coulomb(c, nd);
+  const size_t need = piece(p.bytes_d) + piece(p.bytes_vpart) + piece(p.bytes_jpart) +
+                      (v_out ? 0 : piece(p.bytes_v));
+  if (need > kScratchBytes)
+    return set_error(c, MJK_ERR_ARG,
+                     "coulomb_noalloc: this build needs " + std::to_string(need >> 20) +
+                         " MiB of scratch, more than the fixed " +
+                         std::to_string(kScratchBytes >> 20) + " MiB");
+  Scratch s;
+  int rc;
+  if ((rc = scratch_device_ptr(c, &s.arena))) return rc;
+  s.cap = kScratchBytes;
+  return coulomb_run(c, s, nd, dlt, ldd, jlt, ldj, v_out);
+}
+
+namespace {
+int coulomb_exchange_run(Context *c, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+                         int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
+                         int64_t ldk, double *v_out) {
   if (ldd < c->np || ldj < c->np) return set_error(c, MJK_ERR_ARG, "coulomb: ld < nbas(nbas+1)/2");
   if (c->nloaded != c->nchol)
     return set_error(c, MJK_ERR_SHORT_READ, "coulomb: not all Cholesky vectors were uploaded");
@@ -527,7 +641,19 @@ int coulomb_exchange(Context *c, const double *dlt, int64_t ldd, double *jlt, in
   hipLaunchKernelGGL(k_reduce_v, dim3(unsigned(ceil_div(M, 256))), dim3(256), 0, c->stream, vpart,
                      M, 1, ntile, V);
   phase_end(c, MJK_PHASE_REDUCE);
-  return j_pass(c, 1, 1, V, jlt, ldj);
+  CoulombPlan p;
+  p.ndt = 1;
+  plan_j_stream(c, 1, 1, &p);
+  Scratch s;  // the context's workspaces
+  return j_pass(c, s, p, 1, V, jlt, ldj);
+}
+
+}  // namespace
+
+int coulomb_exchange(Context *c, const double *dlt, int64_t ldd, double *jlt, int64_t ldj,
+                     int32_t nk, const int32_t *nocc, const double *const *corb, double *klt,
+                     int64_t ldk, double *v_out) {
+  return coulomb_exchange_run(c, dlt, ldd, jlt, ldj, nk, nocc, corb, klt, ldk, v_out);
 }
 
 }  // namespace mjk

@@ -178,6 +178,12 @@ int hip_check(Context *c, hipError_t e, const char *what);
 int coulomb(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
             int64_t ldj, double *v_out /* optional [M][nd] */);
 
+// coulomb() with every scratch buffer inside one fixed 64 MiB device array
+// (allocation-free: no hipMalloc / hipFree on any path); MJK_ERR_ARG when the
+// build needs more.  Calls that overlap on one GPU share the array and race.
+int coulomb_noalloc(Context *c, int32_t nd, const double *dlt, int64_t ldd, double *jlt,
+                    int64_t ldj, double *v_out);
+
 // Coulomb (one density) and FP64 exchange in one pass over the vectors: the
 // V = L^T D pass runs inside the half-transform, which reads every L^J anyway;
 // then the J = L V pass.  Same results as coulomb() followed by exchange().
