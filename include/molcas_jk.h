@@ -363,6 +363,21 @@ int mjk_coulomb_exchange(mjk_context *ctx, const double *dlt, int64_t ldd, doubl
                          const double *const *corb, double *klt, int64_t ldk,
                          double *v_out);
 
+/* One orbital set (1 <= nocc <= 256), and in the same pass over the vectors
+ * Z^J = C^T L^J C for every vector, written to z (device, nchol * nocc * nocc
+ * doubles: vector J at z + J*nocc*nocc, nocc x nocc row-major) for the
+ * backward (mjk_grad_vectors_z), which then needs no second half-transform.
+ * mjk_exchange_z: klt (one row) as mjk_exchange with any precision mode; Z
+ * carries the precision of the segment its vector lies in (X of an FP32 or
+ * split-BF16 segment is multiplied by C in FP64).  mjk_coulomb_exchange_z:
+ * jlt, klt and v_out as mjk_coulomb_exchange with nk = 1.  No counterpart in
+ * the reference (which has no gradient); the exchange part replaces
+ * Cho_X_getVtra + DGEMM_Tri (src/fock_util/cho_fscf.F90:250,276). */
+int mjk_exchange_z(mjk_context *ctx, int32_t nocc, const double *corb, double *klt,
+                   int32_t precision, double *z);
+int mjk_coulomb_exchange_z(mjk_context *ctx, const double *dlt, double *jlt, int32_t nocc,
+                           const double *corb, double *klt, double *v_out, double *z);
+
 /* Exchange as mjk_exchange with a precision ladder over the vector index:
  * J in [0, j_fp32) in FP64, [j_fp32, j_bf16) in FP32 and [j_bf16, nchol) in
  * split-BF16 (three products); every partial sum is reduced in FP64.  With
@@ -480,6 +495,13 @@ int mjk_grad_orbitals(mjk_context *ctx, int32_t nocc, const double *corb, const 
  * dE/dL.  X^J = C^T L^J is recomputed in chunks bounded by the X budget. */
 int mjk_grad_vectors(mjk_context *ctx, int32_t nocc, const double *corb, const double *p_lt,
                      const double *v, double alpha, double beta, double *dl, int64_t ldd);
+
+/* mjk_grad_vectors with Z^J = C^T L^J C given (z as written by mjk_exchange_z
+ * or mjk_coulomb_exchange_z for the same orbitals): no half-transform; per
+ * vector chunk W = C Z^J, then the packed dl.  No reference counterpart. */
+int mjk_grad_vectors_z(mjk_context *ctx, int32_t nocc, const double *corb, const double *p_lt,
+                       const double *v, const double *z, double alpha, double beta, double *dl,
+                       int64_t ldd);
 
 /* Waits for all work queued on the context stream. */
 int mjk_sync(mjk_context *ctx);

@@ -63,6 +63,8 @@ SIGNATURES = {
     "mjk_coulomb_noalloc": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp]),
     "mjk_exchange": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32]),
     "mjk_coulomb_exchange": (_c.c_int, [_vp, _vp, _i64, _vp, _i64, _i32, _vp, _vp, _vp, _i64, _vp]),
+    "mjk_exchange_z": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp]),
+    "mjk_coulomb_exchange_z": (_c.c_int, [_vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "mjk_exchange_mixed": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _i64]),
     "mjk_vector_norms": (_c.c_int, [_vp, _vp]),
     "mjk_exchange_screened": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32, _c.c_double,
@@ -86,6 +88,8 @@ SIGNATURES = {
     "mjk_density": (_c.c_int, [_vp, _i32, _vp, _c.c_double, _vp]),
     "mjk_grad_orbitals": (_c.c_int, [_vp, _i32, _vp, _vp, _c.c_double, _vp]),
     "mjk_grad_vectors": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _c.c_double, _c.c_double, _vp, _i64]),
+    "mjk_grad_vectors_z": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _c.c_double, _c.c_double, _vp,
+                                      _i64]),
     "mjk_sync": (_c.c_int, [_vp]),
     "mjk_set_profiling": (_c.c_int, [_vp, _i32]),
     "mjk_phase_times": (_c.c_int, [_vp, _vp, _vp]),

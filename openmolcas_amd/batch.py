@@ -9,7 +9,10 @@ GPU together.  ``step()`` runs, for every system, the closed-shell energy build
 (density, Coulomb, exchange, Fock and energy steps) followed, unless
 ``backward=False``, by the hand-written backward (dE/dC = 4FC and the packed
 dE/dL); in FP64 the Coulomb V pass runs inside the half-transform
-(mjk_coulomb_exchange).  ``tune_ladder`` places each system's FP64 / FP32 /
+(mjk_coulomb_exchange).  When dE/dL is wanted and a system's nchol nocc^2
+doubles fit ``z_budget`` bytes, its forward also keeps Z^J = C^T L^J C
+(mjk_coulomb_exchange_z / mjk_exchange_z) and the backward forms dE/dL from Z
+without a second half-transform; otherwise the backward recomputes X^J.  ``tune_ladder`` places each system's FP64 / FP32 /
 split-BF16 split points for an energy tolerance (openmolcas_amd.ladder.tune,
 verified per system) and precision "ladder" runs them.  ``capture()``
 records one step -- the fork to the lane streams and the join back included --
@@ -28,7 +31,7 @@ from .synthetic import chol_vectors_torch, npair, orbitals_torch
 
 class MoleculeBatch:
     def __init__(self, shapes, device=None, seed: int = 0, with_grad_l: bool = True,
-                 lanes: int = 4):
+                 lanes: int = 4, z_budget: int = 16 << 30):
         """shapes: list of (nbas, nchol, nocc)."""
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         self.stream = torch.cuda.Stream(device=self.device)
@@ -46,7 +49,7 @@ class MoleculeBatch:
         self.ctx = [None] * len(self.shapes)
         self.L, self.C, self.h = [None] * len(shapes), [None] * len(shapes), [None] * len(shapes)
         self.E = torch.zeros(len(shapes), dtype=torch.float64, device=self.device)
-        self.dC, self.dL = [None] * len(shapes), [None] * len(shapes)
+        self.dC, self.dL, self.Z = [None] * len(shapes), [None] * len(shapes), [None] * len(shapes)
         first = [None] * nl
         for i, (n, M, o) in enumerate(self.shapes):
             k = self.lane_of[i]
@@ -68,6 +71,8 @@ class MoleculeBatch:
                 self.dC[i] = torch.zeros((n, o), dtype=torch.float64, device=self.device)
                 self.dL[i] = (torch.zeros((M, npair(n)), dtype=torch.float64, device=self.device)
                               if with_grad_l else None)
+                if with_grad_l and 8 * M * o * o <= z_budget:
+                    self.Z[i] = torch.zeros((M, o, o), dtype=torch.float64, device=self.device)
         for st in self.lane_streams:
             st.synchronize()
         self.graph = None
@@ -99,12 +104,13 @@ class MoleculeBatch:
 
     def _system(self, i, precision, backward=True):
         c, C, h = self.ctx[i], self.C[i], self.h[i]
+        Z = self.Z[i] if backward else None          # Z^J = C^T L^J C for dE/dL
         dlt = c.density(C, 2.0)                      # fold(P), P = 2 C C^T
         if precision == "fp64":                      # V pass inside the half-transform
-            J, K, V = c.coulomb_exchange(dlt, [C], want_v=True)
+            J, K, V = c.coulomb_exchange(dlt, [C], want_v=True, z=Z)
         else:
             J, V = c.coulomb(dlt, want_v=True)
-            K = c.exchange([C], precision=precision)  # K[C C^T]
+            K = c.exchange([C], precision=precision, z=Z)  # K[C C^T]
         G = torch.zeros_like(J)
         c.fock_update(G, J, [0], [1.0], K, [-1.0])   # G = J - 1/2 K[P]
         self.E[i:i + 1].copy_(c.energy(h, dlt, G))
@@ -113,7 +119,7 @@ class MoleculeBatch:
         F = h + G
         self.dC[i].copy_(c.grad_orbitals(C, F, 4.0))
         if self.dL[i] is not None:
-            c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i])
+            c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i], z=Z)
 
     def step(self, precision="fp64", backward=True):
         """Forward (+ backward) of every system: forks from the batch stream to

@@ -16,6 +16,8 @@
 // chunk size follows the context's X budget); Z = X C, W = C Z and the packed
 // lower triangle of W C^T go through one batched FP64-MFMA GEMM whose epilogue
 // writes the vector gradient directly in packed storage.
+#include <type_traits>
+
 #include "mjk_internal.h"
 
 namespace mjk {
@@ -32,6 +34,7 @@ __device__ inline int64_t tri0(int64_t a) { return a * (a + 1) / 2; }
 
 struct GemmBatch {
   const double *A;  // A[b][i][k] at A + b*sA + i*lda + k
+  const float *A32; // the same for an FP32 A operand (k_gemm<..., float>)
   int64_t sA, lda;
   const double *B;  // NN: B[b][k][j] at B + b*sB + k*ldb + j;  NT: B[b][j][k] at ... j*ldb + k
   int64_t sB, ldb;
@@ -51,8 +54,9 @@ struct GemmBatch {
 // grid = (tiles_i * tiles_j, batch).  O = alpha * A B (NN) or A B^T (NT).
 // Double-buffered LDS with one barrier per kBK chunk: chunk c is multiplied
 // out of buffer c&1 while chunk c+1 is stored from registers into the other
-// buffer and chunk c+2 is fetched.
-template <bool BT, int EPI>
+// buffer and chunk c+2 is fetched.  TA: element type of A (an FP32 A is
+// widened to FP64 on load; the products and sums are FP64).
+template <bool BT, int EPI, typename TA = double>
 __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
   __shared__ double sA[2][kTM][kBK + 1];
   __shared__ double sB[2][kBK][kTM + 1];
@@ -61,7 +65,9 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
   if (EPI == 1 && ti < tj) return;  // strictly upper tile of a lower-triangle output
   const int64_t b = blockIdx.y;
   const int i0 = ti * kTM, j0 = tj * kTM;
-  const double *__restrict__ A = g.A + b * g.sA;
+  const TA *__restrict__ A;
+  if constexpr (std::is_same<TA, float>::value) A = g.A32 + b * g.sA;
+  else A = g.A + b * g.sA;
   const double *__restrict__ B = g.B + b * g.sB;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
@@ -86,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
       const int r = e / kBK, kk = e % kBK;  // A: row r, column kk
       const int gi = i0 + r, gk = k0 + kk;
       const bool oka = gi < g.m && gk < g.k;
-      ra[it] = A[oka ? int64_t(gi) * g.lda + gk : 0];
+      ra[it] = double(A[oka ? int64_t(gi) * g.lda + gk : 0]);
       bool okb;
       if (BT) {  // B stored [j][k]: row r of the j tile
         const int gj = j0 + r;
@@ -179,14 +185,52 @@ __global__ void k_density_lt(const double *__restrict__ corb, int o, int n, doub
   }
 }
 
-template <bool BT, int EPI>
+template <bool BT, int EPI, typename TA = double>
 void gemm(Context *c, const GemmBatch &g, int batch) {
   const int tiles = ((g.m + kTM - 1) / kTM) * ((g.n + kTM - 1) / kTM);
-  hipLaunchKernelGGL((k_gemm<BT, EPI>), dim3(unsigned(tiles), unsigned(batch)), dim3(kThreads), 0,
-                     c->stream, g);
+  hipLaunchKernelGGL((k_gemm<BT, EPI, TA>), dim3(unsigned(tiles), unsigned(batch)), dim3(kThreads),
+                     0, c->stream, g);
+}
+
+// dL[J] = s * (alpha V_J P + beta W[J] C^T) for the jn vectors whose W = C Z
+// sits at W (n_pad x ldc per vector): lower triangle, packed, row stride ldd.
+void grad_from_w(Context *c, int nocc, const double *W, int ldc, const double *v, const double *P,
+                 double alpha, double beta, double *dl, int64_t ldd, int64_t jn) {
+  GemmBatch g{};
+  g.A = W; g.sA = int64_t(c->n_pad) * ldc; g.lda = ldc;
+  g.B = static_cast<const double *>(c->w->cbank.ptr); g.sB = 0; g.ldb = ldc;  // NT: B[j][k] = C[b][k]
+  g.m = c->n; g.n = c->n; g.k = nocc; g.alpha = beta;
+  g.G = dl; g.ldg = ldd; g.V = v; g.Pl = P; g.gv = alpha;
+  gemm<true, 1>(c, g, int(jn));
+}
+
+// P = 2 C C^T in plain LT storage (the misc workspace) unless the caller gave it.
+int density_or(Context *c, int nocc, const double *corb, const double *p_lt, const double **P) {
+  *P = p_lt;
+  if (p_lt) return MJK_OK;
+  int rc;
+  if ((rc = ensure(c, c->w->misc, sizeof(double) * size_t(c->np)))) return rc;
+  double *pw = static_cast<double *>(c->w->misc.ptr);
+  hipLaunchKernelGGL(k_density_lt, dim3(unsigned(c->n)), dim3(256), 0, c->stream, corb, nocc, c->n,
+                     2.0, 0, pw);
+  *P = pw;
+  return MJK_OK;
 }
 
 }  // namespace
+
+void z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t jn, double *z) {
+  GemmBatch g{};
+  if (x_f32) g.A32 = static_cast<const float *>(X);
+  else g.A = static_cast<const double *>(X);
+  g.sA = int64_t(ldc) * c->n_pad; g.lda = c->n_pad;                          // X[J][k][a]
+  g.B = static_cast<const double *>(c->w->cbank.ptr); g.sB = 0; g.ldb = ldc;  // C[a][l] (NN)
+  g.O = z; g.sO = int64_t(nocc) * nocc; g.ldo = nocc;                         // Z[J][k][l]
+  g.m = nocc; g.n = nocc; g.k = c->n; g.alpha = 1.0;
+  if (x_f32) gemm<false, 0, float>(c, g, int(jn));
+  else gemm<false, 0>(c, g, int(jn));
+}
+
 }  // namespace mjk
 
 using namespace mjk;
@@ -251,15 +295,8 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
   double *X = static_cast<double *>(c->w->x.ptr);
   double *Z = X + jc * ldc * int64_t(n_pad);
   double *W = Z + jc * ldc * int64_t(ldc);
-  // P: caller's plain LT, or 2 C C^T
-  const double *P = p_lt;
-  if (!P) {
-    if ((rc = ensure(c, c->w->misc, sizeof(double) * size_t(np)))) return rc;
-    double *pw = static_cast<double *>(c->w->misc.ptr);
-    hipLaunchKernelGGL(k_density_lt, dim3(unsigned(n)), dim3(256), 0, c->stream, corb, nocc, n,
-                       2.0, 0, pw);
-    P = pw;
-  }
+  const double *P = nullptr;  // caller's plain LT, or 2 C C^T
+  if ((rc = density_or(c, nocc, corb, p_lt, &P))) return rc;
   for (int64_t ch = 0; ch < nchunk; ++ch) {
     const int64_t j0 = ch * jc, jn = std::min(jc, M - j0);
     phase_begin(c);
@@ -281,13 +318,48 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
     g.m = n; g.n = nocc; g.k = nocc; g.alpha = 1.0;
     gemm<false, 0>(c, g, int(jn));
     // dL[J] = s * (alpha V_J P + beta W[J] C^T), lower triangle, packed
-    g = GemmBatch{};
-    g.A = W; g.sA = int64_t(n_pad) * ldc; g.lda = ldc;
-    g.B = cb; g.sB = 0; g.ldb = ldc;  // NT: B[j][k] = C[b][k]
-    g.m = n; g.n = n; g.k = nocc; g.alpha = beta;
-    g.G = dl + j0 * ldd; g.ldg = ldd; g.V = v + j0; g.Pl = P; g.gv = alpha;
-    gemm<true, 1>(c, g, int(jn));
+    grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn);
     phase_end(c, MJK_PHASE_BACKWARD);
   }
   return hip_check(c, hipGetLastError(), "grad_vectors launch");
+}
+
+extern "C" int mjk_grad_vectors_z(mjk_context *c, int32_t nocc, const double *corb,
+                                  const double *p_lt, const double *v, const double *z,
+                                  double alpha, double beta, double *dl, int64_t ldd) {
+  if (!c) return MJK_ERR_ARG;
+  if (nocc < 1 || nocc > 256 || !corb || !v || !z || !dl || ldd < c->np)
+    return set_error(c, MJK_ERR_ARG, "grad_vectors_z: bad arguments");
+  const int n_pad = c->n_pad;
+  const int64_t M = c->nchol;
+  int rc;
+  std::vector<int> off, opad;
+  int ot = 0, tot = 0;
+  const double *sets[1] = {corb};
+  if ((rc = pack_bank(c, 1, &nocc, sets, off, opad, &ot, &tot))) return rc;
+  const int ldc = 16 * ot;
+  const double *cb = static_cast<const double *>(c->w->cbank.ptr);
+  // per vector: W = C Z (n_pad x ldc)
+  const int64_t per_vec = int64_t(n_pad) * ldc;
+  int64_t jc = std::max<int64_t>(1, std::min<int64_t>(M, c->x_budget / (8 * per_vec)));
+  const int64_t nchunk = ceil_div(M, jc);
+  jc = ceil_div(M, nchunk);
+  if ((rc = ensure(c, c->w->x, sizeof(double) * size_t(jc * per_vec)))) return rc;
+  double *W = static_cast<double *>(c->w->x.ptr);
+  const double *P = nullptr;
+  if ((rc = density_or(c, nocc, corb, p_lt, &P))) return rc;
+  for (int64_t ch = 0; ch < nchunk; ++ch) {
+    const int64_t j0 = ch * jc, jn = std::min(jc, M - j0);
+    phase_begin(c);
+    GemmBatch g{};
+    // W[J] = C Z[J]   (n x o), Z dense nocc x nocc per vector
+    g.A = cb; g.sA = 0; g.lda = ldc;
+    g.B = z + j0 * int64_t(nocc) * nocc; g.sB = int64_t(nocc) * nocc; g.ldb = nocc;
+    g.O = W; g.sO = per_vec; g.ldo = ldc;
+    g.m = c->n; g.n = nocc; g.k = nocc; g.alpha = 1.0;
+    gemm<false, 0>(c, g, int(jn));
+    grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn);
+    phase_end(c, MJK_PHASE_BACKWARD);
+  }
+  return hip_check(c, hipGetLastError(), "grad_vectors_z launch");
 }

@@ -238,6 +238,15 @@ int sym_counts(mjk_context *c, const SymMap &m, int nden, const int32_t *nforb,
 }
 }  // namespace
 
+namespace {
+// The context's Z output, set for the duration of one call on every return path.
+struct ZScope {
+  Context *c;
+  ZScope(Context *ctx, double *z, int32_t nocc) : c(ctx) { c->zs = ZSave{z, nocc}; }
+  ~ZScope() { c->zs = ZSave{}; }
+};
+}  // namespace
+
 extern "C" {
 
 int mjk_create(mjk_context **out, int32_t nbas, int64_t nchol) {
@@ -581,6 +590,28 @@ int mjk_coulomb_exchange(mjk_context *c, const double *dlt, int64_t ldd, double 
   for (int d = 0; d < nk; ++d)
     if (nocc[d] > 0 && !corb[d]) return set_error(c, MJK_ERR_ARG, "exchange: null orbital matrix");
   return coulomb_exchange(c, dlt, ldd, jlt, ldj, nk, nocc, corb, klt, ldk, v_out);
+}
+
+int mjk_exchange_z(mjk_context *c, int32_t nocc, const double *corb, double *klt,
+                   int32_t precision, double *z) {
+  if (!c) return MJK_ERR_ARG;
+  if (!corb || !klt || !z) return set_error(c, MJK_ERR_ARG, "exchange_z: null pointer");
+  if (nocc < 1 || nocc > 256) return set_error(c, MJK_ERR_ARG, "exchange_z: nocc must be in [1,256]");
+  ZScope scope(c, z, nocc);
+  const double *sets[1] = {corb};
+  return exchange(c, 1, &nocc, sets, klt, c->np, precision);
+}
+
+int mjk_coulomb_exchange_z(mjk_context *c, const double *dlt, double *jlt, int32_t nocc,
+                           const double *corb, double *klt, double *v_out, double *z) {
+  if (!c) return MJK_ERR_ARG;
+  if (!dlt || !jlt || !corb || !klt || !z)
+    return set_error(c, MJK_ERR_ARG, "coulomb_exchange_z: null pointer");
+  if (nocc < 1 || nocc > 256)
+    return set_error(c, MJK_ERR_ARG, "coulomb_exchange_z: nocc must be in [1,256]");
+  ZScope scope(c, z, nocc);
+  const double *sets[1] = {corb};
+  return coulomb_exchange(c, dlt, c->np, jlt, c->np, 1, &nocc, sets, klt, c->np, v_out);
 }
 
 int mjk_exchange_mixed(mjk_context *c, int32_t nk, const int32_t *nocc, const double *const *corb,

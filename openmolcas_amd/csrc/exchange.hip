@@ -1166,8 +1166,15 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
   bool use[3] = {false, false, false};
   for (const Segment &s : segs)
     if (s.j1 > s.j0) use[s.prec] = true;
-  // one bank per precision in use; the reduced-precision ones read the FP32 copy of L
-  if (use[MJK_PREC_FP64] && (rc = pack_bank_t<double>(c, nk, nocc, corb, off, ldc, c->w->cbank)))
+  const bool zsave = c->zs.z != nullptr;
+  if (zsave && (nk != 1 || nocc[0] != c->zs.nocc))
+    return set_error(c, MJK_ERR_ARG, "exchange: Z output needs exactly one orbital set");
+  if (zsave && (tau > 0.0 || thr_x > 0.0))
+    return set_error(c, MJK_ERR_ARG, "exchange: Z output cannot be screened");
+  // one bank per precision in use; the reduced-precision ones read the FP32
+  // copy of L.  Z = X C always multiplies by the FP64 bank.
+  if ((use[MJK_PREC_FP64] || zsave) &&
+      (rc = pack_bank_t<double>(c, nk, nocc, corb, off, ldc, c->w->cbank)))
     return rc;
   if (use[MJK_PREC_FP32] && (rc = pack_bank_t<float>(c, nk, nocc, corb, off, ldc, c->w->cb32)))
     return rc;
@@ -1289,6 +1296,12 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
         }
       }
       phase_end(c, MJK_PHASE_HALF_TRANSFORM);
+      if (zsave) {
+        phase_begin(c);
+        z_from_x(c, c->w->x.ptr, s.prec != MJK_PREC_FP64, ldc, nocc[0], jn,
+                 c->zs.z + j0 * int64_t(nocc[0]) * nocc[0]);
+        phase_end(c, MJK_PHASE_BACKWARD);
+      }
       if (screen && tau > 0.0) {
         double *xn2 = static_cast<double *>(c->w->xn2.ptr);
         if (s.prec == MJK_PREC_FP64)

@@ -63,6 +63,14 @@ struct FuseV {
   int ntile = 0;   // AO tiles per vector of the last launch (set by the launcher)
 };
 
+// Z^J = C^T L^J C saved by the exchange in progress for the backward (z null =
+// off; one orbital set of nocc columns): nocc x nocc row-major per vector, at
+// z + J * nocc * nocc for every vector J of the call.
+struct ZSave {
+  double *z = nullptr;
+  int32_t nocc = 0;
+};
+
 struct ReducedSet {
   int64_t nrs = 0;
   bool identity = false;      // rs position r maps to LT index r for all r
@@ -135,6 +143,7 @@ struct Context {
   DevBuf scount;                  // device: [K items skipped, X tiles skipped]
   Screen scr;                     // screening of the exchange launch in progress
   FuseV fv;                       // Coulomb V fused into the exchange in progress
+  ZSave zs;                       // Z^J = C^T L^J C kept by the exchange in progress
   int64_t k_items = 0, x_tiles = 0;  // work items of the last screened exchange
   std::shared_ptr<Workspace> wsp;  // workspaces, possibly shared with other contexts
   Workspace *w = nullptr;
@@ -205,6 +214,11 @@ int pack_bank(Context *c, int32_t nk, const int32_t *nocc, const double *const *
 // Half-transform of vectors j0..j0+jn-1 against the bank (16*ot columns):
 // X[(jr*16*ot + k)*n_pad + a] = sum_b Cb[b][k] L^{j0+jr}_{ab}.
 void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, double *X);
+
+// Z[J] = X[J] C for the jn vectors of one half-transform chunk: X holds
+// 16*ot = ldc rows of n_pad per vector, FP64 (x_f32 false) or FP32; C is the
+// FP64 bank (cbank).  Writes z + J * nocc * nocc, nocc x nocc row-major.
+void z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t jn, double *z);
 
 // Exchange with tile screening (thresholds 0 = off): a K tile of a vector
 // slice is skipped when sum_J ||X^J_A||_F ||X^J_B||_F < tau, a half-transform

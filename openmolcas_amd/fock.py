@@ -371,10 +371,27 @@ class JKContext:
         self._check(self.lib.mjk_coulomb(self._h, nd, _ptr(d), _ld(d), _ptr(j), _ld(j), _ptr(v)))
         return (j, v) if want_v else j
 
-    def exchange(self, corbs, precision="fp64") -> torch.Tensor:
+    def _z_out(self, cs, z):
+        if len(cs) != 1 or cs[0].dim() != 2 or not 1 <= cs[0].shape[1] <= 256:
+            raise ValueError("the Z output needs exactly one orbital matrix of 1..256 columns")
+        o = cs[0].shape[1]
+        if (z.dtype != torch.float64 or z.device != self.device or not z.is_contiguous()
+                or z.numel() != self.nchol * o * o):
+            raise ValueError("z must be a contiguous float64 device tensor of nchol*nocc*nocc elements")
+        return o
+
+    def exchange(self, corbs, precision="fp64", z=None) -> torch.Tensor:
         """K_d = sum_J (L^J C_d)(L^J C_d)^T in LT storage for each orbital
-        matrix C_d (nbas, o_d) on the device (C-order = Porb memory)."""
+        matrix C_d (nbas, o_d) on the device (C-order = Porb memory).  With z
+        (one orbital matrix only; (nchol, o, o) float64 on the device) the
+        same pass also writes Z^J = C^T L^J C for grad_vectors(z=...)."""
         cs = [_dev_f64(c, self.device) for c in corbs]
+        if z is not None:
+            o = self._z_out(cs, z)
+            k = torch.empty((1, self.npair), dtype=torch.float64, device=self.device)
+            self._check(self.lib.mjk_exchange_z(self._h, o, _ptr(cs[0]), _ptr(k),
+                                                precision_code(precision), _ptr(z)))
+            return k
         nk = len(cs)
         nocc = np.array([c.shape[1] if c.dim() == 2 else 0 for c in cs], dtype=np.int32)
         ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
@@ -383,15 +400,24 @@ class JKContext:
                                           precision_code(precision)))
         return k
 
-    def coulomb_exchange(self, dlt: torch.Tensor, corbs, want_v: bool = False):
+    def coulomb_exchange(self, dlt: torch.Tensor, corbs, want_v: bool = False, z=None):
         """J of one folded LT density dlt (npair,) and the FP64 K_d of each
         orbital matrix C_d in one pass over the vectors (the V = L^T D pass
         runs inside the half-transform).  Returns (J (1, npair), K (nk, npair))
-        and V (nchol, 1) when want_v."""
+        and V (nchol, 1) when want_v.  With z (one orbital matrix only) the
+        pass also writes Z^J = C^T L^J C, as exchange(z=...)."""
         d = _dev_f64(dlt, self.device).reshape(1, -1)
         if d.shape[1] != self.npair:
             raise ValueError("dlt must hold one folded density of nbas(nbas+1)/2 elements")
         cs = [_dev_f64(c, self.device) for c in corbs]
+        if z is not None:
+            o = self._z_out(cs, z)
+            j = torch.empty((1, self.npair), dtype=torch.float64, device=self.device)
+            k = torch.empty((1, self.npair), dtype=torch.float64, device=self.device)
+            v = torch.empty((self.nchol, 1), dtype=torch.float64, device=self.device) if want_v else None
+            self._check(self.lib.mjk_coulomb_exchange_z(self._h, _ptr(d), _ptr(j), o, _ptr(cs[0]),
+                                                        _ptr(k), _ptr(v), _ptr(z)))
+            return (j, k, v) if want_v else (j, k)
         nk = len(cs)
         nocc = np.array([c.shape[1] if c.dim() == 2 else 0 for c in cs], dtype=np.int32)
         ptrs = (ctypes.c_void_p * nk)(*[c.data_ptr() for c in cs])
@@ -521,12 +547,19 @@ class JKContext:
         return dc
 
     def grad_vectors(self, corb: torch.Tensor, v: torch.Tensor, alpha: float, beta: float,
-                     p_lt=None, out=None) -> torch.Tensor:
-        """(nchol, npair) packed gradient s_ab (alpha V_J P_ab + beta (C Z^J C^T)_ab)."""
+                     p_lt=None, out=None, z=None) -> torch.Tensor:
+        """(nchol, npair) packed gradient s_ab (alpha V_J P_ab + beta (C Z^J C^T)_ab).
+        z: the Z^J = C^T L^J C an exchange(z=...) or coulomb_exchange(z=...)
+        of the same orbitals wrote (no half-transform then); None recomputes."""
         c = _dev_f64(corb, self.device)
         vv = v.to(torch.float64).contiguous()
         dl = out if out is not None else torch.empty((self.nchol, self.npair), dtype=torch.float64,
                                                      device=self.device)
+        if z is not None:
+            o = self._z_out([c], z)
+            self._check(self.lib.mjk_grad_vectors_z(self._h, o, _ptr(c), _ptr(p_lt), _ptr(vv), _ptr(z),
+                                                    float(alpha), float(beta), _ptr(dl), _ld(dl)))
+            return dl
         self._check(self.lib.mjk_grad_vectors(self._h, c.shape[1], _ptr(c), _ptr(p_lt), _ptr(vv),
                                               float(alpha), float(beta), _ptr(dl), _ld(dl)))
         return dl

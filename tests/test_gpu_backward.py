@@ -83,6 +83,46 @@ def test_vector_gradient_chunked_equals_single_pass(gpu):
     assert np.max(np.abs(g1 - g2)) <= 1e-13 * max(1.0, np.max(np.abs(g1)))
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_saved_z_gradient_equals_recompute(gpu, fused):
+    """Z^J = C^T L^J C kept by the forward (mjk_coulomb_exchange_z or
+    mjk_exchange_z) against numpy, K unchanged by the Z output, and the
+    vector gradient from Z (mjk_grad_vectors_z) equal to the one that
+    recomputes X^J (the same half-transform and GEMMs, so within 1e-13)."""
+    n, M, o = 150, 90, 21
+    Ln = chol_vectors_np(n, M, seed=83)
+    Cn = orbitals_np(n, o, seed=84)
+    L = torch.tensor(Ln, device=gpu)
+    C = torch.tensor(Cn, device=gpu)
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(L)
+    jk.set_x_budget(8 * 16 * (2 * 32 * 256 + 32 * 32))  # several chunks on both sides
+    dlt = jk.density(C, 2.0)
+    Z = torch.zeros((M, o, o), dtype=torch.float64, device=gpu)
+    if fused:
+        J, K, V = jk.coulomb_exchange(dlt, [C], want_v=True, z=Z)
+        J0, K0, V0 = jk.coulomb_exchange(dlt, [C], want_v=True)
+        assert torch.equal(J, J0) and torch.equal(V, V0)
+    else:
+        _, V = jk.coulomb(dlt, want_v=True)
+        K = jk.exchange([C], z=Z)
+        K0 = jk.exchange([C])
+    assert torch.equal(K, K0)
+    Zr = np.einsum("ak,jab,bl->jkl", Cn, np.stack([unpack(Ln[j], n) for j in range(M)]), Cn)
+    assert np.max(np.abs(Z.cpu().numpy() - Zr)) <= 1e-12 * np.max(np.abs(Zr))
+    v = V[:, 0].contiguous()
+    g_rec = jk.grad_vectors(C, v, 1.0, -2.0)
+    g_z = jk.grad_vectors(C, v, 1.0, -2.0, z=Z)
+    assert torch.max(torch.abs(g_z - g_rec)).item() <= 1e-13 * torch.max(torch.abs(g_rec)).item()
+    # a reduced-precision forward keeps Z at that precision (FP32 X, FP64 product)
+    Z32 = torch.zeros_like(Z)
+    jk.exchange([C], precision="fp32", z=Z32)
+    assert torch.max(torch.abs(Z32 - Z)).item() <= 1e-5 * torch.max(torch.abs(Z)).item()
+    with pytest.raises(ValueError):
+        jk.exchange([C], z=torch.zeros((M, o, o + 1), dtype=torch.float64, device=gpu))
+    jk.close()
+
+
 def test_full_size_backward_n1200_m6000_o150(gpu):
     """Mixed precision plus backward workload: nbas=1200, nchol=6000, nocc=150.
     autograd dE/dC and dE/dL against central differences (step 1e-5) on 32
