@@ -29,6 +29,11 @@ HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
 
 
+def progress(msg):
+    """One progress line on stderr per finished phase (the JSON line stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -384,7 +389,9 @@ def batch_workload(args, dev):
 
     shapes = uneven_shapes(64, seed=5)
     batch = MoleculeBatch(shapes, device=dev, seed=3)
+    progress(f"batch of {len(shapes)} systems set up")
     batch.autotune("fp64")
+    progress("autotune done")
     nsys = len(shapes)
 
     def timed(precision, backward, steps, warmup):
@@ -399,6 +406,7 @@ def batch_workload(args, dev):
         return (time.perf_counter() - t0) / steps
 
     t64 = timed("fp64", True, args.steps, args.warmup)
+    progress(f"fp64 forward+backward replays: {1e3 * t64:.3f} ms per step")
     E64 = batch.E.clone()
     dC64 = [d.clone() for d in batch.dC]
     t_fwd = timed("fp64", False, args.steps, 1)
@@ -427,7 +435,9 @@ def batch_workload(args, dev):
     name, (label, flops, (ms, calls)) = max(ph.items(), key=lambda kv: kv[1][2][0])
     achieved = flops / (ms * 1e-3) / 1e12
     # ladder: per-system split points for |dE| <= 1e-6 Eh, timed the same way
+    progress("profiled eager step done")
     plans = batch.tune_ladder(tol=1e-6)
+    progress("ladder tuned")
     t_lad = timed("ladder", True, args.steps, 1)
     Elad = batch.E.clone()
     dClad = [d.clone() for d in batch.dC]
@@ -605,7 +615,9 @@ def single_build(args, dev, steps):
         },
     }
     if args.precision == "fp64" and not args.no_mixed:
+        progress(f"single build n={n}: {ms_step:.4f} ms per step")
         out["mixed"] = mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_step, args)
+        progress(f"single build n={n}: mixed modes done")
     ctx.close()
     torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
@@ -642,9 +654,12 @@ def main():
         line.update(head)
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = batch_cpu_baseline(shapes, seed=3)
+            progress("cpu baseline done")
         if not args.no_secondary:
             single = single_build(args, dev, steps=20)
+            progress("secondary n=512 build done")
             line["secondary"] = {"single_build_n512_m2048_o64": single, "build_n1200_m6000_o150": build1200(dev)}
+            progress("secondary n=1200 build done")
     else:
         single = single_build(args, dev, steps=args.steps)
         line["dtype"] = "f64" if args.precision == "fp64" else args.precision
