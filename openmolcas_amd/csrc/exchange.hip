@@ -79,6 +79,9 @@ __device__ inline int find_set(const SyrkSets &sets, int b) {
 }
 
 __device__ inline int64_t tri0(int64_t a) { return a * (a + 1) / 2; }
+// Offset of packed row a within one vector in 32-bit arithmetic: contexts hold
+// at most 46340 basis functions, so a(a+1) and the triangle fit in 32 bits.
+__device__ inline uint32_t tri0u(uint32_t a) { return a * (a + 1u) / 2u; }
 
 // Maps blockIdx.x so that consecutive logical blocks land on the same XCD
 // (hardware dispatches block i to XCD i % 8).  Returns -1 for padding blocks.
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_ht(
         const int b = b0 + bb, a = a0 + aa;
         const bool ok = (b < n) && (a < n);
         const int hi = b >= a ? b : a, lo = b >= a ? a : b;
-        const int64_t idx = ok ? tri0(hi) + lo : 0;  // branch-free masked load
+        const uint32_t idx = ok ? tri0u(uint32_t(hi)) + uint32_t(lo) : 0u;  // branch-free masked load
 #pragma unroll
         for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx];
         lok = ok ? (lok | (1u << i)) : (lok & ~(1u << i));
@@ -691,7 +694,7 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
 #pragma unroll
         for (int i = 0; i < LG; ++i) {
           const int g = tid + kThreads * i, q = g % (kB3K / 4), aa = g / (kB3K / 4), a = a0 + aa;
-          const float *__restrict__ row = Lj + tri0(a) + b0 + 4 * q;
+          const float *__restrict__ row = Lj + (tri0u(uint32_t(a)) + uint32_t(b0 + 4 * q));
 #pragma unroll
           for (int k = 0; k < 4; ++k) rl[i][k] = (a < n) ? row[k] : 0.f;
           lo[i] = aa * kB3S + 4 * q;
@@ -704,7 +707,8 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
           for (int k = 0; k < 4; ++k) {
             const int b = b0 + 4 * q + k;
             float v = 0.f;
-            if (b < n && a < n) v = (b >= a) ? Lj[tri0(b) + a] : Lj[tri0(a) + b];
+            if (b < n && a < n)
+              v = (b >= a) ? Lj[tri0u(uint32_t(b)) + uint32_t(a)] : Lj[tri0u(uint32_t(a)) + uint32_t(b)];
             rl[i][k] = v;
           }
           lo[i] = aa * kB3S + 4 * q;
