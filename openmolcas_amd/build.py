@@ -24,6 +24,11 @@ FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
     # packed FP32 instructions are kept out of every code object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: otherwise the compiler keeps a
+    # loop-carried accumulator in VGPRs but issues the MFMAs on AGPR copies,
+    # moving every accumulator across at each trip and draining the MFMA
+    # pipeline before the copy back
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall", "-Wno-unused-result", "-Wno-unused-value", "-Wno-pass-failed",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]

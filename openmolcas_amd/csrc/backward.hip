@@ -49,9 +49,18 @@ struct GemmBatch {
   const double *V;
   const double *Pl;
   double gv;
+  // launch shape, set by gemm(): output tiles per product (EPI == 1: the
+  // lower-triangle tiles only) and tiles x batch work items
+  int tiles;
+  int64_t items;
 };
 
-// grid = (tiles_i * tiles_j, batch).  O = alpha * A B (NN) or A B^T (NT).
+// grid.x = round_up(tiles * batch, 8).  O = alpha * A B (NN) or A B^T (NT).
+// Work item l = b * tiles + t (product b, output tile t); the block id is
+// remapped so that consecutive items run on one XCD (the hardware deals
+// blocks to the eight XCDs in turn): the tiles of one product then share
+// that XCD's L2 for their common operand rows instead of each XCD fetching
+// them from HBM.
 // Double-buffered LDS with one barrier per kBK chunk: chunk c is multiplied
 // out of buffer c&1 while chunk c+1 is stored from registers into the other
 // buffer and chunk c+2 is fetched.  TA: element type of A (an FP32 A is
@@ -60,10 +69,22 @@ template <bool BT, int EPI, typename TA = double>
 __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
   __shared__ double sA[2][kTM][kBK + 1];
   __shared__ double sB[2][kBK][kTM + 1];
-  const int ntj = (g.n + kTM - 1) / kTM;
-  const int ti = blockIdx.x / ntj, tj = blockIdx.x % ntj;
-  if (EPI == 1 && ti < tj) return;  // strictly upper tile of a lower-triangle output
-  const int64_t b = blockIdx.y;
+  const int64_t nblk = gridDim.x;  // multiple of 8
+  const int64_t l = (int64_t(blockIdx.x) & 7) * (nblk >> 3) + (int64_t(blockIdx.x) >> 3);
+  if (l >= g.items) return;
+  const int64_t b = l / g.tiles;
+  const int t = int(l - b * g.tiles);
+  int ti, tj;
+  if constexpr (EPI == 1) {  // lower-triangle tile t = ti (ti + 1) / 2 + tj
+    ti = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (ti * (ti + 1) / 2 > t) --ti;
+    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    tj = t - ti * (ti + 1) / 2;
+  } else {
+    const int ntj = (g.n + kTM - 1) / kTM;
+    ti = t / ntj;
+    tj = t - ti * ntj;
+  }
   const int i0 = ti * kTM, j0 = tj * kTM;
   const TA *__restrict__ A;
   if constexpr (std::is_same<TA, float>::value) A = g.A32 + b * g.sA;
@@ -80,30 +101,49 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
   constexpr int PN = kTM * kBK / kThreads;
   static_assert(2 * PN <= 32, "element mask");
   double ra[PN], rb[PN];
+  // Element it of a thread sits at a fixed (row, contraction column) of every
+  // chunk, so its offset inside the product's operand is computed once and a
+  // chunk adds only its first contraction index (32-bit byte offsets from the
+  // product's base: every operand is below 2^28 elements, checked by gemm()).
   // Out-of-range elements load element 0 and are zeroed when stashed (bit it
   // of the mask for A, bit PN + it for B), so a chunk's loads stay in flight
   // across the multiply of the previous chunk.
+  uint32_t aoff[PN], boff[PN];
+  int akk[PN], bkk[PN];
+  uint32_t rowok = 0;  // bit it: A row inside m; bit PN + it: B row / column inside n
+#pragma unroll
+  for (int it = 0; it < PN; ++it) {
+    const int e = tid + kThreads * it;
+    const int r = e / kBK, kk = e % kBK;  // A: row r, column kk
+    const int gi = i0 + r;
+    aoff[it] = uint32_t(gi) * uint32_t(g.lda) + uint32_t(kk);
+    akk[it] = kk;
+    rowok |= gi < g.m ? 1u << it : 0u;
+    if (BT) {  // B stored [j][k]: row r of the j tile
+      const int gj = j0 + r;
+      boff[it] = uint32_t(gj) * uint32_t(g.ldb) + uint32_t(kk);
+      bkk[it] = kk;
+      rowok |= gj < g.n ? 1u << (PN + it) : 0u;
+    } else {  // B stored [k][j]
+      const int kr = e / kTM, jj = e % kTM, gj = j0 + jj;
+      boff[it] = uint32_t(kr) * uint32_t(g.ldb) + uint32_t(gj);
+      bkk[it] = kr;
+      rowok |= gj < g.n ? 1u << (PN + it) : 0u;
+    }
+  }
+  const char *__restrict__ Ab = reinterpret_cast<const char *>(A);
+  const char *__restrict__ Bb = reinterpret_cast<const char *>(B);
   uint32_t okm = 0;
   auto fetch = [&](int k0) {
     okm = 0;
+    const uint32_t bstep = BT ? uint32_t(k0) : uint32_t(k0) * uint32_t(g.ldb);
 #pragma unroll
     for (int it = 0; it < PN; ++it) {
-      const int e = tid + kThreads * it;
-      const int r = e / kBK, kk = e % kBK;  // A: row r, column kk
-      const int gi = i0 + r, gk = k0 + kk;
-      const bool oka = gi < g.m && gk < g.k;
-      ra[it] = double(A[oka ? int64_t(gi) * g.lda + gk : 0]);
-      bool okb;
-      if (BT) {  // B stored [j][k]: row r of the j tile
-        const int gj = j0 + r;
-        okb = gj < g.n && gk < g.k;
-        rb[it] = B[okb ? int64_t(gj) * g.ldb + gk : 0];
-      } else {  // B stored [k][j]
-        const int kr = e / kTM, jj = e % kTM;
-        const int gk2 = k0 + kr, gj = j0 + jj;
-        okb = gk2 < g.k && gj < g.n;
-        rb[it] = B[okb ? int64_t(gk2) * g.ldb + gj : 0];
-      }
+      const bool oka = ((rowok >> it) & 1u) && akk[it] + k0 < g.k;
+      const bool okb = ((rowok >> (PN + it)) & 1u) && bkk[it] + k0 < g.k;
+      ra[it] = double(*reinterpret_cast<const TA *>(
+          Ab + (oka ? (aoff[it] + uint32_t(k0)) * uint32_t(sizeof(TA)) : 0u)));
+      rb[it] = *reinterpret_cast<const double *>(Bb + (okb ? (boff[it] + bstep) * 8u : 0u));
       okm |= (oka ? 1u << it : 0u) | (okb ? 1u << (PN + it) : 0u);
     }
   };
@@ -125,42 +165,77 @@ __global__ __launch_bounds__(kThreads) void k_gemm(GemmBatch g) {
   __syncthreads();
   for (int c = 0; c < nch; ++c) {
     const int cur = c & 1;
+    // operand fragments of the next 4-row step are read from LDS while the
+    // current step's MFMAs run
+    double af[2][2], bf[2][2];
+    auto ld = [&](int ks, int q) {
+      const int kr = ks * 4 + (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[q][i] = sA[cur][wm * 32 + i * 16 + (lane & 15)][kr];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[q][j] = sB[cur][kr][wn * 32 + j * 16 + (lane & 15)];
+    };
+    ld(0, 0);
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
-      const int kr = ks * 4 + (lane >> 4);
-      double af[2], bf[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = sA[cur][wm * 32 + i * 16 + (lane & 15)][kr];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = sB[cur][kr][wn * 32 + j * 16 + (lane & 15)];
+      const int q = ks & 1;
+      if (ks + 1 < kBK / 4) ld(ks + 1, q ^ 1);
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = mfma64(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma64(af[q][i], bf[q][j], acc[i][j]);
     }
     if (c + 1 < nch) stash(cur ^ 1);
     if (c + 2 < nch) fetch((c + 2) * kBK);
     __syncthreads();
   }
   // result layout of v_mfma_f64_16x16x4f64: row 4*r + lane/16, column lane%16
+  if constexpr (EPI == 0) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = i0 + wm * 32 + i * 16 + 4 * r + (lane >> 4);
-        const int gj = j0 + wn * 32 + j * 16 + (lane & 15);
-        if (gi >= g.m || gj >= g.n) continue;
-        if (EPI == 0) {
-          g.O[b * g.sO + int64_t(gi) * g.ldo + gj] = g.alpha * acc[i][j][r];
-        } else {
-          if (gi < gj) continue;
-          const int64_t p = tri0(gi) + gj;
-          const double s = (gi == gj) ? 1.0 : 2.0;
-          g.G[b * g.ldg + p] = s * (g.gv * g.V[b] * g.Pl[p] + g.alpha * acc[i][j][r]);
+        for (int r = 0; r < 4; ++r) {
+          const int gi = i0 + wm * 32 + i * 16 + 4 * r + (lane >> 4);
+          const int gj = j0 + wn * 32 + j * 16 + (lane & 15);
+          if (gi < g.m && gj < g.n) g.O[b * g.sO + int64_t(gi) * g.ldo + gj] = g.alpha * acc[i][j][r];
         }
-      }
+  } else {
+    // Every density value this lane needs is loaded before the first store:
+    // with the loads and stores interleaved, each store would wait for its
+    // load's round trip (G and P may alias as far as the compiler knows).
+    const double *__restrict__ Pl = g.Pl;
+    double *__restrict__ G = g.G + b * g.ldg;
+    const double vb = g.gv * g.V[b];
+    double pv[2][2][4];
+    uint32_t pi[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + 4 * r + (lane >> 4));
+          const uint32_t gj = uint32_t(j0 + wn * 32 + j * 16 + (lane & 15));
+          const bool ok = gi < uint32_t(g.m) && gi >= gj;
+          pi[i][j][r] = ok ? gi * (gi + 1u) / 2u + gj : ~0u;
+          pv[i][j][r] = Pl[ok ? pi[i][j][r] : 0u];
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t p = pi[i][j][r];
+          if (p == ~0u) continue;
+          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + 4 * r + (lane >> 4));
+          const uint32_t gj = uint32_t(j0 + wn * 32 + j * 16 + (lane & 15));
+          const double s = (gi == gj) ? 1.0 : 2.0;
+          G[p] = s * (vb * pv[i][j][r] + g.alpha * acc[i][j][r]);
+        }
+  }
 }
 
 // Dense symmetric n x n (row stride ld) from plain LT storage.
@@ -186,22 +261,31 @@ __global__ void k_density_lt(const double *__restrict__ corb, int o, int n, doub
 }
 
 template <bool BT, int EPI, typename TA = double>
-void gemm(Context *c, const GemmBatch &g, int batch) {
-  const int tiles = ((g.m + kTM - 1) / kTM) * ((g.n + kTM - 1) / kTM);
-  hipLaunchKernelGGL((k_gemm<BT, EPI, TA>), dim3(unsigned(tiles), unsigned(batch)), dim3(kThreads),
-                     0, c->stream, g);
+int gemm(Context *c, const GemmBatch &g0, int batch) {
+  GemmBatch g = g0;
+  // operands are addressed with 32-bit byte offsets inside one product
+  constexpr int64_t kMaxElems = int64_t(1) << 28;
+  if (int64_t(g.m) * g.lda >= kMaxElems || int64_t(g.k) * g.ldb >= kMaxElems ||
+      int64_t(g.n) * g.ldb >= kMaxElems)
+    return set_error(c, MJK_ERR_ARG, "gemm: operand of more than 2^28 elements");
+  const int ti = (g.m + kTM - 1) / kTM, tj = (g.n + kTM - 1) / kTM;
+  g.tiles = EPI == 1 ? ti * (ti + 1) / 2 : ti * tj;  // EPI == 1: square lower-triangle output
+  g.items = int64_t(g.tiles) * batch;
+  hipLaunchKernelGGL((k_gemm<BT, EPI, TA>), dim3(unsigned(round_up(g.items, 8))), dim3(kThreads), 0,
+                     c->stream, g);
+  return MJK_OK;
 }
 
 // dL[J] = s * (alpha V_J P + beta W[J] C^T) for the jn vectors whose W = C Z
 // sits at W (n_pad x ldc per vector): lower triangle, packed, row stride ldd.
-void grad_from_w(Context *c, int nocc, const double *W, int ldc, const double *v, const double *P,
+int grad_from_w(Context *c, int nocc, const double *W, int ldc, const double *v, const double *P,
                  double alpha, double beta, double *dl, int64_t ldd, int64_t jn) {
   GemmBatch g{};
   g.A = W; g.sA = int64_t(c->n_pad) * ldc; g.lda = ldc;
   g.B = static_cast<const double *>(c->w->cbank.ptr); g.sB = 0; g.ldb = ldc;  // NT: B[j][k] = C[b][k]
   g.m = c->n; g.n = c->n; g.k = nocc; g.alpha = beta;
   g.G = dl; g.ldg = ldd; g.V = v; g.Pl = P; g.gv = alpha;
-  gemm<true, 1>(c, g, int(jn));
+  return gemm<true, 1>(c, g, int(jn));
 }
 
 // P = 2 C C^T in plain LT storage (the misc workspace) unless the caller gave it.
@@ -219,7 +303,7 @@ int density_or(Context *c, int nocc, const double *corb, const double *p_lt, con
 
 }  // namespace
 
-void z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t jn, double *z) {
+int z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t jn, double *z) {
   GemmBatch g{};
   if (x_f32) g.A32 = static_cast<const float *>(X);
   else g.A = static_cast<const double *>(X);
@@ -227,8 +311,8 @@ void z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t 
   g.B = static_cast<const double *>(c->w->cbank.ptr); g.sB = 0; g.ldb = ldc;  // C[a][l] (NN)
   g.O = z; g.sO = int64_t(nocc) * nocc; g.ldo = nocc;                         // Z[J][k][l]
   g.m = nocc; g.n = nocc; g.k = c->n; g.alpha = 1.0;
-  if (x_f32) gemm<false, 0, float>(c, g, int(jn));
-  else gemm<false, 0>(c, g, int(jn));
+  if (x_f32) return gemm<false, 0, float>(c, g, int(jn));
+  return gemm<false, 0>(c, g, int(jn));
 }
 
 }  // namespace mjk
@@ -263,7 +347,7 @@ extern "C" int mjk_grad_orbitals(mjk_context *c, int32_t nocc, const double *cor
   g.B = corb; g.sB = 0; g.ldb = nocc;      // C[b][k] (NN)
   g.O = dc; g.sO = 0; g.ldo = nocc;        // dC[a][k]
   g.m = n; g.n = nocc; g.k = n; g.alpha = scale;
-  gemm<false, 0>(c, g, 1);
+  if ((rc = gemm<false, 0>(c, g, 1))) return rc;
   phase_end(c, MJK_PHASE_ASSEMBLY);
   return hip_check(c, hipGetLastError(), "grad_orbitals launch");
 }
@@ -309,16 +393,16 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
     g.B = cb; g.sB = 0; g.ldb = ldc;
     g.O = Z; g.sO = int64_t(ldc) * ldc; g.ldo = ldc;
     g.m = nocc; g.n = nocc; g.k = n; g.alpha = 1.0;
-    gemm<false, 0>(c, g, int(jn));
+    if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
     // W[J] = C Z[J]   (n x o)
     g = GemmBatch{};
     g.A = cb; g.sA = 0; g.lda = ldc;
     g.B = Z; g.sB = int64_t(ldc) * ldc; g.ldb = ldc;
     g.O = W; g.sO = int64_t(n_pad) * ldc; g.ldo = ldc;
     g.m = n; g.n = nocc; g.k = nocc; g.alpha = 1.0;
-    gemm<false, 0>(c, g, int(jn));
+    if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
     // dL[J] = s * (alpha V_J P + beta W[J] C^T), lower triangle, packed
-    grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn);
+    if ((rc = grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn))) return rc;
     phase_end(c, MJK_PHASE_BACKWARD);
   }
   return hip_check(c, hipGetLastError(), "grad_vectors launch");
@@ -357,8 +441,8 @@ extern "C" int mjk_grad_vectors_z(mjk_context *c, int32_t nocc, const double *co
     g.B = z + j0 * int64_t(nocc) * nocc; g.sB = int64_t(nocc) * nocc; g.ldb = nocc;
     g.O = W; g.sO = per_vec; g.ldo = ldc;
     g.m = c->n; g.n = nocc; g.k = nocc; g.alpha = 1.0;
-    gemm<false, 0>(c, g, int(jn));
-    grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn);
+    if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
+    if ((rc = grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn))) return rc;
     phase_end(c, MJK_PHASE_BACKWARD);
   }
   return hip_check(c, hipGetLastError(), "grad_vectors_z launch");

@@ -218,7 +218,7 @@ void ht_launch(Context *c, int ot, const double *cb, int64_t j0, int64_t jn, dou
 // Z[J] = X[J] C for the jn vectors of one half-transform chunk: X holds
 // 16*ot = ldc rows of n_pad per vector, FP64 (x_f32 false) or FP32; C is the
 // FP64 bank (cbank).  Writes z + J * nocc * nocc, nocc x nocc row-major.
-void z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t jn, double *z);
+int z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t jn, double *z);
 
 // Exchange with tile screening (thresholds 0 = off): a K tile of a vector
 // slice is skipped when sum_J ||X^J_A||_F ||X^J_B||_F < tau, a half-transform
