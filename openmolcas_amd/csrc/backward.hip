@@ -288,6 +288,247 @@ int grad_from_w(Context *c, int nocc, const double *W, int ldc, const double *v,
   return gemm<true, 1>(c, g, int(jn));
 }
 
+// ---- fused vector gradient ------------------------------------------------
+// G[J] (packed lower triangle, row stride ldg) = s_ab (gv V_J P_ab +
+// beta (C Z^J C^T)_ab) for the vectors J whose Z^J (nocc x nocc) has element
+// (k, l) at z + J sz + k ldz + l.
+struct GradRows {
+  const double *cb;  // orbital bank C[a][k]: n_pad rows of ldc, columns >= nocc zero
+  int ldc;
+  const double *z;
+  int64_t sz;
+  int ldz;
+  int o;             // nocc
+  const double *v;   // V[J]
+  const double *p;   // P, plain LT
+  double gv, beta;
+  double *g;
+  int64_t ldg;
+  int n, T;          // basis functions, row tiles of 64
+  int64_t items;     // T * vectors
+};
+
+typedef double f4d __attribute__((ext_vector_type(4)));
+
+// Work item (vector J, row tile I of 64 AOs).  Phase 1 forms W_I = C_I Z^J (64
+// x OC, OC = 16 OC16 >= nocc) with FP64 MFMA, both operands staged through LDS
+// in 16-column chunks, double-buffered; W_I goes to LDS.  Phase 2 walks the
+// column tiles j = 0..I: wave (wm, wn) forms its 32 x 32 quarter of W_I C_j^T
+// with the A operand from LDS and the C_j fragments read straight from the
+// bank, which every work item of the system reads and which stays in L2.  In a
+// 16-column chunk lane group g takes the contraction indices 4g..4g+3 in the
+// four MFMA steps (the A reads use the same order), so each fragment is one
+// 32-byte load; the fragments of the chunk after next are in flight while a
+// chunk is multiplied, and phase 2 needs no barrier.  Each finished block is
+// written packed with the V P term, all P values loaded before the first
+// store.  Within a vector the longest row tile runs first; consecutive items
+// run on one XCD.  W never goes through HBM (the two-GEMM form writes n x ldc
+// doubles per vector and reads them back once per column tile).
+template <int OC16>
+__global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
+  constexpr int OC = 16 * OC16;
+  constexpr int WS = OC + 1;                                   // row stride of W_I in LDS
+  constexpr int S1 = 2 * (kTM * (kBK + 1) + kBK * WS);         // phase-1 staging
+  constexpr int SW = kTM * WS;
+  __shared__ double smem[S1 > SW ? S1 : SW];
+  typedef double RowA[kTM][kBK + 1];
+  typedef double RowB[kBK][WS];
+  RowA *sA = reinterpret_cast<RowA *>(smem);                   // [2]
+  RowB *sB = reinterpret_cast<RowB *>(smem + 2 * kTM * (kBK + 1));  // [2]
+  double(*sW)[WS] = reinterpret_cast<double(*)[WS]>(smem);     // aliases the staging
+
+  const int64_t nblk = gridDim.x;  // multiple of 8
+  const int64_t l = (int64_t(blockIdx.x) & 7) * (nblk >> 3) + (int64_t(blockIdx.x) >> 3);
+  if (l >= g.items) return;
+  const int64_t J = l / g.T;
+  const int I = g.T - 1 - int(l - J * g.T);
+  const int i0 = I * kTM;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, gq = lane >> 4;
+  const double *__restrict__ cb = g.cb;
+  const int ldc = g.ldc, o = g.o;
+  const double *__restrict__ zj = g.z + J * g.sz;
+  const int ldz = g.ldz;
+
+  // ---- phase 1: W_I = C_I Z^J; wave w owns rows 16w..16w+15, all OC columns
+  v4d acc1[OC16];
+#pragma unroll
+  for (int t = 0; t < OC16; ++t) acc1[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  double ra[4], rb[OC16];
+  uint32_t zok = 0;  // bit it: Z element it lies inside nocc x nocc (zeroed at stash time)
+  auto fetch1 = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int e = tid + kThreads * it, r = e / kBK, kk = e % kBK;
+      ra[it] = cb[uint32_t(i0 + r) * uint32_t(ldc) + uint32_t(k0 + kk)];
+    }
+    zok = 0;
+#pragma unroll
+    for (int it = 0; it < OC16; ++it) {
+      const int e = tid + kThreads * it, kr = e / OC, cc = e % OC, k = k0 + kr;
+      const bool ok = k < o && cc < o;
+      rb[it] = zj[ok ? uint32_t(k) * uint32_t(ldz) + uint32_t(cc) : 0u];
+      zok |= ok ? 1u << it : 0u;
+    }
+  };
+  auto stash1 = [&](int buf) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int e = tid + kThreads * it;
+      sA[buf][e / kBK][e % kBK] = ra[it];
+    }
+#pragma unroll
+    for (int it = 0; it < OC16; ++it) {
+      const int e = tid + kThreads * it;
+      sB[buf][e / OC][e % OC] = ((zok >> it) & 1u) ? rb[it] : 0.0;
+    }
+  };
+  fetch1(0);
+  stash1(0);
+  if (OC16 > 1) fetch1(kBK);
+  __syncthreads();
+#pragma unroll 1
+  for (int c = 0; c < OC16; ++c) {
+    const int cur = c & 1;
+#pragma unroll
+    for (int ks = 0; ks < kBK / 4; ++ks) {
+      const double a = sA[cur][16 * w + (lane & 15)][4 * ks + gq];
+#pragma unroll
+      for (int t = 0; t < OC16; ++t) acc1[t] = mfma64(a, sB[cur][4 * ks + gq][16 * t + (lane & 15)], acc1[t]);
+    }
+    if (c + 1 < OC16) stash1(cur ^ 1);
+    if (c + 2 < OC16) fetch1((c + 2) * kBK);
+    __syncthreads();
+  }
+  // every wave is past its last read of the staging buffers: W_I replaces them
+#pragma unroll
+  for (int t = 0; t < OC16; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sW[16 * w + 4 * r + gq][16 * t + (lane & 15)] = acc1[t][r];
+  __syncthreads();
+
+  // ---- phase 2: the blocks W_I C_j^T, j = 0..I, as one stream of chunks
+  const int wm = w >> 1, wn = w & 1;
+  const int Q = (I + 1) * OC16;
+  const double vj = g.gv * g.v[J];
+  const double *__restrict__ P = g.p;
+  double *__restrict__ G = g.g + J * g.ldg;
+  f4d bq[2][2];  // [stage][jj]: C_j fragments of the chunks in flight
+  auto loadB = [&](int q, f4d (&dst)[2]) {
+    const int j = q / OC16, kc = q - j * OC16;
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const uint32_t row = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
+      dst[jj] = *reinterpret_cast<const f4d *>(cb + row * uint32_t(ldc) + uint32_t(kc * kBK + 4 * gq));
+    }
+  };
+  v4d acc2[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = v4d{0.0, 0.0, 0.0, 0.0};
+  auto epilogue = [&](int j) {
+    double pv[2][2][4];
+    uint32_t pi[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + 4 * r + gq);
+          const uint32_t gj = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
+          const bool ok = gi < uint32_t(g.n) && gi >= gj;
+          pi[i][jj][r] = ok ? gi * (gi + 1u) / 2u + gj : ~0u;
+          pv[i][jj][r] = P[ok ? pi[i][jj][r] : 0u];
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t p = pi[i][jj][r];
+          if (p == ~0u) continue;
+          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + 4 * r + gq);
+          const uint32_t gj = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
+          G[p] = (gi == gj ? 1.0 : 2.0) * (vj * pv[i][jj][r] + g.beta * acc2[i][jj][r]);
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = v4d{0.0, 0.0, 0.0, 0.0};
+  };
+  // One chunk of the stream; the fragment stage S is a compile-time constant
+  // (the loop below is unrolled by two), so the stages never move.
+  auto step = [&](int q, auto stage) {
+    constexpr int S = decltype(stage)::value;
+    const f4d b0 = bq[S][0], b1 = bq[S][1];
+    if (q + 2 < Q) loadB(q + 2, bq[S]);
+    const int j = q / OC16, kc = q - j * OC16;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      double af[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = sW[wm * 32 + i * 16 + (lane & 15)][kc * kBK + 4 * gq + ks];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        acc2[i][0] = mfma64(af[i], b0[ks], acc2[i][0]);
+        acc2[i][1] = mfma64(af[i], b1[ks], acc2[i][1]);
+      }
+    }
+    if (kc == OC16 - 1) epilogue(j);
+  };
+  loadB(0, bq[0]);
+  if (Q > 1) loadB(1, bq[1]);
+  int q = 0;
+  for (; q + 1 < Q; q += 2) {
+    step(q, std::integral_constant<int, 0>{});
+    step(q + 1, std::integral_constant<int, 1>{});
+  }
+  if (q < Q) step(q, std::integral_constant<int, 0>{});
+}
+
+template <int OC16>
+void launch_grad_rows(Context *c, const GradRows &g) {
+  hipLaunchKernelGGL((k_grad_rows<OC16>), dim3(unsigned(round_up(g.items, 8))), dim3(kThreads), 0,
+                     c->stream, g);
+}
+
+// dL of the jn vectors from Z in one fused launch (nocc <= 128); returns
+// MJK_ERR_ARG when nocc is larger (the caller then forms W = C Z first).
+int grad_rows(Context *c, int nocc, int ldc, const double *z, int64_t sz, int ldz, const double *v,
+              const double *P, double alpha, double beta, double *dl, int64_t ldd, int64_t jn) {
+  const int oc16 = (nocc + 15) / 16;
+  if (oc16 > 8 || int64_t(c->n_pad) * ldc >= (int64_t(1) << 28)) return MJK_ERR_ARG;
+  GradRows g{};
+  g.cb = static_cast<const double *>(c->w->cbank.ptr);
+  g.ldc = ldc;
+  g.z = z;
+  g.sz = sz;
+  g.ldz = ldz;
+  g.o = nocc;
+  g.v = v;
+  g.p = P;
+  g.gv = alpha;
+  g.beta = beta;
+  g.g = dl;
+  g.ldg = ldd;
+  g.n = c->n;
+  g.T = int(ceil_div(c->n, kTM));
+  g.items = int64_t(g.T) * jn;
+  switch (oc16) {
+    case 1: launch_grad_rows<1>(c, g); break;
+    case 2: launch_grad_rows<2>(c, g); break;
+    case 3: launch_grad_rows<3>(c, g); break;
+    case 4: launch_grad_rows<4>(c, g); break;
+    case 5: launch_grad_rows<5>(c, g); break;
+    case 6: launch_grad_rows<6>(c, g); break;
+    case 7: launch_grad_rows<7>(c, g); break;
+    default: launch_grad_rows<8>(c, g); break;
+  }
+  return MJK_OK;
+}
+
 // P = 2 C C^T in plain LT storage (the misc workspace) unless the caller gave it.
 int density_or(Context *c, int nocc, const double *corb, const double *p_lt, const double **P) {
   *P = p_lt;
@@ -394,15 +635,17 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
     g.O = Z; g.sO = int64_t(ldc) * ldc; g.ldo = ldc;
     g.m = nocc; g.n = nocc; g.k = n; g.alpha = 1.0;
     if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
-    // W[J] = C Z[J]   (n x o)
-    g = GemmBatch{};
-    g.A = cb; g.sA = 0; g.lda = ldc;
-    g.B = Z; g.sB = int64_t(ldc) * ldc; g.ldb = ldc;
-    g.O = W; g.sO = int64_t(n_pad) * ldc; g.ldo = ldc;
-    g.m = n; g.n = nocc; g.k = nocc; g.alpha = 1.0;
-    if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
-    // dL[J] = s * (alpha V_J P + beta W[J] C^T), lower triangle, packed
-    if ((rc = grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn))) return rc;
+    // dL[J] = s * (alpha V_J P + beta C Z[J] C^T), lower triangle, packed
+    if (grad_rows(c, nocc, ldc, Z, int64_t(ldc) * ldc, ldc, v + j0, P, alpha, beta, dl + j0 * ldd,
+                  ldd, jn) != MJK_OK) {  // nocc > 128: W = C Z first
+      g = GemmBatch{};
+      g.A = cb; g.sA = 0; g.lda = ldc;
+      g.B = Z; g.sB = int64_t(ldc) * ldc; g.ldb = ldc;
+      g.O = W; g.sO = int64_t(n_pad) * ldc; g.ldo = ldc;
+      g.m = n; g.n = nocc; g.k = nocc; g.alpha = 1.0;
+      if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
+      if ((rc = grad_from_w(c, nocc, W, ldc, v + j0, P, alpha, beta, dl + j0 * ldd, ldd, jn))) return rc;
+    }
     phase_end(c, MJK_PHASE_BACKWARD);
   }
   return hip_check(c, hipGetLastError(), "grad_vectors launch");
@@ -423,6 +666,15 @@ extern "C" int mjk_grad_vectors_z(mjk_context *c, int32_t nocc, const double *co
   if ((rc = pack_bank(c, 1, &nocc, sets, off, opad, &ot, &tot))) return rc;
   const int ldc = 16 * ot;
   const double *cb = static_cast<const double *>(c->w->cbank.ptr);
+  if (nocc <= 128) {  // one fused launch: W = C Z stays on chip
+    const double *P = nullptr;
+    if ((rc = density_or(c, nocc, corb, p_lt, &P))) return rc;
+    phase_begin(c);
+    if ((rc = grad_rows(c, nocc, ldc, z, int64_t(nocc) * nocc, nocc, v, P, alpha, beta, dl, ldd, M)))
+      return set_error(c, rc, "grad_vectors_z: fused gradient launch");
+    phase_end(c, MJK_PHASE_BACKWARD);
+    return hip_check(c, hipGetLastError(), "grad_vectors_z launch");
+  }
   // per vector: W = C Z (n_pad x ldc)
   const int64_t per_vec = int64_t(n_pad) * ldc;
   int64_t jc = std::max<int64_t>(1, std::min<int64_t>(M, c->x_budget / (8 * per_vec)));

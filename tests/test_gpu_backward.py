@@ -123,6 +123,32 @@ def test_saved_z_gradient_equals_recompute(gpu, fused):
     jk.close()
 
 
+@pytest.mark.parametrize("n,M,o", [(1, 2, 1), (64, 3, 16), (70, 9, 5), (130, 7, 33), (200, 6, 64),
+                                   (333, 5, 100), (129, 4, 128), (257, 3, 129)])
+def test_vector_gradient_from_z_matches_dense(gpu, n, M, o):
+    """dE/dL from a given Z (mjk_grad_vectors_z) against the dense formula in
+    numpy: G^J = s (alpha V_J P + beta C Z^J C^T), packed lower triangle with
+    s = 1 on the diagonal and 2 off it.  The orbital counts cover every width
+    of the fused kernel (nocc up to 128, in 16-column steps) and the two-GEMM
+    path beyond it (129); bases cover one AO and ragged 64-AO tiles.  Z is an
+    arbitrary (non-symmetric) matrix, so a transposed operand would show."""
+    rng = np.random.default_rng(n + 7 * o)
+    Cn = rng.standard_normal((n, o))
+    Zn = rng.standard_normal((M, o, o))
+    Pn = pack(rng.standard_normal((n, n)))
+    vn = rng.standard_normal(M)
+    alpha, beta = 0.75, -2.0
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(np.zeros((M, npair(n))))
+    got = jk.grad_vectors(torch.tensor(Cn, device=gpu), torch.tensor(vn, device=gpu), alpha, beta,
+                          p_lt=torch.tensor(Pn, device=gpu), z=torch.tensor(Zn, device=gpu)).cpu().numpy()
+    i, j = np.tril_indices(n)
+    s = np.where(i == j, 1.0, 2.0)
+    want = np.stack([s * (alpha * vn[J] * Pn + beta * (Cn @ Zn[J] @ Cn.T)[i, j]) for J in range(M)])
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    jk.close()
+
+
 def test_full_size_backward_n1200_m6000_o150(gpu):
     """Mixed precision plus backward workload: nbas=1200, nchol=6000, nocc=150.
     autograd dE/dC and dE/dL against central differences (step 1e-5) on 32
