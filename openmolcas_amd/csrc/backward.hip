@@ -426,9 +426,11 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = v4d{0.0, 0.0, 0.0, 0.0};
-  auto epilogue = [&](int j) {
-    double pv[2][2][4];
-    uint32_t pi[2][2][4];
+  // P values of the tile in progress, loaded at its first chunk so the
+  // epilogue only stores
+  double pv[2][2][4];
+  uint32_t pi[2][2][4];
+  auto load_p = [&](int j) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -441,6 +443,8 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
           pi[i][jj][r] = ok ? gi * (gi + 1u) / 2u + gj : ~0u;
           pv[i][jj][r] = P[ok ? pi[i][jj][r] : 0u];
         }
+  };
+  auto epilogue = [&](int j) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -465,6 +469,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
     const f4d b0 = bq[S][0], b1 = bq[S][1];
     if (q + 2 < Q) loadB(q + 2, bq[S]);
     const int j = q / OC16, kc = q - j * OC16;
+    if (kc == 0) load_p(j);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       double af[2];
