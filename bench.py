@@ -57,6 +57,10 @@ def parse():
     p.add_argument("--copy-probe", action="store_true",
                    help="after the timed region, copy 1 GiB device to device (calibrates HBM counters)")
     p.add_argument("--cpu-sample-vectors", type=int, default=1 << 30)
+    p.add_argument("--kernel-profile", action="store_true",
+                   help="64-system batch only: set up, autotune, one graph replay, then (after a 0.5 s "
+                        "pause that marks it in a kernel trace) one FP64 forward + backward step with the "
+                        "systems one after another; prints that step's per-phase kernel times")
     return p.parse_args()
 
 
@@ -192,6 +196,21 @@ def ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, t64, res, steps):
     return {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "nchol": int(ctx.nchol),
             "dE": abs(e - e64), "exchange_ms": round(1e3 * t, 4),
             "exchange_speedup_vs_fp64": round(t64 / t, 3)}
+
+
+def batch_traffic(family, launches):
+    """HBM bytes per launch of a kernel family in the 64-system step, from the
+    committed rocprofv3 summary of that step (FETCH_SIZE + WRITE_SIZE summed
+    over the step, calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "batch64_fp64_fwd_bwd.md")
+    if not os.path.exists(path) or not launches:
+        return None
+    try:
+        data = json.loads(open(path).read().split("```json", 1)[1].split("```", 1)[0])
+    except (IndexError, ValueError):
+        return None
+    v = data.get(family)
+    return None if v is None else (v["fetch_bytes"] + v["write_bytes"]) / launches
 
 
 def profiled_traffic(kernel_prefix, n, M, o, precision):
@@ -411,10 +430,15 @@ def batch_workload(args, dev):
     dC64 = [d.clone() for d in batch.dC]
     t_fwd = timed("fp64", False, args.steps, 1)
     fk = batch.flops()
-    # profiled eager step: per-phase kernel time summed over the 64 systems
+    # profiled eager step, systems one after another (no two kernels overlap,
+    # so every launch's HIP events time that kernel alone): per-phase kernel
+    # time summed over the 64 systems
     batch.set_profiling(True)
-    batch.step("fp64")
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.step("fp64", serial=True)
+    torch.cuda.synchronize()
+    t_serial = time.perf_counter() - t0
     phases = batch.phase_times()
     batch.set_profiling(False)
     # algorithmic work per phase of one step
@@ -423,16 +447,19 @@ def batch_workload(args, dev):
     n_ht = [1.0 if z is not None else 2.0 for z in batch.Z]
     ht_f = sum(k * 2.0 * M * o * n * n for k, (n, M, o) in zip(n_ht, shapes))
     sy_f = sum(1.0 * M * o * n * (n + 1) for n, M, o in shapes)          # SYRK, lower triangle
-    # Z = X C (forward, for the backward), W = C Z, and the packed W C^T
-    bw_f = sum(M * (2.0 * n * o * o + 2.0 * n * o * o + 1.0 * n * (n + 1) * o) for n, M, o in shapes)
+    z_f = sum(2.0 * M * n * o * o for n, M, o in shapes)                 # Z = X C (forward)
+    # W = C Z and the packed W C^T (fused), plus Z = X C where it is recomputed
+    bw_f = sum(M * ((2.0 if z is not None else 4.0) * n * o * o + 1.0 * n * (n + 1) * o)
+               for z, (n, M, o) in zip(batch.Z, shapes))
     ph = {
-        "half_transform": ("k_ht<F64> (X = C^T L with the Coulomb V inside)", ht_f,
+        "half_transform": ("k_ht<F64> (X = C^T L with the Coulomb V inside)", "k_ht", ht_f,
                            phases["half_transform"]),
-        "exchange": ("k_syrk<F64> (K = sum X^T X)", sy_f, phases["exchange"]),
-        "backward": ("k_gemm (dE/dL: Z = XC kept by the forward, W = CZ, W C^T packed)", bw_f,
-                     phases["backward"]),
+        "exchange": ("k_syrk<F64> (K = sum X^T X)", "k_syrk", sy_f, phases["exchange"]),
+        "backward": ("k_grad_rows (dE/dL: W = C Z and the packed W C^T in one launch)", "k_grad_rows",
+                     bw_f, phases["backward"]),
+        "zsave": ("k_gemm (Z = X C kept by the forward)", "k_gemm", z_f, phases["zsave"]),
     }
-    name, (label, flops, (ms, calls)) = max(ph.items(), key=lambda kv: kv[1][2][0])
+    name, (label, family, flops, (ms, calls)) = max(ph.items(), key=lambda kv: kv[1][3][0])
     achieved = flops / (ms * 1e-3) / 1e12
     # ladder: per-system split points for |dE| <= 1e-6 Eh, timed the same way
     progress("profiled eager step done")
@@ -459,16 +486,21 @@ def batch_workload(args, dev):
             "achieved": round(achieved, 3), "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_TFLOPS["fp64"], 4),
             "kernel_ms_per_step": round(ms, 3), "launches_per_step": calls,
-            "timing": "HIP events around every launch of one eager step of the same batch (the graph "
-                      "replays the same launches)",
-            "traffic": None,
+            "timing": "HIP events around every launch of one eager step of the same batch with the "
+                      "systems run one after another (the graph replays the same launches, four lanes "
+                      "at a time)",
+            "traffic": batch_traffic(family, calls),
+            "traffic_source": "profiles/batch64_fp64_fwd_bwd.md (FETCH_SIZE + WRITE_SIZE of the same step, "
+                              "calibrated)",
+            "phases_TFs": {k: round(v[2] / (v[3][0] * 1e-3) / 1e12, 2) for k, v in ph.items() if v[3][0] > 0},
         },
         "batch": {
             "fwd_bwd_replay_ms_fp64": round(1e3 * t64, 3),
             "fwd_replay_ms_fp64": round(1e3 * t_fwd, 3),
             "fwd_exchange_TFs_over_fwd_replay": round(fk / t_fwd / 1e12, 2),
             "fwd_exchange_gflop": round(fk / 1e9, 1),
-            "phase_ms_per_step_eager": {k: round(v[0], 3) for k, v in phases.items()},
+            "phase_ms_per_step_eager_serial": {k: round(v[0], 3) for k, v in phases.items()},
+            "eager_serial_step_ms": round(1e3 * t_serial, 3),
             "ladder": {
                 "fwd_bwd_replay_ms": round(1e3 * t_lad, 3),
                 "builds_per_s": round(nsys / t_lad, 4),
@@ -485,6 +517,32 @@ def batch_workload(args, dev):
     del batch
     torch.cuda.empty_cache()
     return out, shapes
+
+
+def kernel_profile(dev):
+    """The batch workload's kernels for a profiler: one serial eager step,
+    preceded by a 0.5 s pause so that a kernel trace can cut it out."""
+    import torch
+
+    from openmolcas_amd.batch import MoleculeBatch, uneven_shapes
+
+    batch = MoleculeBatch(uneven_shapes(64, seed=5), device=dev, seed=3)
+    batch.autotune("fp64")
+    batch.capture("fp64", True)
+    batch.replay()
+    torch.cuda.synchronize()
+    time.sleep(0.5)
+    batch.set_profiling(True)
+    t0 = time.perf_counter()
+    batch.step("fp64", serial=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    phases = batch.phase_times()
+    batch.close()
+    return {"workload": "64-system batch, one FP64 forward + backward eager step, systems one after another",
+            "step_ms": round(1e3 * t, 3),
+            "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
+            "phase_launches": {k: v[1] for k, v in phases.items()}}
 
 
 def single_build(args, dev, steps):
@@ -649,6 +707,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded N(0,1) symmetric vectors, norms 1..1e-6; QR orbitals; h = sym N(0,1)/nbas)",
     }
+    if args.kernel_profile:
+        print(json.dumps(kernel_profile(dev)), flush=True)
+        return
     if args.workload == "batch64":
         head, shapes = batch_workload(args, dev)
         line.update(head)

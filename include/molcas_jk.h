@@ -59,8 +59,9 @@ enum {
   MJK_PHASE_REDUCE = 4,    /* partial-sum reductions            */
   MJK_PHASE_ASSEMBLY = 5,  /* Fock update / energy               */
   MJK_PHASE_PACK = 6,      /* density / orbital staging          */
-  MJK_PHASE_BACKWARD = 7,  /* dE/dL batched GEMMs (Z, W, dL)     */
-  MJK_NUM_PHASES = 8
+  MJK_PHASE_BACKWARD = 7,  /* dE/dL: W = C Z and the packed W C^T (and the Z = X C of a recompute) */
+  MJK_PHASE_ZSAVE = 8,     /* Z = X C kept by a forward exchange for the backward */
+  MJK_NUM_PHASES = 9
 };
 
 /* ---- lifecycle --------------------------------------------------------- */

@@ -23,7 +23,8 @@ PREC_FP32 = 1
 PREC_BF16X3 = 2
 PREC_LADDER = 3
 
-PHASES = ("coulomb_v", "coulomb_j", "half_transform", "exchange", "reduce", "assembly", "pack", "backward")
+PHASES = ("coulomb_v", "coulomb_j", "half_transform", "exchange", "reduce", "assembly", "pack", "backward",
+          "zsave")
 
 _c = ctypes
 _vp = _c.c_void_p

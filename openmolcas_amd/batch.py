@@ -121,15 +121,22 @@ class MoleculeBatch:
         if self.dL[i] is not None:
             c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i], z=Z)
 
-    def step(self, precision="fp64", backward=True):
+    def step(self, precision="fp64", backward=True, serial=False):
         """Forward (+ backward) of every system: forks from the batch stream to
-        the lane streams and joins back (all queued, nothing synchronised)."""
+        the lane streams and joins back (all queued, nothing synchronised).
+        serial=True makes every system wait for the one queued before it, so
+        no two systems' kernels overlap (per-kernel timing)."""
         with torch.cuda.stream(self.stream):
             for st in self.lane_streams:
                 st.wait_stream(self.stream)
+            prev = None
             for i in range(len(self.shapes)):
-                with torch.cuda.stream(self.lane_streams[self.lane_of[i]]):
+                st = self.lane_streams[self.lane_of[i]]
+                if serial and prev is not None and prev is not st:
+                    st.wait_stream(prev)
+                with torch.cuda.stream(st):
                     self._system(i, precision, backward)
+                prev = st
             for st in self.lane_streams:
                 self.stream.wait_stream(st)
 

@@ -1305,7 +1305,7 @@ int exchange_segments(Context *c, int32_t nk, const int32_t *nocc, const double 
         if ((rc = z_from_x(c, c->w->x.ptr, s.prec != MJK_PREC_FP64, ldc, nocc[0], jn,
                            c->zs.z + j0 * int64_t(nocc[0]) * nocc[0])))
           return rc;
-        phase_end(c, MJK_PHASE_BACKWARD);
+        phase_end(c, MJK_PHASE_ZSAVE);
       }
       if (screen && tau > 0.0) {
         double *xn2 = static_cast<double *>(c->w->xn2.ptr);
