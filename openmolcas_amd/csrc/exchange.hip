@@ -990,10 +990,12 @@ inline int ht_tile(const Context *c, int ot) {
 template <class P, int OT>
 void launch_ht_w(Context *c, bool wide, const typename P::T *L, const typename P::T *cb,
                  int64_t j0, int64_t jn, typename P::T *X) {
+  // 8-row chunks where the staging pieces divide, else 16 (odd OT)
+  constexpr int BK = (8 * 16 * OT) % kThreads == 0 ? 8 : 16;
   if constexpr (std::is_same<P, F32>::value) {
-    if (wide) return launch_ht<P, OT, 128, 8>(c, L, cb, j0, jn, X);
+    if (wide) return launch_ht<P, OT, 128, BK>(c, L, cb, j0, jn, X);
   }
-  launch_ht<P, OT, 64, 8>(c, L, cb, j0, jn, X);
+  launch_ht<P, OT, 64, BK>(c, L, cb, j0, jn, X);
 }
 
 template <class P>
@@ -1008,7 +1010,9 @@ void ht_dispatch(Context *c, int ot, const typename P::T *L, const typename P::T
     case 2: wide ? launch_ht<P, 2, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 2, 64, 8>(c, L, cb, j0, jn, X); break;
     case 3: wide ? launch_ht<P, 3, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 3, 64, 16>(c, L, cb, j0, jn, X); break;
     case 4: wide ? launch_ht<P, 4, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 4, 64, 8>(c, L, cb, j0, jn, X); break;
+    case 5: wide ? launch_ht<P, 5, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 5, 64, 16>(c, L, cb, j0, jn, X); break;
     case 6: wide ? launch_ht<P, 6, 128, 16>(c, L, cb, j0, jn, X) : launch_ht<P, 6, 64, 8>(c, L, cb, j0, jn, X); break;
+    case 7: launch_ht_w<P, 7>(c, wide, L, cb, j0, jn, X); break;
     case 8: launch_ht_w<P, 8>(c, wide, L, cb, j0, jn, X); break;
     case 10: launch_ht_w<P, 10>(c, wide, L, cb, j0, jn, X); break;
     case 12: launch_ht_w<P, 12>(c, wide, L, cb, j0, jn, X); break;
@@ -1048,7 +1052,9 @@ void ht_dispatch_b3(Context *c, int ot, const float *L, const __bf16 *ch, const 
     case 2: wide ? launch_ht_b3<2, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<2, 64>(c, L, ch, cl, j0, jn, X); break;
     case 3: wide ? launch_ht_b3<3, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<3, 64>(c, L, ch, cl, j0, jn, X); break;
     case 4: wide ? launch_ht_b3<4, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<4, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 5: wide ? launch_ht_b3<5, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<5, 64>(c, L, ch, cl, j0, jn, X); break;
     case 6: wide ? launch_ht_b3<6, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<6, 64>(c, L, ch, cl, j0, jn, X); break;
+    case 7: wide ? launch_ht_b3<7, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<7, 64>(c, L, ch, cl, j0, jn, X); break;
     case 8: wide ? launch_ht_b3<8, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<8, 64>(c, L, ch, cl, j0, jn, X); break;
     case 10: wide ? launch_ht_b3<10, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<10, 64>(c, L, ch, cl, j0, jn, X); break;
     case 12: wide ? launch_ht_b3<12, 128>(c, L, ch, cl, j0, jn, X) : launch_ht_b3<12, 64>(c, L, ch, cl, j0, jn, X); break;
@@ -1103,7 +1109,7 @@ int bank_layout(Context *c, int32_t nk, const int32_t *nocc, std::vector<int> &o
   *tot_out = tot;
   *ot_out = 0;
   if (tot == 0) return MJK_OK;
-  static const int kOT[] = {1, 2, 3, 4, 6, 8, 10, 12, 14, 16};
+  static const int kOT[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16};
   for (int v : kOT)
     if (16 * v >= tot) {
       *ot_out = v;

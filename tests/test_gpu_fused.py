@@ -64,7 +64,8 @@ def check(jk, L, dlt, Cs, n):
 
 
 @pytest.mark.parametrize("n,M,occs", [(37, 21, (5,)), (129, 17, (1,)), (200, 33, (40, 9)),
-                                      (70, 9, (190,)), (256, 40, (64,)), (730, 7, (20,))])
+                                      (70, 9, (190,)), (256, 40, (64,)), (730, 7, (20,)),
+                                      (150, 13, (70,)), (170, 11, (100,))])
 def test_fused_matches_oracle(gpu, n, M, occs):
     L = chol_vectors_np(n, M, seed=n + M)
     Cs = [orbitals_np(n, o, seed=o + 3) for o in occs]

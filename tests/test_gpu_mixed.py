@@ -25,7 +25,7 @@ def oracle_K(L, C):
 
 
 @pytest.mark.parametrize("n,M,occs", [(64, 33, [16, 1]), (200, 40, [64, 13]), (300, 64, [150]),
-                                      (160, 30, [120, 80])])
+                                      (160, 30, [120, 80]), (150, 21, [70]), (170, 17, [100])])
 def test_exchange_fp32_matches_fp64_oracle(gpu, n, M, occs):
     L = chol_vectors_np(n, M, seed=3 * n + M)
     Cs = [orbitals_np(n, o, seed=500 + i) for i, o in enumerate(occs)]
@@ -55,7 +55,7 @@ def test_fp32_shadow_refreshed_after_upload(gpu):
 
 
 @pytest.mark.parametrize("n,M,occs", [(64, 33, [16, 1]), (200, 40, [64, 13]), (300, 64, [150]),
-                                      (160, 30, [120, 80])])
+                                      (160, 30, [120, 80]), (150, 21, [70]), (170, 17, [100])])
 def test_exchange_bf16x3_matches_fp64_oracle(gpu, n, M, occs):
     L = chol_vectors_np(n, M, seed=5 * n + M)
     Cs = [orbitals_np(n, o, seed=700 + i) for i, o in enumerate(occs)]

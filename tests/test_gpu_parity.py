@@ -84,6 +84,9 @@ def test_coulomb_matches_oracle(gpu, n, M, nd):
     (129, 40, [33]),
     (200, 25, [64, 13, 7, 2]),
     (300, 12, [150]),
+    (150, 21, [70]),        # 80-column bank
+    (140, 15, [40, 36]),    # 80-column bank, two sets
+    (170, 17, [100]),       # 112-column bank
 ])
 def test_exchange_matches_oracle(gpu, n, M, occs):
     L = chol_vectors_np(n, M, seed=7 * n + M)
