@@ -211,12 +211,48 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     // Element i of a thread: "row-major" chunks (b > a and diagonal blocks)
     // walk e -> (bb = e / TA, aa = e % TA); "column" chunks (b < a, where the
     // packed row a holds b contiguous) walk e -> (bb = e % BK, aa = e / BK).
+    // Chunks wholly inside the matrix and wholly below (column chunks) or above
+    // the diagonal block (uniform per workgroup) step each thread's packed
+    // offsets instead of forming tri0 per element: a thread's elements lie DA
+    // columns (column chunks) or DB rows apart, and tri0(x + d) - tri0(x) =
+    // d x + d (d + 1) / 2.  Diagonal and edge chunks take the general path.
+    constexpr int DA = kThreads / BK, DB = kThreads / TA;
+    static_assert(kThreads % BK == 0 && kThreads % TA == 0, "element walk");
     auto fetch = [&](int b0) {
       const bool colmajor = b0 + BK <= a0;
+      const bool inside = b0 + BK <= n && a0 + TA <= n;
       const T *__restrict__ cs = Cb + int64_t(b0) * LDC;
 #pragma unroll
       for (int i = 0; i < CV; ++i)
         rc[i] = *reinterpret_cast<const VT *>(cs + (tid + kThreads * i) * VW);
+      if (inside && colmajor) {  // packed row a = a0 + tid/BK + DA i holds b0 + tid%BK
+        uint32_t a = uint32_t(a0 + tid / BK);
+        uint32_t idx = tri0u(a) + uint32_t(b0 + tid % BK);
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+#pragma unroll
+          for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx];
+          idx += uint32_t(DA) * a + uint32_t(DA * (DA + 1) / 2);
+          a += uint32_t(DA);
+        }
+        lok = LN == 32 ? ~0u : (1u << LN) - 1u;
+        return;
+      }
+      if (inside && b0 >= a0 + TA) {  // packed row b = b0 + tid/TA + DB i at column a0 + tid%TA
+        uint32_t b = uint32_t(b0 + tid / TA);
+        uint32_t idx = tri0u(b) + uint32_t(a0 + tid % TA);
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+#pragma unroll
+          for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx];
+          if constexpr (FV) rd[i] = fv.p[idx];
+          idx += uint32_t(DB) * b + uint32_t(DB * (DB + 1) / 2);
+          b += uint32_t(DB);
+        }
+        lok = LN == 32 ? ~0u : (1u << LN) - 1u;
+        if constexpr (FV) vok = lok;
+        return;
+      }
 #pragma unroll
       for (int i = 0; i < LN; ++i) {
         const int e = tid + kThreads * i;
@@ -690,7 +726,38 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
           rcl[i] = *reinterpret_cast<const uint4 *>(CbL + src);
         }
       }
-      if (b0 + kB3K <= a0) {  // every b < a
+      // chunks wholly inside the matrix step each thread's packed offsets
+      // (tri0(x + d) - tri0(x) = d x + d (d + 1) / 2) instead of forming tri0
+      // per element; diagonal and edge chunks take the general path
+      const bool inside = b0 + kB3K <= n && a0 + TA <= n;
+      constexpr int GA = kThreads / (kB3K / 4);  // column step between a thread's groups
+      constexpr int GB = 4 * (kThreads / TA);     // row step between a thread's groups
+      if (inside && b0 + kB3K <= a0) {  // packed row a holds b0 + 4q .. contiguous
+        uint32_t a = uint32_t(a0 + tid / (kB3K / 4));
+        uint32_t off = tri0u(a) + uint32_t(b0 + 4 * (tid % (kB3K / 4)));
+#pragma unroll
+        for (int i = 0; i < LG; ++i) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) rl[i][k] = Lj[off + uint32_t(k)];
+          lo[i] = (tid / (kB3K / 4) + GA * i) * kB3S + 4 * (tid % (kB3K / 4));
+          off += uint32_t(GA) * a + uint32_t(GA * (GA + 1) / 2);
+          a += uint32_t(GA);
+        }
+      } else if (inside && b0 >= a0 + TA) {  // packed rows b, b+1, b+2, b+3 at column a
+        const uint32_t a = uint32_t(a0 + tid % TA);
+        uint32_t b = uint32_t(b0 + 4 * (tid / TA));
+#pragma unroll
+        for (int i = 0; i < LG; ++i) {
+          uint32_t idx = tri0u(b) + a;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            rl[i][k] = Lj[idx];
+            idx += b + uint32_t(k) + 1u;
+          }
+          lo[i] = (tid % TA) * kB3S + 4 * (tid / TA + (GB / 4) * i);
+          b += uint32_t(GB);
+        }
+      } else if (b0 + kB3K <= a0) {  // every b < a
 #pragma unroll
         for (int i = 0; i < LG; ++i) {
           const int g = tid + kThreads * i, q = g % (kB3K / 4), aa = g / (kB3K / 4), a = a0 + aa;
