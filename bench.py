@@ -193,9 +193,15 @@ def ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, t64, res, steps):
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / steps
     e = energy_of(ctx, dlt, h, J, K)
+    r32 = t64 / (res["fp32"]["exchange_ms"] * 1e-3)
+    rb3 = t64 / (res["bf16x3"]["exchange_ms"] * 1e-3)
     return {"j_fp32": plan.j_fp32, "j_bf16": plan.j_bf16, "nchol": int(ctx.nchol),
             "dE": abs(e - e64), "exchange_ms": round(1e3 * t, 4),
-            "exchange_speedup_vs_fp64": round(t64 / t, 3)}
+            "exchange_speedup_vs_fp64": round(t64 / t, 3),
+            # the FP64 head's cap (free reduced segments) and the Amdahl bound at
+            # the measured whole-build FP32 / split-BF16 rates
+            "speedup_cap_fp64_head": round(ladder.speedup_bound(plan), 3),
+            "speedup_bound_at_mode_rates": round(ladder.speedup_bound(plan, None, r32, rb3), 3)}
 
 
 def batch_traffic(family, launches):
@@ -404,6 +410,7 @@ def batch_workload(args, dev):
     eager step for the per-kernel roofline."""
     import torch
 
+    from openmolcas_amd import ladder
     from openmolcas_amd.batch import MoleculeBatch, uneven_shapes
 
     shapes = uneven_shapes(64, seed=5)
@@ -508,6 +515,10 @@ def batch_workload(args, dev):
                 "max_abs_dE_Eh": float(torch.max(dE).item()),
                 "max_abs_dEdC_err": dCerr,
                 "fp64_vector_fraction": round(sum(p.j_fp32 for p in plans) / m_tot, 4),
+                # exchange work is n^2 o per vector: the FP64 head's cap with the
+                # reduced-precision segments free
+                "exchange_speedup_cap_fp64_head": round(ladder.speedup_bound(
+                    plans, [n * n * o for n, _, o in shapes]), 3),
                 "fp32_vector_fraction": round(sum(p.j_bf16 - p.j_fp32 for p in plans) / m_tot, 4),
                 "tol_Eh": 1e-6,
             },

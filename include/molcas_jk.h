@@ -290,6 +290,28 @@ int mjk_cho_fmcscf(mjk_context *ctx, double *const *flt, int32_t nforb, int32_t 
                    const double *porb3, double *w_pwxy, const double *cmo,
                    int32_t nfro, int32_t norb, double exfac, int32_t precision);
 
+/* CHO_FMCSCF (src/fock_util/cho_fmcscf.F90:14) with point-group symmetry
+ * (nsym irreps, basis ordered irrep by irrep, nbas[s] functions each):
+ * flt[d], dlt[d] hold the irrep LT blocks back to back (FLT(d)%SB(s)%A1,
+ * DLT(d)%SB(s)%A1); nforb, niorb, naorb, nchm, nfro, norb are the per-irrep
+ * counts of the reference's 8-element arrays; porb1/porb2/porb3 are
+ * POrb(1..3)%SB(s)%A2(nk(s), nBas(s)) back to back with nk = nforb + niorb,
+ * nchm, naorb; cmo is CMO%SB(s)%A2(nBas(s), nBas(s)) back to back and p of
+ * irrep s runs over its norb[s] MOs after nfro[s].  w_pwxy receives (pw|xy)
+ * in the symmetry-blocked order of src/fock_util/cho_eval_waxy.F90:86-141
+ * (blocks sp, sw, sx <= sy with sp x sw = sx x sy; p fastest, then w, then
+ * the xy pair, LT within a diagonal block).  The vectors (every JSYM) must be
+ * in the context with reduced-set maps into the C1 triangle; the build runs
+ * on the C1 engine with the irrep blocks embedded.  Called from
+ * src/fock_util/cho_cas_drv.F90:277 when nSym > 1. */
+int mjk_cho_fmcscf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, double *const *flt,
+                       const int32_t *nforb, const int32_t *niorb, const int32_t *naorb,
+                       double factxi, const double *const *dlt, int32_t do_active,
+                       const double *porb1, const int32_t *nchm, const double *porb2,
+                       const double *porb3, double *w_pwxy, const double *cmo,
+                       const int32_t *nfro, const int32_t *norb, double exfac,
+                       int32_t precision);
+
 /* CD_InCore (src/cholesky_util/cd_incore.F90:35-60, cd_incore_1.F90:14-105)
  * on the device: pivoted incomplete Cholesky of the n x n matrix x
  * (column-major device array, destroyed) into vec (n x mxvec, column-major
@@ -319,6 +341,23 @@ int mjk_cho_lk_casscf(mjk_context *ctx, const double *const *dlt, double *const 
                       int32_t do_active, int32_t nscreen, double dmpk, double dfmat,
                       const double *cmo, int32_t nfro, int32_t norb, double exfac,
                       int32_t precision);
+
+/* CHO_LK_CASSCF (src/fock_util/cho_lk_casscf.F90:14) with point-group
+ * symmetry, unscreened (LK screening on with nsym > 1 returns MJK_ERR_ARG:
+ * the reference screens per irrep shell block).  Storage as
+ * mjk_cho_fmcscf_sym: flt[d], dlt[d] irrep LT blocks back to back; msq and
+ * ash2 are MSQ%SB(s)%A2 and CVa(2)%SB(s)%A2(nBas(s), nBas(s)) back to back
+ * (AO index fastest, the first nchi[s] / nchm[s] columns used); ash1 is
+ * CVa(1)%SB(s)%A2(naorb[s], nBas(s)) back to back; cmo, nfro, norb and
+ * w_pwxy as for mjk_cho_fmcscf_sym.  Called from
+ * src/fock_util/cho_cas_drv.F90:285 when nSym > 1. */
+int mjk_cho_lk_casscf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas,
+                          const double *const *dlt, double *const *flt, const double *msq,
+                          double *w_pwxy, double factxi, const int32_t *nchi,
+                          const int32_t *naorb, const int32_t *nchm, const double *ash1,
+                          const double *ash2, int32_t do_active, int32_t nscreen, double dmpk,
+                          double dfmat, const double *cmo, const int32_t *nfro,
+                          const int32_t *norb, double exfac, int32_t precision);
 
 /* ---- device engine (device pointers, asynchronous on the context stream) */
 

@@ -60,6 +60,11 @@ SIGNATURES = {
                                   _i32, _vp, _i32]),
     "mjk_cho_fmcscf": (_c.c_int, [_vp, _vp, _i32, _i32, _i32, _c.c_double, _vp, _i32, _vp, _i32,
                                   _vp, _vp, _vp, _vp, _i32, _i32, _c.c_double, _i32]),
+    "mjk_cho_fmcscf_sym": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _vp, _i32,
+                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
+    "mjk_cho_lk_casscf_sym": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _vp, _vp,
+                                         _vp, _vp, _vp, _i32, _i32, _c.c_double, _c.c_double, _vp,
+                                         _vp, _vp, _c.c_double, _i32]),
     "mjk_coulomb": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp]),
     "mjk_coulomb_noalloc": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp]),
     "mjk_exchange": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _i64, _i32]),

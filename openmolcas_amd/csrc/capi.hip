@@ -1139,6 +1139,115 @@ static int fmcscf_core(mjk_context *c, double *const *flt, int32_t nforb, int32_
   return hip_check(c, hipStreamSynchronize(c->stream), "cho_fmcscf sync");
 }
 
+namespace {
+// CASSCF drop-ins with point-group symmetry on the C1 engine: FI/FA and DI/DA
+// irrep blocks, the orbital sets, the active MOs and CMO are embedded in the
+// full basis, the C1 build runs, and FI/FA and (pw|xy) are read back into the
+// irrep layout.  Orbital counts are concatenated irrep by irrep: irrep s's k-th
+// orbital of a set with per-irrep offsets ok is the set's orbital ok[s] + k.
+struct CasSym {
+  SymMap m;
+  int64_t oa[9] = {0}, oo[9] = {0};  // active-orbital and MO offsets per irrep
+  int64_t NA = 0, NO = 0;
+};
+
+// A2(nk(s), nBas(s)) blocks back to back (orbital index fastest) -> the
+// orbital-major Nk x n matrix of the full basis
+std::vector<double> embed_rows(const SymMap &m, const double *src, const int64_t *ok) {
+  const int64_t Nk = ok[m.nsym], n = m.off[m.nsym];
+  std::vector<double> dst(size_t(std::max<int64_t>(Nk, 1) * n), 0.0);
+  if (!src) return dst;
+  const double *ps = src;
+  for (int s = 0; s < m.nsym && Nk > 0; ++s) {
+    const int64_t nk = ok[s + 1] - ok[s], nb = m.off[s + 1] - m.off[s];
+    for (int64_t b = 0; b < nb; ++b)
+      for (int64_t k = 0; k < nk; ++k) dst[size_t((ok[s] + k) + Nk * (m.off[s] + b))] = ps[k + nk * b];
+    ps += nk * nb;
+  }
+  return dst;
+}
+
+// square A2(nBas(s), nBas(s)) blocks back to back (AO index fastest), their
+// first ok[s+1] - ok[s] columns -> columns ok[s].. of the full n x n square
+std::vector<double> embed_cols(const SymMap &m, const double *src, const int64_t *ok) {
+  const int64_t n = m.off[m.nsym];
+  std::vector<double> dst(size_t(n * n), 0.0);
+  if (!src) return dst;
+  const double *ps = src;
+  for (int s = 0; s < m.nsym; ++s) {
+    const int64_t nk = ok[s + 1] - ok[s], nb = m.off[s + 1] - m.off[s];
+    for (int64_t k = 0; k < nk; ++k)
+      for (int64_t a = 0; a < nb; ++a) dst[size_t((m.off[s] + a) + n * (ok[s] + k))] = ps[a + nb * k];
+    ps += nb * nb;
+  }
+  return dst;
+}
+
+int cas_sym_setup(mjk_context *c, int32_t nsym, const int32_t *nbas, const int32_t *naorb,
+                  const int32_t *nfro, const int32_t *norb, CasSym *cs, const char *who) {
+  int rc;
+  if ((rc = sym_map(c, nsym, nbas, &cs->m, who))) return rc;
+  if (!naorb || !nfro || !norb) return set_error(c, MJK_ERR_ARG, std::string(who) + ": null pointer");
+  for (int s = 0; s < nsym; ++s) {
+    if (naorb[s] < 0 || naorb[s] > nbas[s] || nfro[s] < 0 || norb[s] < 0 || nfro[s] + norb[s] > nbas[s])
+      return set_error(c, MJK_ERR_ARG, std::string(who) + ": bad orbital count");
+    cs->oa[s + 1] = cs->oa[s] + naorb[s];
+    cs->oo[s + 1] = cs->oo[s] + norb[s];
+  }
+  cs->NA = cs->oa[nsym];
+  cs->NO = cs->oo[nsym];
+  return MJK_OK;
+}
+
+// CMO%SB(s)%A2(nBas(s), nBas(s)) back to back: columns nFro(s) ..
+// nFro(s) + nOrb(s) - 1 of irrep s become columns oo(s).. of the C1 CMO(n, n)
+std::vector<double> embed_cmo(const CasSym &cs, const double *cmo, const int32_t *nfro) {
+  const SymMap &m = cs.m;
+  const int64_t n = m.off[m.nsym];
+  std::vector<double> cf(size_t(n * n), 0.0);
+  if (!cmo) return cf;
+  const double *ps = cmo;
+  for (int s = 0; s < m.nsym; ++s) {
+    const int64_t nb = m.off[s + 1] - m.off[s];
+    for (int64_t p = 0; p < cs.oo[s + 1] - cs.oo[s]; ++p)
+      for (int64_t a = 0; a < nb; ++a) cf[size_t((m.off[s] + a) + n * (cs.oo[s] + p))] = ps[a + nb * (nfro[s] + p)];
+    ps += nb * nb;
+  }
+  return cf;
+}
+
+// (pw|xy) of the embedded active space, wf[npw * (Y(Y+1)/2 + X) + P + NO W],
+// into the symmetry-blocked order of CHO_eval_waxy
+// (src/fock_util/cho_eval_waxy.F90:86-141): blocks (sp, sw, sx <= sy) with
+// sp x sw = sx x sy, stacked in the loop order sp, sw, sy; inside a block p
+// fastest, then w, then the xy pair: y(y+1)/2 + x on a diagonal block, x +
+// nA(sx) y otherwise.  Irreps are 0-based here, so the D2h product is XOR.
+void waxy_blocks(const CasSym &cs, const int32_t *naorb, const int32_t *norb, const double *wf,
+                 double *w_pwxy) {
+  const int nsym = cs.m.nsym;
+  const int64_t npw = cs.NO * cs.NA;
+  int64_t stack = 0;
+  for (int sp = 0; sp < nsym; ++sp)
+    for (int sw = 0; sw < nsym; ++sw)
+      for (int sy = 0; sy < nsym; ++sy) {
+        const int sx = sp ^ sw ^ sy;
+        if (sx > sy) continue;
+        const int64_t nxs = naorb[sx], nys = naorb[sy], np_ = norb[sp], nw = naorb[sw];
+        const int64_t pairs = sx == sy ? nys * (nys + 1) / 2 : nxs * nys;
+        for (int64_t y = 0; y < nys; ++y)
+          for (int64_t x = 0; x < (sx == sy ? y + 1 : nxs); ++x) {
+            const int64_t ixy = sx == sy ? y * (y + 1) / 2 + x : x + nxs * y;
+            const int64_t X = cs.oa[sx] + x, Y = cs.oa[sy] + y;  // X <= Y
+            const double *src = wf + npw * (Y * (Y + 1) / 2 + X);
+            double *dst = w_pwxy + stack + np_ * nw * ixy;
+            for (int64_t w = 0; w < nw; ++w)
+              for (int64_t p = 0; p < np_; ++p) dst[p + np_ * w] = src[(cs.oo[sp] + p) + cs.NO * (cs.oa[sw] + w)];
+          }
+        stack += np_ * nw * pairs;
+      }
+}
+}  // namespace
+
 extern "C" {
 
 int mjk_cho_fmcscf(mjk_context *c, double *const *flt, int32_t nforb, int32_t niorb, int32_t naorb,
@@ -1150,6 +1259,55 @@ int mjk_cho_fmcscf(mjk_context *c, double *const *flt, int32_t nforb, int32_t ni
   if (!c) return MJK_ERR_ARG;
   return fmcscf_core(c, flt, nforb, niorb, naorb, factxi, -0.5, dlt, do_active, porb1, nchm, porb2,
                      porb3, w_pwxy, cmo, nfro, norb, precision);  // FactXA = -1/2, cho_fmcscf.F90:74
+}
+
+int mjk_cho_fmcscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, double *const *flt,
+                       const int32_t *nforb, const int32_t *niorb, const int32_t *naorb,
+                       double factxi, const double *const *dlt, int32_t do_active,
+                       const double *porb1, const int32_t *nchm, const double *porb2,
+                       const double *porb3, double *w_pwxy, const double *cmo,
+                       const int32_t *nfro, const int32_t *norb, double exfac, int32_t precision) {
+  if (!c) return MJK_ERR_ARG;
+  if (!flt || !dlt || !nforb || !niorb || !nchm)
+    return set_error(c, MJK_ERR_ARG, "cho_fmcscf_sym: null pointer");
+  CasSym cs;
+  int rc;
+  if ((rc = cas_sym_setup(c, nsym, nbas, naorb, nfro, norb, &cs, "cho_fmcscf_sym"))) return rc;
+  const SymMap &m = cs.m;
+  const int64_t np = c->np;
+  int64_t oi[9] = {0}, oc[9] = {0};
+  for (int s = 0; s < nsym; ++s) {
+    if (nforb[s] < 0 || niorb[s] < 0 || nforb[s] + niorb[s] > nbas[s] || nchm[s] < 0 || nchm[s] > nbas[s])
+      return set_error(c, MJK_ERR_ARG, "cho_fmcscf_sym: bad orbital count");
+    oi[s + 1] = oi[s] + nforb[s] + niorb[s];
+    oc[s + 1] = oc[s] + (do_active ? nchm[s] : 0);
+  }
+  const int nd = do_active ? 2 : 1;
+  for (int d = 0; d < nd; ++d)
+    if (!flt[d] || !dlt[d]) return set_error(c, MJK_ERR_ARG, "cho_fmcscf_sym: null FLT(d) or DLT(d)");
+  if ((oi[nsym] > 0 && !porb1) || (oc[nsym] > 0 && !porb2) || (cs.NA > 0 && (!porb3 || !w_pwxy || !cmo)))
+    return set_error(c, MJK_ERR_ARG, "cho_fmcscf_sym: null orbital array");
+  std::vector<double> ff(size_t(2 * np), 0.0), df(size_t(2 * np), 0.0);
+  m.each([&](int64_t t, int64_t g) {
+    for (int d = 0; d < nd; ++d) {
+      ff[size_t(d * np + g)] = flt[d][t];
+      df[size_t(d * np + g)] = dlt[d][t];
+    }
+  });
+  const std::vector<double> p1 = embed_rows(m, porb1, oi), p2 = embed_rows(m, porb2, oc),
+                            p3 = embed_rows(m, porb3, cs.oa), cf = embed_cmo(cs, cmo, nfro);
+  std::vector<double> wf(size_t(std::max<int64_t>(cs.NO * cs.NA * (cs.NA * (cs.NA + 1) / 2), 1)), 0.0);
+  double *fp[2] = {ff.data(), ff.data() + np};
+  const double *dp[2] = {df.data(), df.data() + np};
+  if ((rc = mjk_cho_fmcscf(c, fp, 0, int32_t(oi[nsym]), int32_t(cs.NA), factxi, dp, do_active, p1.data(),
+                           int32_t(oc[nsym]), p2.data(), p3.data(), cs.NA > 0 ? wf.data() : nullptr,
+                           cf.data(), 0, int32_t(cs.NO), exfac, precision)))
+    return rc;
+  m.each([&](int64_t t, int64_t g) {
+    for (int d = 0; d < nd; ++d) flt[d][t] = ff[size_t(d * np + g)];
+  });
+  if (cs.NA > 0) waxy_blocks(cs, naorb, norb, wf.data(), w_pwxy);
+  return MJK_OK;
 }
 
 int mjk_cho_lk_casscf(mjk_context *c, const double *const *dlt, double *const *flt, const double *msq,
@@ -1188,6 +1346,59 @@ int mjk_cho_lk_casscf(mjk_context *c, const double *const *dlt, double *const *f
   return fmcscf_core(c, flt, 0, nfiorb, naorb, factxi * exfac, -0.5 * exfac, dlt, do_active,
                      p1.data(), nchm, p2.data(), ash1, w_pwxy, cmo, nfro, norb, precision,
                      c->lk_on ? tau : nullptr, c->lk_on ? thrv : nullptr);
+}
+
+int mjk_cho_lk_casscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, const double *const *dlt,
+                          double *const *flt, const double *msq, double *w_pwxy, double factxi,
+                          const int32_t *nchi, const int32_t *naorb, const int32_t *nchm,
+                          const double *ash1, const double *ash2, int32_t do_active, int32_t nscreen,
+                          double dmpk, double dfmat, const double *cmo, const int32_t *nfro,
+                          const int32_t *norb, double exfac, int32_t precision) {
+  if (!c) return MJK_ERR_ARG;
+  if (!flt || !dlt || !nchi || !nchm) return set_error(c, MJK_ERR_ARG, "cho_lk_casscf_sym: null pointer");
+  if (c->lk_on && nsym > 1)  // the reference screens per irrep shell blocks
+    return set_error(c, MJK_ERR_ARG, "cho_lk_casscf_sym: LK screening with nSym > 1 is not supported");
+  CasSym cs;
+  int rc;
+  if ((rc = cas_sym_setup(c, nsym, nbas, naorb, nfro, norb, &cs, "cho_lk_casscf_sym"))) return rc;
+  const SymMap &m = cs.m;
+  const int64_t np = c->np;
+  int64_t oi[9] = {0}, oc[9] = {0};
+  for (int s = 0; s < nsym; ++s) {
+    if (nchi[s] < 0 || nchi[s] > nbas[s] || nchm[s] < 0 || nchm[s] > nbas[s])
+      return set_error(c, MJK_ERR_ARG, "cho_lk_casscf_sym: bad orbital count");
+    oi[s + 1] = oi[s] + nchi[s];
+    oc[s + 1] = oc[s] + (do_active ? nchm[s] : 0);
+  }
+  const int nd = do_active ? 2 : 1;
+  for (int d = 0; d < nd; ++d)
+    if (!flt[d] || !dlt[d]) return set_error(c, MJK_ERR_ARG, "cho_lk_casscf_sym: null FLT(d) or DLT(d)");
+  if ((oi[nsym] > 0 && !msq) || (oc[nsym] > 0 && !ash2) || (cs.NA > 0 && (!ash1 || !w_pwxy || !cmo)))
+    return set_error(c, MJK_ERR_ARG, "cho_lk_casscf_sym: null orbital array");
+  std::vector<double> ff(size_t(2 * np), 0.0), df(size_t(2 * np), 0.0);
+  m.each([&](int64_t t, int64_t g) {
+    for (int d = 0; d < nd; ++d) {
+      ff[size_t(d * np + g)] = flt[d][t];
+      df[size_t(d * np + g)] = dlt[d][t];
+    }
+  });
+  // MSQ%SB(s) and CVa(2)%SB(s) are square A2(nBas(s),nBas(s)), AO index
+  // fastest; CVa(1)%SB(s) is A2(nAorb(s),nBas(s))
+  const std::vector<double> sq1 = embed_cols(m, msq, oi), sq2 = embed_cols(m, do_active ? ash2 : nullptr, oc),
+                            a1 = embed_rows(m, ash1, cs.oa), cf = embed_cmo(cs, cmo, nfro);
+  std::vector<double> wf(size_t(std::max<int64_t>(cs.NO * cs.NA * (cs.NA * (cs.NA + 1) / 2), 1)), 0.0);
+  double *fp[2] = {ff.data(), ff.data() + np};
+  const double *dp[2] = {df.data(), df.data() + np};
+  if ((rc = mjk_cho_lk_casscf(c, dp, fp, sq1.data(), cs.NA > 0 ? wf.data() : nullptr, factxi,
+                              int32_t(oi[nsym]), int32_t(cs.NA), int32_t(oc[nsym]), a1.data(), sq2.data(),
+                              do_active, nscreen, dmpk, dfmat, cf.data(), 0, int32_t(cs.NO), exfac,
+                              precision)))
+    return rc;
+  m.each([&](int64_t t, int64_t g) {
+    for (int d = 0; d < nd; ++d) flt[d][t] = ff[size_t(d * np + g)];
+  });
+  if (cs.NA > 0) waxy_blocks(cs, naorb, norb, wf.data(), w_pwxy);
+  return MJK_OK;
 }
 
 }  // extern "C"

@@ -359,6 +359,93 @@ class JKContext:
         self._check(rc)
         return rc
 
+    def cho_fmcscf_sym(self, nbas, flt, nforb, niorb, naorb, factxi: float, dlt,
+                       do_active: bool, porb1, nchm, porb2, porb3, w_pwxy, cmo, nfro, norb,
+                       exfac: float = 1.0, precision="fp64") -> int:
+        """CHO_FMCSCF with point-group symmetry (nSym = len(nbas) irreps, basis
+        ordered irrep by irrep).
+
+        flt, dlt: (2, ntri) or [FI, FA] / [DI, DA], the irrep LT blocks back to
+        back (flt updated in place).  nforb, niorb, naorb, nchm, nfro, norb:
+        per-irrep counts.  porb1[s], porb2[s], porb3[s]: (nbas[s], nk(s)) AO x
+        orbital matrices (POrb(i)%SB(s)%A2(nk,nBas) memory); cmo[s]: (nbas[s],
+        nbas[s]) with cmo[s][a, p] = CMO%SB(s)%A2(a,p).  w_pwxy: filled in the
+        symmetry-blocked (pw|xy) order of CHO_eval_waxy."""
+        nsym = len(nbas)
+        ntri = sum(int(b) * (int(b) + 1) // 2 for b in nbas)
+        nd = 2 if do_active else 1
+        fl = _rows_inout(flt, nd, ntri, "flt")
+        dl = [_host_f64(x).reshape(-1) for x in (list(dlt) if isinstance(dlt, (list, tuple))
+                                                 else np.asarray(dlt).reshape(-1, ntri))]
+        counts = [np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(nsym))
+                  for v in (nforb, niorb, naorb, nchm, nfro, norb)]
+
+        def blocks(mats, ks):
+            parts = [_host_f64(mats[s])[:, :ks[s]].copy().reshape(-1)
+                     for s in range(nsym) if mats is not None and ks[s] > 0]
+            return np.concatenate(parts) if parts else np.zeros(1)
+
+        p1 = blocks(porb1, counts[0] + counts[1])
+        p2 = blocks(porb2, counts[3]) if do_active else np.zeros(1)
+        p3 = blocks(porb3, counts[2])
+        cm = (np.concatenate([_host_f64(np.asarray(cmo[s]).T).reshape(-1) for s in range(nsym)])
+              if cmo is not None else np.zeros(1))  # CMO%SB(s)%A2, column-major
+        if w_pwxy is not None and (w_pwxy.dtype != np.float64 or not w_pwxy.flags.c_contiguous):
+            raise TypeError("w_pwxy must be a C-contiguous float64 array")
+        nb = np.ascontiguousarray(nbas, dtype=np.int32)
+        rc = self.lib.mjk_cho_fmcscf_sym(self._h, int(nsym), _ptr(nb), _ptr_array(fl),
+                                         _ptr(counts[0]), _ptr(counts[1]), _ptr(counts[2]),
+                                         float(factxi), _ptr_array(dl), int(bool(do_active)),
+                                         _ptr(p1), _ptr(counts[3]), _ptr(p2), _ptr(p3),
+                                         _ptr(w_pwxy), _ptr(cm), _ptr(counts[4]), _ptr(counts[5]),
+                                         float(exfac), precision_code(precision))
+        self._check(rc)
+        return rc
+
+    def cho_lk_casscf_sym(self, nbas, dlt, flt, msq, w_pwxy, factxi: float, nchi, naorb, nchm,
+                          ash1, ash2, do_active: bool, cmo, nfro, norb, nscreen: int = 10,
+                          dmpk: float = 0.1, dfmat: float = 0.0, exfac: float = 1.0,
+                          precision="fp64") -> int:
+        """CHO_LK_CASSCF with point-group symmetry, unscreened (storage as
+        cho_fmcscf_sym).  msq[s] (nbas[s], >= nchi[s]) and ash2[s] (nbas[s], >=
+        nchm[s]) are AO x vector matrices (the MSQ / CVa(2) columns), ash1[s]
+        (nbas[s], naorb[s]) the active MOs, cmo[s] (nbas[s], nbas[s])."""
+        nsym = len(nbas)
+        ntri = sum(int(b) * (int(b) + 1) // 2 for b in nbas)
+        fl = _rows_inout(flt, 2 if do_active else 1, ntri, "flt")
+        dl = [_host_f64(x).reshape(-1) for x in (list(dlt) if isinstance(dlt, (list, tuple))
+                                                 else np.asarray(dlt).reshape(-1, ntri))]
+        counts = [np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(nsym))
+                  for v in (nchi, naorb, nchm, nfro, norb)]
+
+        def squares(mats, ks):  # A2(nBas(s),nBas(s)) blocks, AO index fastest
+            parts = []
+            for s in range(nsym):
+                sq = np.zeros((nbas[s], nbas[s]))
+                if mats is not None and ks[s] > 0:
+                    sq[:, :ks[s]] = _host_f64(mats[s])[:, :ks[s]]
+                parts.append(sq.T.reshape(-1))
+            return np.concatenate(parts)
+
+        m = squares(msq, counts[0])
+        a2 = squares(ash2 if do_active else None, counts[2])
+        parts = [_host_f64(ash1[s])[:, :counts[1][s]].copy().reshape(-1)
+                 for s in range(nsym) if ash1 is not None and counts[1][s] > 0]
+        a1 = np.concatenate(parts) if parts else np.zeros(1)
+        cm = (np.concatenate([_host_f64(np.asarray(cmo[s]).T).reshape(-1) for s in range(nsym)])
+              if cmo is not None else np.zeros(1))
+        if w_pwxy is not None and (w_pwxy.dtype != np.float64 or not w_pwxy.flags.c_contiguous):
+            raise TypeError("w_pwxy must be a C-contiguous float64 array")
+        nb = np.ascontiguousarray(nbas, dtype=np.int32)
+        rc = self.lib.mjk_cho_lk_casscf_sym(self._h, int(nsym), _ptr(nb), _ptr_array(dl), _ptr_array(fl),
+                                            _ptr(m), _ptr(w_pwxy), float(factxi), _ptr(counts[0]),
+                                            _ptr(counts[1]), _ptr(counts[2]), _ptr(a1), _ptr(a2),
+                                            int(bool(do_active)), int(nscreen), float(dmpk), float(dfmat),
+                                            _ptr(cm), _ptr(counts[3]), _ptr(counts[4]), float(exfac),
+                                            precision_code(precision))
+        self._check(rc)
+        return rc
+
     # ------------------------------------------------------------ device engine
     def coulomb(self, dlt: torch.Tensor, want_v: bool = False):
         """J for folded LT densities dlt (nd, npair) on the device."""

@@ -23,7 +23,8 @@ module molcas_jk_iface
   public :: mjk_create, mjk_destroy, mjk_set_reduced_set, mjk_upload_vectors, mjk_upload_vectors_file, mjk_cho_fscf, &
             mjk_cho_fmcscf, mjk_cho_lk_scf, mjk_cho_fock_dft_red, cho_fscf_gpu, cho_lk_scf_gpu, &
             mjk_cho_fscf_sym, cho_fscf_sym_gpu, mjk_cho_lk_scf_sym, cho_lk_scf_sym_gpu, &
-            cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_lk_casscf, cho_lk_casscf_gpu, jk_gpu_init, jk_gpu_final, &
+            cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_fmcscf_sym, cho_fmcscf_sym_gpu, mjk_cho_lk_casscf, &
+            cho_lk_casscf_gpu, mjk_cho_lk_casscf_sym, cho_lk_casscf_sym_gpu, jk_gpu_init, jk_gpu_final, &
             mjk_set_ladder
 
   integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1, MJK_PREC_BF16X3 = 2, &
@@ -141,6 +142,30 @@ module molcas_jk_iface
       real(c_double), intent(in) :: porb1(*), porb2(*), porb3(*), cmo(*)
       real(c_double), intent(inout) :: w_pwxy(*)
     end function mjk_cho_fmcscf
+    integer(c_int) function mjk_cho_fmcscf_sym(ctx, nsym, nbas, flt, nforb, niorb, naorb, factxi, dlt, &
+        do_active, porb1, nchm, porb2, porb3, w_pwxy, cmo, nfro, norb, exfac, prec) &
+        bind(C, name='mjk_cho_fmcscf_sym')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsym, do_active, prec
+      integer(c_int32_t), intent(in) :: nbas(*), nforb(*), niorb(*), naorb(*), nchm(*), nfro(*), norb(*)
+      type(c_ptr), intent(in) :: flt(*), dlt(*)
+      real(c_double), value :: factxi, exfac
+      real(c_double), intent(in) :: porb1(*), porb2(*), porb3(*), cmo(*)
+      real(c_double), intent(inout) :: w_pwxy(*)
+    end function mjk_cho_fmcscf_sym
+    integer(c_int) function mjk_cho_lk_casscf_sym(ctx, nsym, nbas, dlt, flt, msq, w_pwxy, factxi, nchi, naorb, &
+        nchm, ash1, ash2, do_active, nscreen, dmpk, dfmat, cmo, nfro, norb, exfac, prec) &
+        bind(C, name='mjk_cho_lk_casscf_sym')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsym, do_active, nscreen, prec
+      integer(c_int32_t), intent(in) :: nbas(*), nchi(*), naorb(*), nchm(*), nfro(*), norb(*)
+      type(c_ptr), intent(in) :: dlt(*), flt(*)
+      real(c_double), intent(in) :: msq(*), ash1(*), ash2(*), cmo(*)
+      real(c_double), intent(inout) :: w_pwxy(*)
+      real(c_double), value :: factxi, dmpk, dfmat, exfac
+    end function mjk_cho_lk_casscf_sym
   end interface
 
 contains
@@ -370,5 +395,82 @@ contains
                             real(FactXI, c_double), pd, act, POrb1, int(nChM, c_int32_t), POrb2, POrb3, W_PWXY, &
                             CMO, int(nFro, c_int32_t), int(nOrb, c_int32_t), real(ExFac, c_double), prec_of(prec)))
   end subroutine cho_fmcscf_gpu
+
+  ! CHO_FMCSCF with nSym irreps (src/fock_util/cho_cas_drv.F90:277): FLT1/FLT2,
+  ! DLT1/DLT2 are FLT(1:2)%A0, DLT(1:2)%A0 (irrep LT blocks back to back),
+  ! POrb1..3 the POrb(1:3)%A0 and CMO the CMO%A0 memory; the counts are the
+  ! reference's 8-element arrays.  W_PWXY in the CHO_eval_waxy block order.
+  subroutine cho_fmcscf_sym_gpu(rc, nSym, nBas, FLT1, FLT2, nForb, nIorb, nAorb, FactXI, DLT1, DLT2, DoActive, &
+                                POrb1, nChM, POrb2, POrb3, W_PWXY, CMO, nFro, nOrb, ExFac, prec)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nSym, nBas(nSym), nForb(8), nIorb(8), nAorb(8), nChM(8), nFro(8), nOrb(8)
+    logical, intent(in) :: DoActive
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
+    real(c_double), intent(in), target :: DLT1(*), DLT2(*)
+    real(8), intent(inout) :: W_PWXY(*)
+    real(8), intent(in) :: FactXI, POrb1(*), POrb2(*), POrb3(*), CMO(*), ExFac
+    integer, intent(in), optional :: prec
+    integer(c_int32_t) :: act, nb(8), nf(8), ni(8), na(8), nc(8), nr(8), no(8)
+    type(c_ptr) :: pd(2), pf(2)
+    nb = 0
+    nb(1:nSym) = int(nBas(1:nSym), c_int32_t)
+    nf = int(nForb, c_int32_t)
+    ni = int(nIorb, c_int32_t)
+    na = int(nAorb, c_int32_t)
+    nc = int(nChM, c_int32_t)
+    nr = int(nFro, c_int32_t)
+    no = int(nOrb, c_int32_t)
+    act = 0
+    pd(1) = c_loc(DLT1(1))
+    pf(1) = c_loc(FLT1(1))
+    pd(2) = pd(1)
+    pf(2) = pf(1)
+    if (DoActive) then
+      act = 1
+      pd(2) = c_loc(DLT2(1))
+      pf(2) = c_loc(FLT2(1))
+    end if
+    rc = int(mjk_cho_fmcscf_sym(jk_ctx, int(nSym, c_int32_t), nb, pf, nf, ni, na, real(FactXI, c_double), pd, &
+                                act, POrb1, nc, POrb2, POrb3, W_PWXY, CMO, nr, no, real(ExFac, c_double), &
+                                prec_of(prec)))
+  end subroutine cho_fmcscf_sym_gpu
+
+  ! CHO_LK_CASSCF(DLT,FLT,MSQ,W_PWXY,FactXI,nChI,nAorb,nChM,CVa,DoActive,nScreen,dmpK,dFmat,CMO,ExFac)
+  ! with nSym irreps (src/fock_util/cho_cas_drv.F90:285), unscreened: DLTd, FLTd
+  ! the DLT(d)%A0, FLT(d)%A0 irrep blocks, MSQ / Ash2 the MSQ%A0 and CVa(2)%A0
+  ! squares, Ash1 the CVa(1)%A0 blocks A2(nAorb(s),nBas(s)).
+  subroutine cho_lk_casscf_sym_gpu(rc, nSym, nBas, DLT1, DLT2, FLT1, FLT2, MSQ, W_PWXY, FactXI, nChI, nAorb, &
+                                   nChM, Ash1, Ash2, DoActive, nScreen, dmpk, dFmat, CMO, nFro, nOrb, ExFac, prec)
+    integer, intent(out) :: rc
+    integer, intent(in) :: nSym, nBas(nSym), nChI(8), nAorb(8), nChM(8), nScreen, nFro(8), nOrb(8)
+    logical, intent(in) :: DoActive
+    real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
+    real(c_double), intent(in), target :: DLT1(*), DLT2(*)
+    real(8), intent(inout) :: W_PWXY(*)
+    real(8), intent(in) :: MSQ(*), FactXI, Ash1(*), Ash2(*), dmpk, dFmat, CMO(*), ExFac
+    integer, intent(in), optional :: prec
+    integer(c_int32_t) :: act, nb(8), ni(8), na(8), nc(8), nr(8), no(8)
+    type(c_ptr) :: pd(2), pf(2)
+    nb = 0
+    nb(1:nSym) = int(nBas(1:nSym), c_int32_t)
+    ni = int(nChI, c_int32_t)
+    na = int(nAorb, c_int32_t)
+    nc = int(nChM, c_int32_t)
+    nr = int(nFro, c_int32_t)
+    no = int(nOrb, c_int32_t)
+    act = 0
+    pd(1) = c_loc(DLT1(1))
+    pf(1) = c_loc(FLT1(1))
+    pd(2) = pd(1)
+    pf(2) = pf(1)
+    if (DoActive) then
+      act = 1
+      pd(2) = c_loc(DLT2(1))
+      pf(2) = c_loc(FLT2(1))
+    end if
+    rc = int(mjk_cho_lk_casscf_sym(jk_ctx, int(nSym, c_int32_t), nb, pd, pf, MSQ, W_PWXY, real(FactXI, c_double), &
+                                   ni, na, nc, Ash1, Ash2, act, int(nScreen, c_int32_t), real(dmpk, c_double), &
+                                   real(dFmat, c_double), CMO, nr, no, real(ExFac, c_double), prec_of(prec)))
+  end subroutine cho_lk_casscf_sym_gpu
 
 end module molcas_jk_iface

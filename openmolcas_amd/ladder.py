@@ -39,6 +39,7 @@ class Ladder:
     j_bf16: int
     predicted: float  # predicted |dE| of the two reduced-precision segments
     measured: float = float("nan")  # |dE| of this ladder against FP64, when verified
+    nchol: int = -1  # number of vectors the split points refer to
 
 
 def tail_sums(w) -> np.ndarray:
@@ -64,7 +65,30 @@ def plan(w, kappa_fp32: float, kappa_bf16: float, tol: float = 1e-6,
     while j3 > 0 and kappa_fp32 * (t[j3 - 1] - t[jb]) <= budget:
         j3 -= 1
     pred = kappa_fp32 * (t[j3] - t[jb]) + (kappa_bf16 * t[jb] if jb < m else 0.0)
-    return Ladder(int(j3), int(jb), float(pred))
+    return Ladder(int(j3), int(jb), float(pred), nchol=int(m))
+
+
+def speedup_bound(plans, work=None, rate_fp32: float = float("inf"),
+                  rate_bf16: float = float("inf")) -> float:
+    """Exchange speedup over FP64 of ladders whose segments run at the given
+    per-vector rates relative to FP64: 1 / (f64 + f32 / rate_fp32 + fb /
+    rate_bf16), with f the shares of the exchange work in each mode.  The
+    work of one vector of a system is the same for every vector (2 n^2 o +
+    n (n + 1) o flop), so a system's shares are its vector counts; ``work``
+    weights the systems of a batch (per-vector work of each, default 1).
+    With the default infinite rates this is the cap the FP64 head sets:
+    even free reduced-precision segments give no more than 1 / f64."""
+    plans = [plans] if isinstance(plans, Ladder) else list(plans)
+    work = [1.0] * len(plans) if work is None else [float(x) for x in work]
+    nchol = [q.nchol for q in plans]
+    if any(m < 0 for m in nchol):
+        raise ValueError("speedup_bound needs ladders that carry nchol (plan/tune set it)")
+    tot = sum(w * m for w, m in zip(work, nchol))
+    f64 = sum(w * q.j_fp32 for w, q in zip(work, plans)) / tot
+    f32 = sum(w * (q.j_bf16 - q.j_fp32) for w, q in zip(work, plans)) / tot
+    fb = sum(w * (m - q.j_bf16) for w, q, m in zip(work, plans, nchol)) / tot
+    t = f64 + (f32 / rate_fp32 if f32 > 0 else 0.0) + (fb / rate_bf16 if fb > 0 else 0.0)
+    return float("inf") if t == 0.0 else 1.0 / t
 
 
 def exchange_energy_error(k_test: torch.Tensor, k_ref: torch.Tensor, dlt: torch.Tensor,
@@ -126,4 +150,4 @@ def tune(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, tol: float = 1e-6,
         if de <= accept * tol:
             return p
         s *= 0.5 * accept * tol / max(de, 1e-300)
-    return Ladder(m, m, 0.0, 0.0)
+    return Ladder(m, m, 0.0, 0.0, nchol=m)

@@ -1,6 +1,7 @@
 """The drop-ins called from Fortran through the ISO_C_BINDING module
 (openmolcas_amd/fortran): CHO_FSCF C1 and with two irreps, and the
-two-density calls (UHF, CASSCF FI/FA) on separately allocated arrays,
+two-density calls (UHF, CASSCF FI/FA) on separately allocated arrays and
+CHO_FMCSCF with two irreps,
 checked against the oracle (FP64, relative 1e-11)."""
 import os
 import subprocess
@@ -9,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from openmolcas_amd.build import FORTRAN_BIN, FORTRAN_SYM_BIN, FORTRAN_UHF_CAS_BIN
+from openmolcas_amd.build import FORTRAN_BIN, FORTRAN_CAS_SYM_BIN, FORTRAN_SYM_BIN, FORTRAN_UHF_CAS_BIN
 from openmolcas_amd.synthetic import chol_vectors_np, fold, npair, orbitals_np
 
 pytestmark = pytest.mark.gpu
@@ -112,3 +113,53 @@ def test_fortran_two_density_separate_arrays(gpu, tmp_path):
               lc_f[0], lc_f[1], lc_w]
     for got, exp in zip(parts, expect):
         assert np.max(np.abs(got - exp)) / max(np.max(np.abs(exp)), 1e-300) < 1e-11
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_CAS_SYM_BIN), reason="Fortran driver not built (no amdflang)")
+@pytest.mark.parametrize("do_active", [True, False])
+def test_fortran_cho_fmcscf_symmetry_dropin(gpu, tmp_path, do_active):
+    """CHO_FMCSCF and unscreened CHO_LK_CASSCF with nSym = 2 from Fortran
+    (FI/FA, DI/DA separate arrays): FI, FA and W_PWXY of each against the C1
+    oracle and the dense (pw|xy)."""
+    from _sym import NBAS, make_casscf_problem
+    q = make_casscf_problem(do_active, seed=41)
+    p = q["vectors"]
+    rs1, rs2 = (np.array(r, dtype=np.float64) for r in p["rs"])
+    nfi = [q["nforb"][s] + q["niorb"][s] for s in range(2)]
+
+    def blocks(mats, ks):  # A2(nk(s), nBas(s)) irrep blocks back to back
+        return np.concatenate([np.ascontiguousarray(mats[s][:, :ks[s]]).reshape(-1) for s in range(2)])
+
+    def squares(mats, ks):  # A2(nBas(s), nBas(s)) blocks, AO index fastest, first ks(s) columns
+        out = []
+        for s in range(2):
+            sq = np.zeros((NBAS[s], NBAS[s]))
+            sq[:, :ks[s]] = mats[s][:, :ks[s]]
+            out.append(np.ascontiguousarray(sq.T).reshape(-1))
+        return np.concatenate(out)
+
+    nw = q["want_w"].size
+    hdr = (NBAS + [p["V1"].shape[0], p["V2"].shape[0], rs1.size, rs2.size] + q["nforb"] + q["niorb"]
+           + q["naorb"] + q["nchm"] + q["nfro"] + q["norb"] + [q["factxi"], float(do_active), nw])
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array(hdr, dtype=np.float64).tofile(f)
+        for a in (rs1, rs2, p["V1"], p["V2"], q["dsym"][0], q["dsym"][1], q["fsym0"][0], q["fsym0"][1],
+                  blocks(q["porb1"], nfi), blocks(q["porb2"], q["nchm"]), blocks(q["porb3"], q["naorb"]),
+                  np.concatenate([np.ascontiguousarray(c.T).reshape(-1) for c in q["cmo"]]),
+                  squares(q["porb1"], nfi), squares(q["porb2"], q["nchm"])):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+    r = subprocess.run([FORTRAN_CAS_SYM_BIN, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    out = np.fromfile(fout, dtype=np.float64)
+    nt = q["fsym0"].shape[1]
+    assert out.size == 2 * (2 * nt + nw)
+    for fi, fa, w in (np.split(half, [nt, 2 * nt]) for half in np.split(out, 2)):
+        assert np.max(np.abs(fi - q["want_f"][0])) <= 1e-11 * np.max(np.abs(q["want_f"][0]))
+        if do_active:
+            assert np.max(np.abs(fa - q["want_f"][1])) <= 1e-11 * np.max(np.abs(q["want_f"][1]))
+        else:
+            assert np.array_equal(fa, q["fsym0"][1])
+        assert np.max(np.abs(w - q["want_w"])) <= 1e-11 * np.max(np.abs(q["want_w"]))

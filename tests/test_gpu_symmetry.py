@@ -6,7 +6,8 @@ Two irreps (nbas 7 + 4), vectors of JSYM = 1 and 2 in their own reduced sets
 import numpy as np
 import pytest
 
-from _sym import NBAS, make_problem
+from _sym import NBAS, make_casscf_problem, make_problem
+from openmolcas_amd import MjkError
 from openmolcas_amd.fock import JKContext
 
 pytestmark = pytest.mark.gpu
@@ -33,3 +34,60 @@ def test_cho_fscf_symmetry_matches_c1_oracle(gpu, nden, exfac, lk):
     jk.close()
     want = p["want"]
     assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want))
+
+
+
+@pytest.mark.parametrize("do_active", [True, False])
+def test_cho_fmcscf_symmetry_matches_c1_oracle(gpu, do_active):
+    """CHO_FMCSCF, nSym = 2: FI/FA irrep blocks against the oracle's C1
+    CHO_FMCSCF on the embedded problem; W_PWXY against (pw|xy) formed from the
+    dense vectors and MOs, laid out in blocks (sp, sw, sx <= sy) with
+    sp x sw = sx x sy (cho_eval_waxy.F90:86-141).  Tolerance: relative 1e-11."""
+    q = make_casscf_problem(do_active, seed=31)
+    p = q["vectors"]
+    jk = JKContext(p["n"], p["V1"].shape[0] + p["V2"].shape[0], device=gpu)
+    jk.set_reduced_set(0, np.array(p["rs"][0], dtype=np.int64))
+    jk.set_reduced_set(1, np.array(p["rs"][1], dtype=np.int64))
+    jk.upload_vectors(0, 0, p["V1"])
+    jk.upload_vectors(1, p["V1"].shape[0], p["V2"])
+    fl = q["fsym0"].copy()
+    w = np.full(q["want_w"].size, np.nan)
+    jk.cho_fmcscf_sym(NBAS, fl, q["nforb"], q["niorb"], q["naorb"], q["factxi"], q["dsym"], do_active,
+                      q["porb1"], q["nchm"], q["porb2"], q["porb3"], w, q["cmo"], q["nfro"], q["norb"])
+    jk.close()
+    nd = 2 if do_active else 1
+    want = q["want_f"][:nd]
+    assert np.max(np.abs(fl[:nd] - want)) <= 1e-11 * np.max(np.abs(want))
+    if not do_active:
+        assert np.array_equal(fl[1], q["fsym0"][1])  # FA untouched
+    assert np.max(np.abs(w - q["want_w"])) <= 1e-11 * np.max(np.abs(q["want_w"]))
+
+
+@pytest.mark.parametrize("do_active", [True, False])
+def test_cho_lk_casscf_symmetry_matches_c1_oracle(gpu, do_active):
+    """CHO_LK_CASSCF, nSym = 2, unscreened with ExFac = 1: the same FI, FA and
+    W_PWXY as CHO_FMCSCF on that problem (MSQ columns = POrb(1), CVa(2)
+    columns = POrb(2)); LK screening with nSym > 1 is refused."""
+    q = make_casscf_problem(do_active, seed=37)
+    p = q["vectors"]
+    jk = JKContext(p["n"], p["V1"].shape[0] + p["V2"].shape[0], device=gpu)
+    jk.set_reduced_set(0, np.array(p["rs"][0], dtype=np.int64))
+    jk.set_reduced_set(1, np.array(p["rs"][1], dtype=np.int64))
+    jk.upload_vectors(0, 0, p["V1"])
+    jk.upload_vectors(1, p["V1"].shape[0], p["V2"])
+    nchi = [q["nforb"][s] + q["niorb"][s] for s in range(2)]
+    fl = q["fsym0"].copy()
+    w = np.full(q["want_w"].size, np.nan)
+    jk.cho_lk_casscf_sym(NBAS, q["dsym"], fl, q["porb1"], w, q["factxi"], nchi, q["naorb"], q["nchm"],
+                         q["porb3"], q["porb2"], do_active, q["cmo"], q["nfro"], q["norb"])
+    nd = 2 if do_active else 1
+    want = q["want_f"][:nd]
+    assert np.max(np.abs(fl[:nd] - want)) <= 1e-11 * np.max(np.abs(want))
+    if not do_active:
+        assert np.array_equal(fl[1], q["fsym0"][1])
+    assert np.max(np.abs(w - q["want_w"])) <= 1e-11 * np.max(np.abs(q["want_w"]))
+    jk.set_lk_screening(True)
+    with pytest.raises(MjkError, match="nSym > 1"):
+        jk.cho_lk_casscf_sym(NBAS, q["dsym"], fl, q["porb1"], w, q["factxi"], nchi, q["naorb"], q["nchm"],
+                             q["porb3"], q["porb2"], do_active, q["cmo"], q["nfro"], q["norb"])
+    jk.close()
