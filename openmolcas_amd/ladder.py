@@ -92,12 +92,18 @@ def speedup_bound(plans, work=None, rate_fp32: float = float("inf"),
 
 
 def exchange_energy_error(k_test: torch.Tensor, k_ref: torch.Tensor, dlt: torch.Tensor,
-                          factor: float = 0.5) -> float:
-    """|factor * sum_d sum_LT (K_test - K_ref)_d * dlt_d| for folded densities."""
-    return abs(factor * torch.sum((k_test - k_ref) * dlt).item())
+                          factor: float = 0.5, per_density: bool = False) -> float:
+    """|factor * sum_d sum_LT (K_test - K_ref)_d * dlt_d| for folded densities;
+    per_density: the largest of the densities' own |factor * sum_LT (...)_d|
+    (the bound every state of a state-averaged batch must meet)."""
+    prod = (k_test - k_ref) * dlt
+    if per_density:
+        rows = prod.reshape(-1, prod.shape[-1])
+        return float(torch.max(torch.abs(factor * torch.sum(rows, dim=1))).item())
+    return abs(factor * torch.sum(prod).item())
 
 
-def calibrate(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, w=None):
+def calibrate(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, w=None, per_density: bool = False):
     """Measure kappa_fp32 and kappa_bf16 for this system: one exchange build
     per precision against FP64.  dlt holds the folded densities that weight
     K in the energy (one row per orbital set) and factor the exchange energy
@@ -110,7 +116,7 @@ def calibrate(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, w=None):
     kappas = []
     for prec in ("fp32", "bf16x3"):
         k = jk.exchange(corbs, precision=prec)
-        de = exchange_energy_error(k, k64, dlt, factor)
+        de = exchange_energy_error(k, k64, dlt, factor, per_density)
         kappas.append(max(de, 1e-300) / max(total, 1e-300))
     return kappas[0], kappas[1], w, k64
 
@@ -130,11 +136,14 @@ def mode_times(jk, corbs, repeats: int = 2):
 
 
 def tune(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, tol: float = 1e-6,
-         share: float = 0.25, accept: float = 0.5, rounds: int = 6) -> Ladder:
+         share: float = 0.25, accept: float = 0.5, rounds: int = 6,
+         per_density: bool = False) -> Ladder:
     """Calibrate, plan and verify: returns a ladder whose measured |dE|
     against the FP64 build is at most accept * tol (FP64 throughout if no
-    ladder within ``rounds`` tightening steps qualifies)."""
-    k32, kb3, w, k64 = calibrate(jk, corbs, dlt, factor)
+    ladder within ``rounds`` tightening steps qualifies).  per_density: |dE|
+    is the largest error of any one density (orbital set) instead of the
+    error of their sum."""
+    k32, kb3, w, k64 = calibrate(jk, corbs, dlt, factor, per_density=per_density)
     t32, tb3 = mode_times(jk, corbs)
     if tb3 >= t32:
         kb3 = float("inf")  # no split-BF16 segment: FP32 is as fast and more accurate
@@ -145,7 +154,7 @@ def tune(jk, corbs, dlt: torch.Tensor, factor: float = 0.5, tol: float = 1e-6,
         if p.j_fp32 >= m:
             break
         k = jk.exchange_mixed(corbs, p.j_fp32, p.j_bf16)
-        de = exchange_energy_error(k, k64, dlt, factor)
+        de = exchange_energy_error(k, k64, dlt, factor, per_density)
         p.measured = de
         if de <= accept * tol:
             return p

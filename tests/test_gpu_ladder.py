@@ -8,7 +8,8 @@
 * Vector weights ||L^J||_F^2 against a dense numpy brute force.
 * On the n=512, nchol=2048, nocc=64 workload the tuned (calibrated, planned
   and verified) ladder keeps the closed-shell energy within 0.5e-6 Eh of the
-  FP64 build.
+  FP64 build; on n=1200, nchol=6000, nocc=150 within 1e-6 Eh, with dE/dC
+  within 1e-5 (max-abs) of the FP64 gradient.
 """
 import numpy as np
 import pytest
@@ -103,4 +104,31 @@ def test_tuned_ladder_energy_n512(gpu):
     assert de <= 0.5e-6, de
     # the same ladder under the other energy expression used by the benchmark
     assert ladder.exchange_energy_error(K, K64, dlt, 0.25) <= 0.25e-6
+    jk.close()
+
+
+def test_tuned_ladder_energy_and_gradient_n1200(gpu):
+    """nbas=1200, nchol=6000, nocc=150 at full size: the tuned ladder against
+    the FP64 kernels, |dE| <= 1e-6 Eh and max-abs dE/dC error <= 1e-5 (dE/dC =
+    4 F C with F = h + J - K[C], so a K error enters as -4 dK C)."""
+    n, M, o = 1200, 6000, 150
+    jk = JKContext(n, M, device=gpu)
+    chol_vectors_torch(n, M, 3, device=gpu, out=jk.device_vectors())
+    C = orbitals_torch(n, o, 4, device=gpu)
+    dlt = jk.density(C, 2.0)
+    plan = ladder.tune(jk, [C], dlt, factor=0.5, tol=1e-6)
+    print(f"ladder n=1200: split {plan.j_fp32} / {plan.j_bf16} of {M}, measured {plan.measured:.2e}")
+    assert plan.j_fp32 < M
+    J = jk.coulomb(dlt)
+    K64 = jk.exchange([C], "fp64")
+    K = jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
+    G64, G = torch.zeros_like(J), torch.zeros_like(J)
+    jk.fock_update(G64, J, [0], [1.0], K64, [-1.0])
+    jk.fock_update(G, J, [0], [1.0], K, [-1.0])
+    h = torch.zeros_like(J)
+    de = abs(jk.energy(h, dlt, G).item() - jk.energy(h, dlt, G64).item())
+    assert de <= 1e-6, de
+    dc_err = torch.max(torch.abs(jk.grad_orbitals(C, G, 4.0) - jk.grad_orbitals(C, G64, 4.0))).item()
+    print(f"ladder n=1200: |dE| {de:.2e} Eh, max |d(dE/dC)| {dc_err:.2e}")
+    assert dc_err <= 1e-5, dc_err
     jk.close()
