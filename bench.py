@@ -641,12 +641,14 @@ def single_build(args, dev, steps):
     k_ms = phases["half_transform"][0] / steps + phases["exchange"][0] / steps
     k_tf = (flops_ht + flops_syrk) / (k_ms * 1e-3) / 1e12
     # dominant kernel: the larger of the two exchange kernels by time
-    ptag = {"fp64": "F64", "fp32": "F32", "bf16x3": "BF16"}[args.precision]
+    # kernel names as the profiles list them (split-BF16 has kernels of its own)
+    kht, ksy = {"fp64": ("k_ht<F64", "k_syrk<F64"), "fp32": ("k_ht<F32", "k_syrk<F32"),
+                "bf16x3": ("k_ht_b3", "k_syrk_b3")}[args.precision]
     if ht_ms * ht_calls >= sy_ms * sy_calls:
         what = "half-transform X = C^T L" + (" with the Coulomb V = L^T D; MFMA flops only" if fused else "")
-        dom = (f"k_ht<{ptag}> ({what})", ht_tf, f"k_ht<{ptag},")
+        dom = (f"{kht}> ({what})", ht_tf, kht + ("," if "<" in kht else "<"))
     else:
-        dom = (f"k_syrk<{ptag}> (K = sum X^T X)", sy_tf, f"k_syrk<{ptag},")
+        dom = (f"{ksy}> (K = sum X^T X)", sy_tf, ksy + ("," if "<" in ksy else "<"))
     peaks = {name: round(ctx.measure_peak(kind), 2) for name, kind in
              (("mfma_f64_TFs", 0), ("mfma_f32_TFs", 1), ("mfma_bf16_TFs", 2), ("fma_f64_TFs", 3),
               ("copy_TBs", 4))}
