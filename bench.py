@@ -150,6 +150,7 @@ def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
     tk = {}
     for prec in ("fp64", "fp32", "bf16x3"):
         try:
+            ctx.autotune([C], prec)  # each mode timed with its own tiles
             ctx.exchange([C], precision=prec)
         except Exception as exc:  # mode not built
             res[prec] = {"error": str(exc)}
@@ -171,6 +172,7 @@ def mixed_vs_fp64(ctx, dlt, h, C, n, M, o, ms_fp64, args):
     if "fp64" in tk:  # the plain FP64 exchange (mjk_exchange, no V work) against the MFMA peak
         res["fp64"]["exchange_frac_peak"] = round(fk / tk["fp64"] / 1e12 / PEAK_TFLOPS["fp64"], 4)
     if "fp32" in tk and "bf16x3" in tk:
+        ctx.autotune([C], "fp64")  # the ladder's head is FP64
         res["ladder"] = ladder_vs_fp64(ctx, dlt, h, C, J, K64, e64, tk["fp64"], res, args.steps)
     return res
 
@@ -273,10 +275,12 @@ def build1200(dev):
     e64 = jk.energy(h, dlt, G).item()
     res["coulomb_ms"] = round(1e3 * _timeit(lambda: jk.coulomb(dlt), 3), 3)
     res["coulomb_GBs"] = round(2.0 * M * npair(n) * 8 / (res["coulomb_ms"] * 1e-3) / 1e9, 1)
+    jk.autotune([C], "fp64")
     res["builds_per_s_fp64"] = round(1.0 / _timeit(lambda: jk.coulomb_exchange(dlt, [C]), 3), 3)
     fk = M * o * (2.0 * n * n + n * (n + 1))
     dC64 = jk.grad_orbitals(C, G, 4.0)
     for prec in ("fp64", "fp32", "bf16x3"):
+        jk.autotune([C], prec)  # each mode timed with its own tiles
         t = _timeit(lambda: jk.exchange([C], precision=prec), 2)
         K = jk.exchange([C], precision=prec)
         Gp = torch.zeros_like(J)
@@ -286,6 +290,7 @@ def build1200(dev):
         res[f"dE_{prec}"] = abs(jk.energy(h, dlt, Gp).item() - e64)
         if prec != "fp64":
             res[f"dEdC_maxabs_err_{prec}"] = float(torch.max(torch.abs(jk.grad_orbitals(C, Gp, 4.0) - dC64)).item())
+    jk.autotune([C], "fp64")  # the ladder's head is FP64
     plan = ladder.tune(jk, [C], dlt, factor=0.5, tol=1e-6)  # G = J - K[C]: dE = 1/2 dK.D
     t = _timeit(lambda: jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16), 2)
     K = jk.exchange_mixed([C], plan.j_fp32, plan.j_bf16)
@@ -582,6 +587,7 @@ def single_build(args, dev, steps):
     ctx = JKContext(n, M, device=dev)
     ctx.upload_packed(L)
     del L
+    ctx.autotune([C], args.precision)  # tiles of the timed precision (HIP-event timed, median of 3)
     torch.cuda.synchronize()
     F = torch.empty((1, npr), dtype=torch.float64, device=dev)
     jsrc, fj, fk = [0], [1.0], [-0.5]

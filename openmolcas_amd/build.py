@@ -55,8 +55,8 @@ FORTRAN_CAS_SYM_BIN = os.path.join(FORTRAN, "bin", "fmcscf_sym_driver")
 def build_fortran(verbose: bool = False):
     """ISO_C_BINDING interface module + the Fortran drivers of the drop-ins:
     CHO_FSCF C1 and with symmetry, the two-density UHF / CASSCF calls on
-    separately allocated arrays and CHO_FMCSCF with symmetry (amdflang).  Returns the C1 driver path, or
-    None without a compiler."""
+    separately allocated arrays, and CHO_FMCSCF / CHO_LK_CASSCF with symmetry
+    (amdflang).  Returns the C1 driver path, or None without a compiler."""
     if not os.path.exists(FLANG):
         if verbose:
             print("amdflang not found: Fortran interface not built")

@@ -607,7 +607,7 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
   if (c->nloaded != c->nchol)
     return set_error(c, MJK_ERR_SHORT_READ, "grad_vectors: not all Cholesky vectors were uploaded");
   const int n = c->n, n_pad = c->n_pad;
-  const int64_t M = c->nchol, np = c->np;
+  const int64_t M = c->nchol;
   int rc;
   // bank with the orbitals (FP64), X chunk, Z and W per vector
   std::vector<int> off, opad;

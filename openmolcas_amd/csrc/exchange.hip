@@ -483,16 +483,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       }
     }
   };
+  // FP32: a fragment read pass (32 lanes of 4 bytes, 32 banks) covers rows kr,
+  // kr + 1 of TM floats, a multiple of 32 apart; column c of row f sits at
+  // c ^ 16 (f & 1) so the two rows fall in different bank halves (the 16-byte
+  // stores keep their four columns together)
+  auto spos = [](int f, int c) {
+    if constexpr (sizeof(T) == 4) return f * TM + (c ^ ((f & 1) << 4));
+    else return f * TM + c;
+  };
   auto stash = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < PN; ++i)
-      *reinterpret_cast<VT *>(&sA[buf][(ff0 + STEP * i) * TM + cc]) =
-          ((rok >> i) & 1u) ? ra[i] : VT(0);
+      *reinterpret_cast<VT *>(&sA[buf][spos(ff0 + STEP * i, cc)]) = ((rok >> i) & 1u) ? ra[i] : VT(0);
     if (!diag) {
 #pragma unroll
       for (int i = 0; i < PN; ++i)
-        *reinterpret_cast<VT *>(&sB[buf][(ff0 + STEP * i) * TM + cc]) =
-            ((rok >> i) & 1u) ? rb[i] : VT(0);
+        *reinterpret_cast<VT *>(&sB[buf][spos(ff0 + STEP * i, cc)]) = ((rok >> i) & 1u) ? rb[i] : VT(0);
     }
   };
   // Double-buffered LDS: chunk c is multiplied out of buffer c&1 while chunk
@@ -515,9 +521,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
       auto ld = [&](int ks, int q) {
         const int kr = ks * 4 + (lane >> 4);
 #pragma unroll
-        for (int i = 0; i < MT; ++i) af[q][i] = sa[kr * TM + wm * WT + i * 16 + (lane & 15)];
+        for (int i = 0; i < MT; ++i) af[q][i] = sa[spos(kr, wm * WT + i * 16 + (lane & 15))];
 #pragma unroll
-        for (int j = 0; j < MT; ++j) bf[q][j] = sb[kr * TM + wn * WT + j * 16 + (lane & 15)];
+        for (int j = 0; j < MT; ++j) bf[q][j] = sb[spos(kr, wn * WT + j * 16 + (lane & 15))];
       };
       // LO: a wave on the diagonal of a diagonal tile, MFMA tiles i >= j only
       // (a compile-time pattern, so the MFMAs issue back to back)
@@ -644,13 +650,18 @@ __global__ __launch_bounds__(kThreads) void k_cnorm(const T *__restrict__ Cb, in
 // keep 16-byte alignment).  Result layout: row 4*(l/16) + r, column l%16.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-struct B3 {
-  typedef float T;
-  static constexpr bool kFlush = true;
-};
-
 constexpr int kB3K = 32;          // contraction indices per MFMA
 constexpr int kB3S = kB3K + 8;    // LDS row stride in bf16 elements
+
+// LDS position of contraction index k (0..31) of row r in a split-BF16 tile:
+// the 16-byte granule k / 8 is XOR-ed with bits 3-4 of the row.  A fragment
+// read pass (8 lanes of 16 bytes, 32 banks of 4 bytes) covers 8 consecutive
+// rows of one granule and stays conflict-free (rows 80 bytes apart); stores
+// whose lanes walk rows 4 or 8 apart (SYRK staging, the half-transform's row
+// chunks) no longer pile onto two bank groups.
+__device__ __forceinline__ int b3pos(int r, int k) {
+  return r * kB3S + ((((k >> 3) ^ (r >> 3)) & 3) << 3) + (k & 7);
+}
 
 __device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
   hi = __bf16(x);
@@ -715,15 +726,16 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
     static_assert(LN % 4 == 0, "staging shape");
     float rl[LG][4];
     int lo[LG];
-    uint4 rch[CN], rcl[CN];
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // 16 bytes, promotable to registers
+    u32x4 rch[CN], rcl[CN];
     auto fetch = [&](int b0) {
 #pragma unroll
       for (int i = 0; i < CN; ++i) {
         const int p = tid + kThreads * i;
         if (p < CCH) {
           const int64_t src = int64_t(p / 4) * n_pad + b0 + (p % 4) * 8;
-          rch[i] = *reinterpret_cast<const uint4 *>(CbH + src);
-          rcl[i] = *reinterpret_cast<const uint4 *>(CbL + src);
+          rch[i] = *reinterpret_cast<const u32x4 *>(CbH + src);
+          rcl[i] = *reinterpret_cast<const u32x4 *>(CbL + src);
         }
       }
       // chunks wholly inside the matrix step each thread's packed offsets
@@ -739,7 +751,7 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
         for (int i = 0; i < LG; ++i) {
 #pragma unroll
           for (int k = 0; k < 4; ++k) rl[i][k] = Lj[off + uint32_t(k)];
-          lo[i] = (tid / (kB3K / 4) + GA * i) * kB3S + 4 * (tid % (kB3K / 4));
+          lo[i] = b3pos(tid / (kB3K / 4) + GA * i, 4 * (tid % (kB3K / 4)));
           off += uint32_t(GA) * a + uint32_t(GA * (GA + 1) / 2);
           a += uint32_t(GA);
         }
@@ -754,7 +766,7 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
             rl[i][k] = Lj[idx];
             idx += b + uint32_t(k) + 1u;
           }
-          lo[i] = (tid % TA) * kB3S + 4 * (tid / TA + (GB / 4) * i);
+          lo[i] = b3pos(tid % TA, 4 * (tid / TA + (GB / 4) * i));
           b += uint32_t(GB);
         }
       } else if (b0 + kB3K <= a0) {  // every b < a
@@ -764,7 +776,7 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
           const float *__restrict__ row = Lj + (tri0u(uint32_t(a)) + uint32_t(b0 + 4 * q));
 #pragma unroll
           for (int k = 0; k < 4; ++k) rl[i][k] = (a < n) ? row[k] : 0.f;
-          lo[i] = aa * kB3S + 4 * q;
+          lo[i] = b3pos(aa, 4 * q);
         }
       } else {
 #pragma unroll
@@ -778,7 +790,7 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
               v = (b >= a) ? Lj[tri0u(uint32_t(b)) + uint32_t(a)] : Lj[tri0u(uint32_t(a)) + uint32_t(b)];
             rl[i][k] = v;
           }
-          lo[i] = aa * kB3S + 4 * q;
+          lo[i] = b3pos(aa, 4 * q);
         }
       }
     };
@@ -790,8 +802,8 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
         const int p = tid + kThreads * i;
         if (p < CCH) {
           const int dst = (p / 4) * kB3S + (p % 4) * 8;
-          *reinterpret_cast<uint4 *>(sCh + dst) = rch[i];
-          *reinterpret_cast<uint4 *>(sCl + dst) = rcl[i];
+          *reinterpret_cast<u32x4 *>(sCh + dst) = rch[i];
+          *reinterpret_cast<u32x4 *>(sCl + dst) = rcl[i];
         }
       }
 #pragma unroll
@@ -814,8 +826,8 @@ __global__ __launch_bounds__(kThreads) void k_ht_b3(
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         const int r = w * (TA / 4) + 16 * j + (lane & 15);
-        bh[j] = *reinterpret_cast<const bf16x8 *>(sLh + r * kB3S + ko);
-        bl[j] = *reinterpret_cast<const bf16x8 *>(sLl + r * kB3S + ko);
+        bh[j] = *reinterpret_cast<const bf16x8 *>(sLh + b3pos(r, ko));
+        bl[j] = *reinterpret_cast<const bf16x8 *>(sLl + b3pos(r, ko));
       }
 #pragma unroll
       for (int t = 0; t < OT; ++t) {
@@ -917,8 +929,8 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
         h[k] = hh;
         l[k] = ll;
       }
-      *reinterpret_cast<bfq *>(sh + (4 * cq + c) * kB3S + RQ * fq) = h;
-      *reinterpret_cast<bfq *>(sl + (4 * cq + c) * kB3S + RQ * fq) = l;
+      *reinterpret_cast<bfq *>(sh + b3pos(4 * cq + c, RQ * fq)) = h;
+      *reinterpret_cast<bfq *>(sl + b3pos(4 * cq + c, RQ * fq)) = l;
     }
   };
   if (nf > 0) fetch(0);
@@ -936,14 +948,14 @@ __global__ __launch_bounds__(kThreads) void k_syrk_b3(
 #pragma unroll
       for (int i = 0; i < MT; ++i) {
         const int r = wm * WT + 16 * i + (lane & 15);
-        ah[i] = *reinterpret_cast<const bf16x8 *>(sAh + r * kB3S + ko);
-        al[i] = *reinterpret_cast<const bf16x8 *>(sAl + r * kB3S + ko);
+        ah[i] = *reinterpret_cast<const bf16x8 *>(sAh + b3pos(r, ko));
+        al[i] = *reinterpret_cast<const bf16x8 *>(sAl + b3pos(r, ko));
       }
 #pragma unroll
       for (int j = 0; j < MT; ++j) {
         const int r = wn * WT + 16 * j + (lane & 15);
-        bh[j] = *reinterpret_cast<const bf16x8 *>(sbh + r * kB3S + ko);
-        bl[j] = *reinterpret_cast<const bf16x8 *>(sbl + r * kB3S + ko);
+        bh[j] = *reinterpret_cast<const bf16x8 *>(sbh + b3pos(r, ko));
+        bl[j] = *reinterpret_cast<const bf16x8 *>(sbl + b3pos(r, ko));
       }
       if (!diag || wm > wn) {
 #pragma unroll
