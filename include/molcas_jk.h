@@ -238,14 +238,19 @@ int mjk_cho_fscf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, int32_
  * storage and embedding as mjk_cho_fscf_sym, with porb[d] the irrep blocks
  * Porb(d)%SB(s)%A2(nbas[s], nbas[s]) (AO index fastest, the first
  * nforb + niorb columns are the orbitals) back to back and plt the folded
- * total density in irrep LT blocks.  FLT(d)%SB(s) += J[PLT] - FactXI K.  The
- * KLT scratch of the reference is not returned (its content is folded into
- * FLT by the reference itself, cho_lk_scf.F90:963-1028); LK screening as for
- * mjk_cho_lk_scf, with nOcc the total over irreps. */
+ * total density in irrep LT blocks.  FLT(d)%SB(s) += J[PLT] - FactXI K.
+ * klt is NULL or an array of nden pointers to KLT(d)%A0: per irrep s the
+ * block KLT(d)%SB(s) (nbas[s](nbas[s]+1)/2 values, blocks back to back) is
+ * accumulated with -FactXI K in the reference's shell-pair-blocked storage
+ * of that irrep's shells (cho_lk_scf.F90:274-292), nbassh[s + nsym*sh] =
+ * nBasSh(s, sh) for the nshell shells (nshell = 0: one shell per irrep, plain
+ * LT order).  Runs unscreened: with LK screening switched on and nsym > 1 it
+ * returns MJK_ERR_ARG (the reference screens per irrep shell block). */
 int mjk_cho_lk_scf_sym(mjk_context *ctx, int32_t nsym, const int32_t *nbas, int32_t nden,
-                       double *const *flt, const int32_t *nforb, const int32_t *niorb,
-                       const double *const *porb, const double *plt, double factxi,
-                       int32_t nscreen, double dmpk, double dfmat, int32_t precision);
+                       double *const *flt, double *const *klt, const int32_t *nforb,
+                       const int32_t *niorb, const double *const *porb, const double *plt,
+                       double factxi, int32_t nscreen, double dmpk, double dfmat,
+                       int32_t nshell, const int32_t *nbassh, int32_t precision);
 
 /* CHO_LK_SCF (src/fock_util/cho_lk_scf.F90:14), nSym = 1:
  *   FLT(d) += J[PLT(1)] - FactXI * sum_{J,k} L(a,J)[k] L(b,J)[k],  d < nden,

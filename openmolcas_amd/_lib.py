@@ -49,8 +49,8 @@ SIGNATURES = {
     "mjk_set_ladder": (_c.c_int, [_vp, _i64, _i64]),
     "mjk_cho_fscf": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
     "mjk_cho_fscf_sym": (_c.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32]),
-    "mjk_cho_lk_scf_sym": (_c.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _c.c_double, _i32,
-                                      _c.c_double, _c.c_double, _i32]),
+    "mjk_cho_lk_scf_sym": (_c.c_int, [_vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _c.c_double,
+                                      _i32, _c.c_double, _c.c_double, _i32, _vp, _i32]),
     "mjk_cho_fock_dft_red": (_c.c_int, [_vp, _vp, _vp]),
     "mjk_cho_lk_casscf": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_double, _i32, _i32, _i32, _vp, _vp,
                                      _i32, _i32, _c.c_double, _c.c_double, _vp, _i32, _i32,

@@ -875,9 +875,10 @@ int mjk_cho_fscf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t 
 }
 
 int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_t nden,
-                       double *const *flt, const int32_t *nforb, const int32_t *niorb,
-                       const double *const *porb, const double *plt, double factxi,
-                       int32_t nscreen, double dmpk, double dfmat, int32_t precision) {
+                       double *const *flt, double *const *klt, const int32_t *nforb,
+                       const int32_t *niorb, const double *const *porb, const double *plt,
+                       double factxi, int32_t nscreen, double dmpk, double dfmat,
+                       int32_t nshell, const int32_t *nbassh, int32_t precision) {
   if (!c) return MJK_ERR_ARG;
   if (nden != 1 && nden != 2)
     return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: invalid parameter nDen");  // cho_lk_scf.F90:93-96
@@ -892,8 +893,39 @@ int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_
     return rc;
   const int64_t n = c->n, np = c->np;
   for (int d = 0; d < nden; ++d)
-    if (!flt[d] || (nocc[d] > 0 && (!porb || !porb[d])))
-      return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: null FLT(d) or Porb(d)");
+    if (!flt[d] || (nocc[d] > 0 && (!porb || !porb[d])) || (klt && !klt[d]))
+      return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: null FLT(d), KLT(d) or Porb(d)");
+  // KLT(d)%SB(s): shell-pair blocks of irrep s's shells (cho_lk_scf.F90:274-292)
+  std::vector<int64_t> kpos;  // full LT index of the embedded basis -> KLT(d)%A0 index
+  if (klt) {
+    kpos.assign(size_t(np), -1);
+    for (int s = 0; s < nsym; ++s) {
+      std::vector<int64_t> sh;
+      int64_t sum = 0;
+      for (int i = 0; i < (nshell > 0 && nbassh ? nshell : 1); ++i) {
+        const int64_t v = nshell > 0 && nbassh ? nbassh[s + nsym * i] : nbas[s];
+        if (v < 0) return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: negative shell size");
+        sh.push_back(v);
+        sum += v;
+      }
+      if (sum != nbas[s])
+        return set_error(c, MJK_ERR_ARG, "cho_lk_scf_sym: shell sizes of an irrep do not add up to its nbas");
+      int64_t off = m.toff[s], first_a = 0;
+      for (size_t sa = 0; sa < sh.size(); ++sa) {
+        int64_t first_b = 0;
+        for (size_t sb = 0; sb <= sa; ++sb) {
+          for (int64_t bl = 0; bl < sh[sb]; ++bl)
+            for (int64_t al = (sa == sb ? bl : 0); al < sh[sa]; ++al) {
+              const int64_t a = m.off[s] + first_a + al, b = m.off[s] + first_b + bl;
+              kpos[size_t(a * (a + 1) / 2 + b)] = off + (sa == sb ? al * (al + 1) / 2 + bl : sh[sa] * bl + al);
+            }
+          off += sa == sb ? sh[sa] * (sh[sa] + 1) / 2 : sh[sa] * sh[sb];
+          first_b += sh[sb];
+        }
+        first_a += sh[sa];
+      }
+    }
+  }
   // Porb(d)%SB(s)%A2(nBas(s),nBas(s)), AO index fastest, its first nForb+nIorb
   // columns the orbitals: they become columns k0(s).. of an n x n matrix
   std::vector<double> pfull(static_cast<size_t>(np), 0.0), ffull(static_cast<size_t>(np * nden), 0.0);
@@ -917,12 +949,18 @@ int mjk_cho_lk_scf_sym(mjk_context *c, int32_t nsym, const int32_t *nbas, int32_
   const int32_t nf0[2] = {0, 0};
   double *fp[2] = {ffull.data(), ffull.data() + np};
   const double *cp[2] = {cfull.data(), cfull.data() + n * n};
-  if ((rc = mjk_cho_lk_scf(c, nden, fp, nullptr, nf0, nocc, cp, pfull.data(), factxi, nscreen, dmpk,
-                           dfmat, 0, nullptr, precision)))
+  std::vector<double> kfull(klt ? size_t(np * nden) : 0, 0.0);  // -FactXI K, plain LT of the embedding
+  double *kp[2] = {kfull.data(), kfull.data() + (klt ? np : 0)};
+  if ((rc = mjk_cho_lk_scf(c, nden, fp, klt ? kp : nullptr, nf0, nocc, cp, pfull.data(), factxi, nscreen,
+                           dmpk, dfmat, 0, nullptr, precision)))
     return rc;
   m.each([&](int64_t t, int64_t g) {
     for (int d = 0; d < nden; ++d) flt[d][t] = ffull[size_t(d * np + g)];
   });
+  if (klt)  // K is block diagonal: only the irrep blocks have a place in KLT
+    for (int d = 0; d < nden; ++d)
+      for (int64_t g = 0; g < np; ++g)
+        if (kpos[size_t(g)] >= 0) klt[d][kpos[size_t(g)]] += kfull[size_t(d * np + g)];
   return MJK_OK;
 }
 

@@ -226,15 +226,19 @@ class JKContext:
 
     def cho_lk_scf_sym(self, nbas, nden: int, flt: np.ndarray, nforb, niorb, porb,
                        plt: np.ndarray, factxi: float, nscreen: int = 10, dmpk: float = 0.1,
-                       dfmat: float = 0.0, precision="fp64") -> int:
-        """CHO_LK_SCF with point-group symmetry (storage as cho_fscf_sym).
+                       dfmat: float = 0.0, klt=None, nbassh=None, precision="fp64") -> int:
+        """CHO_LK_SCF with point-group symmetry (storage as cho_fscf_sym),
+        unscreened.
 
         porb[d][s]: (nbas[s], >= nocc(s)) AO x orbital matrices (the columns of
         Porb(d)%SB(s)%A2(nBas,nBas)); plt: folded total density in irrep LT
-        blocks; flt updated in place.  No KLT output."""
+        blocks; flt updated in place.  klt: None or the same form as flt,
+        accumulated with -FactXI K in each irrep's shell-pair-blocked storage;
+        nbassh: (nsym, nshell) shell sizes nBasSh (None: one shell per irrep)."""
         nsym = len(nbas)
         ntri = sum(int(b) * (int(b) + 1) // 2 for b in nbas)
         fl = _rows_inout(flt, nden, ntri, "flt")
+        kl = _rows_inout(klt, nden, ntri, "klt") if klt is not None else None
         nb = np.ascontiguousarray(nbas, dtype=np.int32)
         nf = np.zeros((max(nden, 1), 8), dtype=np.int32)
         ni = np.zeros((max(nden, 1), 8), dtype=np.int32)
@@ -250,10 +254,16 @@ class JKContext:
                 parts.append(sq.T.reshape(-1))
             pc.append(np.concatenate(parts) if parts else np.zeros(1))
         p = _host_f64(plt).reshape(-1)
+        nshell = 0
+        sh = np.zeros(1, dtype=np.int32)
+        if nbassh is not None:
+            shm = np.asarray(nbassh, dtype=np.int32).reshape(nsym, -1)
+            nshell = shm.shape[1]
+            sh = np.ascontiguousarray(shm.T).reshape(-1)  # nBasSh(iSym, iSh), iSym fastest
         rc = self.lib.mjk_cho_lk_scf_sym(self._h, int(nsym), _ptr(nb), int(nden), _ptr_array(fl),
-                                         _ptr(nf), _ptr(ni), _ptr_array(pc), _ptr(p), float(factxi),
-                                         int(nscreen), float(dmpk), float(dfmat),
-                                         precision_code(precision))
+                                         _ptr_array(kl) if kl is not None else None, _ptr(nf), _ptr(ni),
+                                         _ptr_array(pc), _ptr(p), float(factxi), int(nscreen), float(dmpk),
+                                         float(dfmat), int(nshell), _ptr(sh), precision_code(precision))
         self._check(rc)
         return rc
 

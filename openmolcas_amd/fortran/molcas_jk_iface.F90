@@ -95,12 +95,12 @@ module molcas_jk_iface
       real(c_double), intent(in) :: dlt(*)
       real(c_double), value :: exfac
     end function mjk_cho_fscf_sym
-    integer(c_int) function mjk_cho_lk_scf_sym(ctx, nsym, nbas, nden, flt, nforb, niorb, porb, plt, &
-        factxi, nscreen, dmpk, dfmat, prec) bind(C, name='mjk_cho_lk_scf_sym')
+    integer(c_int) function mjk_cho_lk_scf_sym(ctx, nsym, nbas, nden, flt, klt, nforb, niorb, porb, plt, &
+        factxi, nscreen, dmpk, dfmat, nshell, nbassh, prec) bind(C, name='mjk_cho_lk_scf_sym')
       import :: c_ptr, c_int, c_int32_t, c_double
-      type(c_ptr), value :: ctx
-      integer(c_int32_t), value :: nsym, nden, nscreen, prec
-      integer(c_int32_t), intent(in) :: nbas(*), nforb(*), niorb(*)
+      type(c_ptr), value :: ctx, klt
+      integer(c_int32_t), value :: nsym, nden, nscreen, nshell, prec
+      integer(c_int32_t), intent(in) :: nbas(*), nforb(*), niorb(*), nbassh(*)
       type(c_ptr), intent(in) :: flt(*), porb(*)
       real(c_double), intent(in) :: plt(*)
       real(c_double), value :: factxi, dmpk, dfmat
@@ -257,7 +257,7 @@ contains
   ! CHO_LK_SCF with nSym irreps: FLTd, PLT and Porbd (MSQ A2(nBas(s),nBas(s))
   ! blocks) as the %A0 memory of the DSBA types; KLT is not returned.
   subroutine cho_lk_scf_sym_gpu(rc, nSym, nBas, nDen, FLT1, FLT2, nForb, nIorb, Porb1, Porb2, PLT, FactXI, &
-                                nScreen, dmpk, dFmat, prec)
+                                nScreen, dmpk, dFmat, prec, KLT1, KLT2, nShell, nBasSh)
     integer, intent(out) :: rc
     integer, intent(in) :: nSym, nBas(nSym), nDen, nForb(8, *), nIorb(8, *), nScreen
     real(c_double), intent(inout), target :: FLT1(*), FLT2(*)
@@ -265,8 +265,14 @@ contains
     real(c_double), intent(in) :: PLT(*)
     real(8), intent(in) :: FactXI, dmpk, dFmat
     integer, intent(in), optional :: prec
-    integer(c_int32_t) :: nb(8), nf(8, 2), ni(8, 2)
+    ! KLT(1:nDen)%A0 (irrep blocks in shell-pair storage) and nBasSh(nSym, nShell)
+    real(c_double), intent(inout), target, optional :: KLT1(*), KLT2(*)
+    integer, intent(in), optional :: nShell, nBasSh(nSym, *)
+    integer(c_int32_t) :: nb(8), nf(8, 2), ni(8, 2), nsh
+    integer(c_int32_t), allocatable :: sh(:)
     type(c_ptr) :: pf(2), pp(2)
+    type(c_ptr), target :: pk(2)
+    type(c_ptr) :: kp
     nb = 0
     nf = 0
     ni = 0
@@ -281,9 +287,22 @@ contains
       pf(2) = c_loc(FLT2(1))
       pp(2) = c_loc(Porb2(1))
     end if
-    rc = int(mjk_cho_lk_scf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), pf, nf, ni, &
+    kp = c_null_ptr
+    if (present(KLT1)) then
+      pk(1) = c_loc(KLT1(1))
+      pk(2) = pk(1)
+      if (nDen == 2 .and. present(KLT2)) pk(2) = c_loc(KLT2(1))
+      kp = c_loc(pk)
+    end if
+    nsh = 0
+    if (present(nShell) .and. present(nBasSh)) nsh = int(nShell, c_int32_t)
+    allocate(sh(max(1, nSym*nsh)))
+    sh = 0
+    if (nsh > 0) sh(1:nSym*nsh) = int(reshape(nBasSh(1:nSym, 1:nsh), [nSym*nsh]), c_int32_t)
+    rc = int(mjk_cho_lk_scf_sym(jk_ctx, int(nSym, c_int32_t), nb, int(nDen, c_int32_t), pf, kp, nf, ni, &
                                 pp, PLT, real(FactXI, c_double), int(nScreen, c_int32_t), &
-                                real(dmpk, c_double), real(dFmat, c_double), prec_of(prec)))
+                                real(dmpk, c_double), real(dFmat, c_double), nsh, sh, prec_of(prec)))
+    deallocate(sh)
   end subroutine cho_lk_scf_sym_gpu
 
   ! CHO_FOCK_DFT_RED(irc,DLT,FLT), nSym = 1: FLT = J[DLT] (pure DFT, no exchange).

@@ -91,3 +91,82 @@ def test_cho_lk_casscf_symmetry_matches_c1_oracle(gpu, do_active):
         jk.cho_lk_casscf_sym(NBAS, q["dsym"], fl, q["porb1"], w, q["factxi"], nchi, q["naorb"], q["nchm"],
                              q["porb3"], q["porb2"], do_active, q["cmo"], q["nfro"], q["norb"])
     jk.close()
+
+
+def _fold_klt(klt_sym, nbassh):
+    """Adds KLT(d)%SB(s) to a zero FLT the way CHO_LK_SCF folds it
+    (cho_lk_scf.F90:963-1028): shell pairs iaSh > ibSh rectangular with ia
+    fastest, diagonal shell pairs lower-triangular, block offsets accumulated
+    over the shell pairs (cho_lk_scf.F90:274-292)."""
+    out = np.zeros_like(klt_sym)
+    toff = 0
+    for s, nb in enumerate(NBAS):
+        sh = list(nbassh[s])
+        koff = np.concatenate([[0], np.cumsum(sh)])
+        shp_off, acc = {}, 0
+        for ia in range(len(sh)):
+            for ib in range(ia + 1):
+                shp_off[(ia, ib)] = acc
+                acc += sh[ia] * sh[ib] - (sh[ia] * (sh[ia] - 1) // 2 if ia == ib else 0)
+        for ia_sh in range(len(sh)):
+            for ib_sh in range(ia_sh):
+                o = shp_off[(ia_sh, ib_sh)]
+                for ib in range(sh[ib_sh]):
+                    for ia in range(sh[ia_sh]):
+                        ag, bg = koff[ia_sh] + ia, koff[ib_sh] + ib
+                        out[toff + ag * (ag + 1) // 2 + bg] += klt_sym[toff + o + sh[ia_sh] * ib + ia]
+            o = shp_off[(ia_sh, ia_sh)]
+            for ib in range(sh[ia_sh]):
+                for ia in range(ib, sh[ia_sh]):
+                    ag, bg = koff[ia_sh] + ia, koff[ia_sh] + ib
+                    out[toff + ag * (ag + 1) // 2 + bg] += klt_sym[toff + o + ia * (ia + 1) // 2 + ib]
+        toff += nb * (nb + 1) // 2
+    return out
+
+
+@pytest.mark.parametrize("nden", [1, 2])
+def test_cho_lk_scf_symmetry_klt(gpu, nden):
+    """KLT of CHO_LK_SCF with two irreps in each irrep's shell-pair storage:
+    folded back the reference's way it equals -FactXI K of the oracle's C1
+    build restricted to the irrep blocks (and FLT is unchanged by asking for
+    it)."""
+    import oracle
+
+    p = make_problem(nden, 0.7, seed=17 + nden)
+    n, factxi = p["n"], 0.7
+    nbassh = [[3, 4], [1, 3]]  # nBasSh(iSym, iSh)
+    st = oracle.CholStore(n, [np.array(p["rs"][0]), np.array(p["rs"][1])],
+                          [(0, v) for v in p["V1"]] + [(1, v) for v in p["V2"]])
+    cols = []  # the orbitals in the full basis, as make_problem builds them
+    off = [0, NBAS[0]]
+    for d in range(nden):
+        c = []
+        for s in range(2):
+            for k in range(p["porb"][d][s].shape[1]):
+                v = np.zeros(n)
+                v[off[s]:off[s] + NBAS[s]] = p["porb"][d][s][:, k]
+                c.append(v)
+        cols.append(np.stack(c, axis=1))
+    ii, jj = np.tril_indices(n)
+    dfull = np.zeros(n * (n + 1) // 2)
+    gidx = np.array([(off[s] + a) * (off[s] + a + 1) // 2 + off[s] + b
+                     for s in range(2) for a in range(NBAS[s]) for b in range(a + 1)])
+    dfull[gidx] = p["dsym"]
+    _, kfull = oracle.cho_lk_scf(st, np.zeros((nden, n * (n + 1) // 2)), [c.shape[1] for c in cols], cols,
+                                 dfull, factxi)
+    want = kfull[:, gidx]
+    jk = JKContext(n, p["V1"].shape[0] + p["V2"].shape[0], device=gpu)
+    jk.set_reduced_set(0, np.array(p["rs"][0], dtype=np.int64))
+    jk.set_reduced_set(1, np.array(p["rs"][1], dtype=np.int64))
+    jk.upload_vectors(0, 0, p["V1"])
+    jk.upload_vectors(1, p["V1"].shape[0], p["V2"])
+    f1, f2 = p["fsym"].copy(), p["fsym"].copy()
+    klt = np.zeros_like(p["fsym"])
+    jk.cho_lk_scf_sym(NBAS, nden, f1, p["nforb"], p["niorb"], p["porb"], p["dsym"], factxi)
+    jk.cho_lk_scf_sym(NBAS, nden, f2, p["nforb"], p["niorb"], p["porb"], p["dsym"], factxi,
+                      klt=klt, nbassh=nbassh)
+    jk.close()
+    assert np.array_equal(f1, f2)
+    for d in range(nden):
+        got = _fold_klt(klt[d], nbassh)
+        assert np.max(np.abs(got - want[d])) <= 1e-11 * np.max(np.abs(want[d]))
