@@ -547,9 +547,167 @@ int density_or(Context *c, int nocc, const double *corb, const double *p_lt, con
   return MJK_OK;
 }
 
+// ---- Z^J = X^J C, lower blocks only ------------------------------------------
+// Z^J = C^T L^J C is symmetric, so only its 16 x 16 blocks (i, j), j <= i are
+// multiplied (OC16 = ceil(nocc/16) block rows, so at most 15 padding rows
+// instead of a 64-wide tile's 63) and each off-diagonal block is also written
+// transposed.  One workgroup per vector.  The block rows are dealt to the four
+// waves at compile time (largest first, each to the wave with the fewest
+// blocks), so every fragment index is static.  Per 16-row contraction chunk
+// the bank rows C[a0 .. a0+15][0 .. 16 OC16) go through LDS (double-buffered,
+// one barrier per chunk); a wave's X fragments are one 32-byte load per block
+// row, lane group g taking the contraction indices 4g .. 4g+3 in the four
+// MFMA steps (the B reads from LDS use the same order).
+struct ZPlan {
+  int nrows[4];
+  int rows[4][16];
+  int nblk[4];
+};
+
+constexpr ZPlan zplan(int oc16) {
+  ZPlan p{};
+  for (int i = oc16 - 1; i >= 0; --i) {  // row i holds i + 1 lower blocks
+    int best = 0;
+    for (int w = 1; w < 4; ++w)
+      if (p.nblk[w] < p.nblk[best]) best = w;
+    p.rows[best][p.nrows[best]++] = i;
+    p.nblk[best] += i + 1;
+  }
+  return p;
+}
+
+template <int OC16, typename TX>
+__global__ __launch_bounds__(kThreads) void k_zsym(const TX *__restrict__ X, int64_t sx, int ldx,
+                                                  const double *__restrict__ cb, int ldc, int n,
+                                                  int o, double *__restrict__ z, int64_t sz,
+                                                  int ldz) {
+  constexpr int OC = 16 * OC16;
+  constexpr ZPlan P = zplan(OC16);
+  constexpr int CL = (16 * OC) / kThreads > 0 ? (16 * OC) / kThreads : 1;  // bank doubles per thread
+  __shared__ double sC[2][16 * OC];
+  const int64_t J = blockIdx.x;
+  const TX *__restrict__ xj = X + J * sx;
+  const int tid = threadIdx.x, lane = tid & 63, gq = lane >> 4, w = tid >> 6;
+  const int nch = (n + 15) / 16;
+  double rc[CL];
+  auto fetch = [&](int a0) {
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const int e = tid + kThreads * q;  // 16 * OC is a multiple of 256 or below it
+      rc[q] = e < 16 * OC ? cb[(a0 + e / OC) * int64_t(ldc) + e % OC] : 0.0;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < CL; ++q) {
+      const int e = tid + kThreads * q;
+      if (e < 16 * OC) sC[buf][e] = rc[q];
+    }
+  };
+  auto wave = [&](auto wc) {
+    constexpr int W = decltype(wc)::value;
+    constexpr int NR = P.nrows[W];
+    v4d acc[NR > 0 ? NR : 1][OC16];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int j = 0; j < OC16; ++j) acc[r][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    typedef TX TX4 __attribute__((ext_vector_type(4)));
+    TX4 xa[NR > 0 ? NR : 1], xn[NR > 0 ? NR : 1];
+    auto loadx = [&](int a0, TX4 (&dst)[NR > 0 ? NR : 1]) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        dst[r] = *reinterpret_cast<const TX4 *>(xj + int64_t(16 * P.rows[W][r] + (lane & 15)) * ldx + a0 +
+                                                4 * gq);
+    };
+    fetch(0);
+    stash(0);
+    loadx(0, xa);
+    if (nch > 1) fetch(16);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+      const int cur = c & 1;
+      if (c + 1 < nch) loadx(16 * (c + 1), xn);
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const double *__restrict__ brow = sC[cur] + (4 * gq + ks) * OC + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const double a = double(xa[r][ks]);
+#pragma unroll
+          for (int j = 0; j < OC16; ++j)
+            if (j <= P.rows[W][r]) acc[r][j] = mfma64(a, brow[16 * j], acc[r][j]);
+        }
+      }
+      if (c + 1 < nch) stash(cur ^ 1);
+      if (c + 2 < nch) fetch(16 * (c + 2));
+#pragma unroll
+      for (int r = 0; r < NR; ++r) xa[r] = xn[r];
+      __syncthreads();
+    }
+    double *__restrict__ zj = z + J * sz;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int i = P.rows[W][r];
+#pragma unroll
+      for (int j = 0; j < OC16; ++j) {
+        if (j > i) continue;
+        const int l = 16 * j + (lane & 15);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int k = 16 * i + 4 * rr + gq;
+          if (k < o && l < o) {
+            zj[int64_t(k) * ldz + l] = acc[r][j][rr];
+            if (j < i) zj[int64_t(l) * ldz + k] = acc[r][j][rr];
+          }
+        }
+      }
+    }
+  };
+  // every wave runs the chunk loop (its barriers) even when it owns no rows
+  switch (w) {
+    case 0: wave(std::integral_constant<int, 0>{}); break;
+    case 1: wave(std::integral_constant<int, 1>{}); break;
+    case 2: wave(std::integral_constant<int, 2>{}); break;
+    default: wave(std::integral_constant<int, 3>{}); break;
+  }
+}
+
+template <int OC16, typename TX>
+void launch_zsym(Context *c, const TX *X, int64_t sx, const double *cb, int ldc, int nocc, double *z,
+                 int64_t sz, int ldz, int64_t jn) {
+  hipLaunchKernelGGL((k_zsym<OC16, TX>), dim3(unsigned(jn)), dim3(kThreads), 0, c->stream, X, sx,
+                     c->n_pad, cb, ldc, c->n, nocc, z, sz, ldz);
+}
+
+// Z^J = X^J C for jn vectors (X rows of n_pad, sx apart per vector; bank cb of
+// row stride ldc), Z^J at z + J sz with row stride ldz; nocc <= 128.
+template <typename TX>
+int zsym(Context *c, const TX *X, int64_t sx, const double *cb, int ldc, int nocc, double *z,
+         int64_t sz, int ldz, int64_t jn) {
+  if (nocc < 1 || nocc > 128 || jn < 1) return MJK_ERR_ARG;
+  if (ldc < 16 * ((nocc + 15) / 16)) return set_error(c, MJK_ERR_ARG, "zsym: bank narrower than the blocks");
+  switch ((nocc + 15) / 16) {
+    case 1: launch_zsym<1>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+    case 2: launch_zsym<2>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+    case 3: launch_zsym<3>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+    case 4: launch_zsym<4>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+    case 5: launch_zsym<5>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+    case 6: launch_zsym<6>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+    case 7: launch_zsym<7>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+    default: launch_zsym<8>(c, X, sx, cb, ldc, nocc, z, sz, ldz, jn); break;
+  }
+  return hip_check(c, hipGetLastError(), "zsym launch");
+}
+
 }  // namespace
 
 int z_from_x(Context *c, const void *X, bool x_f32, int ldc, int nocc, int64_t jn, double *z) {
+  const double *cb = static_cast<const double *>(c->w->cbank.ptr);
+  const int64_t sx = int64_t(ldc) * c->n_pad, sz = int64_t(nocc) * nocc;
+  if (nocc <= 128)
+    return x_f32 ? zsym(c, static_cast<const float *>(X), sx, cb, ldc, nocc, z, sz, nocc, jn)
+                 : zsym(c, static_cast<const double *>(X), sx, cb, ldc, nocc, z, sz, nocc, jn);
   GemmBatch g{};
   if (x_f32) g.A32 = static_cast<const float *>(X);
   else g.A = static_cast<const double *>(X);
@@ -634,12 +792,18 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
     phase_end(c, MJK_PHASE_HALF_TRANSFORM);
     phase_begin(c);
     GemmBatch g{};
-    // Z[J] = X[J] C   (o x o)
-    g.A = X; g.sA = int64_t(ldc) * n_pad; g.lda = n_pad;
-    g.B = cb; g.sB = 0; g.ldb = ldc;
-    g.O = Z; g.sO = int64_t(ldc) * ldc; g.ldo = ldc;
-    g.m = nocc; g.n = nocc; g.k = n; g.alpha = 1.0;
-    if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
+    // Z[J] = X[J] C   (o x o), the same kernel as the forward's saved Z
+    if (nocc <= 128) {
+      if ((rc = zsym(c, static_cast<const double *>(X), int64_t(ldc) * n_pad, cb, ldc, nocc, Z,
+                     int64_t(ldc) * ldc, ldc, jn)))
+        return rc;
+    } else {
+      g.A = X; g.sA = int64_t(ldc) * n_pad; g.lda = n_pad;
+      g.B = cb; g.sB = 0; g.ldb = ldc;
+      g.O = Z; g.sO = int64_t(ldc) * ldc; g.ldo = ldc;
+      g.m = nocc; g.n = nocc; g.k = n; g.alpha = 1.0;
+      if ((rc = gemm<false, 0>(c, g, int(jn)))) return rc;
+    }
     // dL[J] = s * (alpha V_J P + beta C Z[J] C^T), lower triangle, packed
     if (grad_rows(c, nocc, ldc, Z, int64_t(ldc) * ldc, ldc, v + j0, P, alpha, beta, dl + j0 * ldd,
                   ldd, jn) != MJK_OK) {  // nocc > 128: W = C Z first

@@ -123,6 +123,26 @@ def test_saved_z_gradient_equals_recompute(gpu, fused):
     jk.close()
 
 
+@pytest.mark.parametrize("n,M,o", [(40, 7, 5), (64, 5, 16), (130, 6, 33), (200, 5, 64), (150, 4, 70),
+                                   (333, 3, 100), (170, 3, 112), (129, 3, 128), (257, 2, 129)])
+def test_saved_z_matches_dense(gpu, n, M, o):
+    """Z^J = C^T L^J C written by the forward (lower 16 x 16 blocks mirrored,
+    every block-row count 1..8; nocc 129 takes the general GEMM) against
+    numpy, FP64 to 1e-12 and from the FP32 half-transform to 1e-5."""
+    Ln = chol_vectors_np(n, M, seed=90 + o)
+    Cn = orbitals_np(n, o, seed=91 + o)
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(torch.tensor(Ln, device=gpu))
+    C = torch.tensor(Cn, device=gpu)
+    Zr = np.einsum("ak,jab,bl->jkl", Cn, np.stack([unpack(Ln[j], n) for j in range(M)]), Cn)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        Z = torch.full((M, o, o), float("nan"), dtype=torch.float64, device=gpu)
+        jk.exchange([C], precision=prec, z=Z)
+        err = np.max(np.abs(Z.cpu().numpy() - Zr)) / np.max(np.abs(Zr))
+        assert err <= tol, (prec, err)
+    jk.close()
+
+
 @pytest.mark.parametrize("n,M,o", [(1, 2, 1), (64, 3, 16), (70, 9, 5), (130, 7, 33), (200, 6, 64),
                                    (333, 5, 100), (129, 4, 128), (257, 3, 129)])
 def test_vector_gradient_from_z_matches_dense(gpu, n, M, o):
