@@ -1033,8 +1033,10 @@ template <class P, int OT, int TA, int BK>
 void launch_ht(Context *c, const typename P::T *L, const typename P::T *cb, int64_t j0, int64_t jc,
                typename P::T *X) {
   // two vectors per workgroup while the accumulators stay small (measured
-  // 5-6% faster at OT = 4: half the orbital-fragment LDS reads per MFMA)
-  constexpr int NJ = OT * (TA / 64) <= 4 ? 2 : 1;
+  // 5-6% faster at OT = 4: half the orbital-fragment LDS reads per MFMA), and
+  // at OT = 7 (4% faster; at OT = 5 and 6 it measured 3-7% slower: those keep
+  // one vector, 6 in the 2 x 2 wave layout)
+  constexpr int NJ = (OT * (TA / 64) <= 4 || (TA == 64 && OT == 7)) ? 2 : 1;
   const int ntile_a = c->n_pad / TA;
   const int64_t nblocks = ceil_div(jc, NJ) * ntile_a;
   const int64_t grid = round_up(nblocks, 8);
