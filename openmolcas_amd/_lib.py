@@ -96,7 +96,9 @@ SIGNATURES = {
     "mjk_grad_vectors": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _c.c_double, _c.c_double, _vp, _i64]),
     "mjk_grad_vectors_z": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _c.c_double, _c.c_double, _vp,
                                       _i64]),
-    "mjk_sync": (_c.c_int, [_vp]),
+    "mjk_grad_vectors_z_mixed": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _c.c_double, _c.c_double,
+                                            _vp, _i64, _i64]),
+    "mjk_sync":(_c.c_int, [_vp]),
     "mjk_set_profiling": (_c.c_int, [_vp, _i32]),
     "mjk_phase_times": (_c.c_int, [_vp, _vp, _vp]),
     "mjk_measure_peak": (_c.c_int, [_vp, _i32, _vp]),

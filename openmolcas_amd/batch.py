@@ -76,6 +76,7 @@ class MoleculeBatch:
         for st in self.lane_streams:
             st.synchronize()
         self.graph = None
+        self.ladders = []   # per-system ladder plans (tune_ladder)
 
     def autotune(self, precision="fp64"):
         for i, (c, C) in enumerate(zip(self.ctx, self.C)):
@@ -119,7 +120,9 @@ class MoleculeBatch:
         F = h + G
         self.dC[i].copy_(c.grad_orbitals(C, F, 4.0))
         if self.dL[i] is not None:
-            c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i], z=Z)
+            # past the ladder's FP64 head the products C Z^J C^T are formed in FP32
+            tail = self.ladders[i].j_fp32 if precision == "ladder" and Z is not None and self.ladders else None
+            c.grad_vectors(C, V[:, 0].contiguous(), 1.0, -2.0, out=self.dL[i], z=Z, fp32_from=tail)
 
     def step(self, precision="fp64", backward=True, serial=False):
         """Forward (+ backward) of every system: forks from the batch stream to

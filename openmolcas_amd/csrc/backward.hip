@@ -294,6 +294,7 @@ int grad_from_w(Context *c, int nocc, const double *W, int ldc, const double *v,
 // (k, l) at z + J sz + k ldz + l.
 struct GradRows {
   const double *cb;  // orbital bank C[a][k]: n_pad rows of ldc, columns >= nocc zero
+  const float *cb32; // the same bank in FP32 (FP32 products only)
   int ldc;
   const double *z;
   int64_t sz;
@@ -309,6 +310,29 @@ struct GradRows {
 };
 
 typedef double f4d __attribute__((ext_vector_type(4)));
+typedef float f4f __attribute__((ext_vector_type(4)));
+
+// Arithmetic of k_grad_rows: FP64 MFMA, or FP32 MFMA (the split-precision
+// backward's tail vectors: contractions of at most 128 terms, FP64 output).
+// Operand layouts are those of v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32
+// (A[l%16][l/16], B[l/16][l%16]); result row of register r: 4r + l/16 (FP64),
+// 4(l/16) + r (FP32), column l%16.
+struct GR64 {
+  typedef double T;
+  typedef v4d Acc;
+  typedef f4d T4;
+  __device__ static Acc mma(T a, T b, Acc c) { return mfma64(a, b, c); }
+  __device__ static int row(int lane, int r) { return 4 * r + (lane >> 4); }
+  __device__ static const T *bank(const GradRows &g) { return g.cb; }
+};
+struct GR32 {
+  typedef float T;
+  typedef v4f Acc;
+  typedef f4f T4;
+  __device__ static Acc mma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  __device__ static const T *bank(const GradRows &g) { return g.cb32; }
+};
 
 // Work item (vector J, row tile I of 64 AOs).  Phase 1 forms W_I = C_I Z^J (64
 // x OC, OC = 16 OC16 >= nocc) with FP64 MFMA, both operands staged through LDS
@@ -324,18 +348,20 @@ typedef double f4d __attribute__((ext_vector_type(4)));
 // store.  Within a vector the longest row tile runs first; consecutive items
 // run on one XCD.  W never goes through HBM (the two-GEMM form writes n x ldc
 // doubles per vector and reads them back once per column tile).
-template <int OC16>
+template <int OC16, class R>
 __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
+  typedef typename R::T T;
+  typedef typename R::Acc Acc;
   constexpr int OC = 16 * OC16;
   constexpr int WS = OC + 1;                                   // row stride of W_I in LDS
   constexpr int S1 = 2 * (kTM * (kBK + 1) + kBK * WS);         // phase-1 staging
   constexpr int SW = kTM * WS;
-  __shared__ double smem[S1 > SW ? S1 : SW];
-  typedef double RowA[kTM][kBK + 1];
-  typedef double RowB[kBK][WS];
+  __shared__ T smem[S1 > SW ? S1 : SW];
+  typedef T RowA[kTM][kBK + 1];
+  typedef T RowB[kBK][WS];
   RowA *sA = reinterpret_cast<RowA *>(smem);                   // [2]
   RowB *sB = reinterpret_cast<RowB *>(smem + 2 * kTM * (kBK + 1));  // [2]
-  double(*sW)[WS] = reinterpret_cast<double(*)[WS]>(smem);     // aliases the staging
+  T(*sW)[WS] = reinterpret_cast<T(*)[WS]>(smem);               // aliases the staging
 
   const int64_t nblk = gridDim.x;  // multiple of 8
   const int64_t l = (int64_t(blockIdx.x) & 7) * (nblk >> 3) + (int64_t(blockIdx.x) >> 3);
@@ -344,16 +370,16 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
   const int I = g.T - 1 - int(l - J * g.T);
   const int i0 = I * kTM;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, gq = lane >> 4;
-  const double *__restrict__ cb = g.cb;
+  const T *__restrict__ cb = R::bank(g);
   const int ldc = g.ldc, o = g.o;
   const double *__restrict__ zj = g.z + J * g.sz;
   const int ldz = g.ldz;
 
   // ---- phase 1: W_I = C_I Z^J; wave w owns rows 16w..16w+15, all OC columns
-  v4d acc1[OC16];
+  Acc acc1[OC16];
 #pragma unroll
-  for (int t = 0; t < OC16; ++t) acc1[t] = v4d{0.0, 0.0, 0.0, 0.0};
-  double ra[4], rb[OC16];
+  for (int t = 0; t < OC16; ++t) acc1[t] = Acc{0, 0, 0, 0};
+  T ra[4], rb[OC16];
   uint32_t zok = 0;  // bit it: Z element it lies inside nocc x nocc (zeroed at stash time)
   auto fetch1 = [&](int k0) {
 #pragma unroll
@@ -366,7 +392,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
     for (int it = 0; it < OC16; ++it) {
       const int e = tid + kThreads * it, kr = e / OC, cc = e % OC, k = k0 + kr;
       const bool ok = k < o && cc < o;
-      rb[it] = zj[ok ? uint32_t(k) * uint32_t(ldz) + uint32_t(cc) : 0u];
+      rb[it] = T(zj[ok ? uint32_t(k) * uint32_t(ldz) + uint32_t(cc) : 0u]);
       zok |= ok ? 1u << it : 0u;
     }
   };
@@ -379,7 +405,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
 #pragma unroll
     for (int it = 0; it < OC16; ++it) {
       const int e = tid + kThreads * it;
-      sB[buf][e / OC][e % OC] = ((zok >> it) & 1u) ? rb[it] : 0.0;
+      sB[buf][e / OC][e % OC] = ((zok >> it) & 1u) ? rb[it] : T(0);
     }
   };
   fetch1(0);
@@ -391,9 +417,9 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
     const int cur = c & 1;
 #pragma unroll
     for (int ks = 0; ks < kBK / 4; ++ks) {
-      const double a = sA[cur][16 * w + (lane & 15)][4 * ks + gq];
+      const T a = sA[cur][16 * w + (lane & 15)][4 * ks + gq];
 #pragma unroll
-      for (int t = 0; t < OC16; ++t) acc1[t] = mfma64(a, sB[cur][4 * ks + gq][16 * t + (lane & 15)], acc1[t]);
+      for (int t = 0; t < OC16; ++t) acc1[t] = R::mma(a, sB[cur][4 * ks + gq][16 * t + (lane & 15)], acc1[t]);
     }
     if (c + 1 < OC16) stash1(cur ^ 1);
     if (c + 2 < OC16) fetch1((c + 2) * kBK);
@@ -403,7 +429,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
 #pragma unroll
   for (int t = 0; t < OC16; ++t)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) sW[16 * w + 4 * r + gq][16 * t + (lane & 15)] = acc1[t][r];
+    for (int r = 0; r < 4; ++r) sW[16 * w + R::row(lane, r)][16 * t + (lane & 15)] = acc1[t][r];
   __syncthreads();
 
   // ---- phase 2: the blocks W_I C_j^T, j = 0..I, as one stream of chunks
@@ -412,20 +438,21 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
   const double vj = g.gv * g.v[J];
   const double *__restrict__ P = g.p;
   double *__restrict__ G = g.g + J * g.ldg;
-  f4d bq[2][2];  // [stage][jj]: C_j fragments of the chunks in flight
-  auto loadB = [&](int q, f4d (&dst)[2]) {
+  typedef typename R::T4 T4;
+  T4 bq[2][2];  // [stage][jj]: C_j fragments of the chunks in flight
+  auto loadB = [&](int q, T4 (&dst)[2]) {
     const int j = q / OC16, kc = q - j * OC16;
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj) {
       const uint32_t row = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
-      dst[jj] = *reinterpret_cast<const f4d *>(cb + row * uint32_t(ldc) + uint32_t(kc * kBK + 4 * gq));
+      dst[jj] = *reinterpret_cast<const T4 *>(cb + row * uint32_t(ldc) + uint32_t(kc * kBK + 4 * gq));
     }
   };
-  v4d acc2[2][2];
+  Acc acc2[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = Acc{0, 0, 0, 0};
   // P values of the tile in progress, loaded at its first chunk so the
   // epilogue only stores
   double pv[2][2][4];
@@ -437,7 +464,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
       for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + 4 * r + gq);
+          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + R::row(lane, r));
           const uint32_t gj = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
           const bool ok = gi < uint32_t(g.n) && gi >= gj;
           pi[i][jj][r] = ok ? gi * (gi + 1u) / 2u + gj : ~0u;
@@ -453,32 +480,32 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
         for (int r = 0; r < 4; ++r) {
           const uint32_t p = pi[i][jj][r];
           if (p == ~0u) continue;
-          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + 4 * r + gq);
+          const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + R::row(lane, r));
           const uint32_t gj = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
-          G[p] = (gi == gj ? 1.0 : 2.0) * (vj * pv[i][jj][r] + g.beta * acc2[i][jj][r]);
+          G[p] = (gi == gj ? 1.0 : 2.0) * (vj * pv[i][jj][r] + g.beta * double(acc2[i][jj][r]));
         }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = Acc{0, 0, 0, 0};
   };
   // One chunk of the stream; the fragment stage S is a compile-time constant
   // (the loop below is unrolled by two), so the stages never move.
   auto step = [&](int q, auto stage) {
     constexpr int S = decltype(stage)::value;
-    const f4d b0 = bq[S][0], b1 = bq[S][1];
+    const T4 b0 = bq[S][0], b1 = bq[S][1];
     if (q + 2 < Q) loadB(q + 2, bq[S]);
     const int j = q / OC16, kc = q - j * OC16;
     if (kc == 0) load_p(j);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      double af[2];
+      T af[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) af[i] = sW[wm * 32 + i * 16 + (lane & 15)][kc * kBK + 4 * gq + ks];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        acc2[i][0] = mfma64(af[i], b0[ks], acc2[i][0]);
-        acc2[i][1] = mfma64(af[i], b1[ks], acc2[i][1]);
+        acc2[i][0] = R::mma(af[i], b0[ks], acc2[i][0]);
+        acc2[i][1] = R::mma(af[i], b1[ks], acc2[i][1]);
       }
     }
     if (kc == OC16 - 1) epilogue(j);
@@ -495,18 +522,26 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
 
 template <int OC16>
 void launch_grad_rows(Context *c, const GradRows &g) {
-  hipLaunchKernelGGL((k_grad_rows<OC16>), dim3(unsigned(round_up(g.items, 8))), dim3(kThreads), 0,
-                     c->stream, g);
+  if (g.cb32)
+    hipLaunchKernelGGL((k_grad_rows<OC16, GR32>), dim3(unsigned(round_up(g.items, 8))), dim3(kThreads), 0,
+                       c->stream, g);
+  else
+    hipLaunchKernelGGL((k_grad_rows<OC16, GR64>), dim3(unsigned(round_up(g.items, 8))), dim3(kThreads), 0,
+                       c->stream, g);
 }
 
 // dL of the jn vectors from Z in one fused launch (nocc <= 128); returns
 // MJK_ERR_ARG when nocc is larger (the caller then forms W = C Z first).
+// cb32: the bank in FP32 for FP32 products, or null for FP64.
 int grad_rows(Context *c, int nocc, int ldc, const double *z, int64_t sz, int ldz, const double *v,
-              const double *P, double alpha, double beta, double *dl, int64_t ldd, int64_t jn) {
+              const double *P, double alpha, double beta, double *dl, int64_t ldd, int64_t jn,
+              const float *cb32) {
   const int oc16 = (nocc + 15) / 16;
   if (oc16 > 8 || int64_t(c->n_pad) * ldc >= (int64_t(1) << 28)) return MJK_ERR_ARG;
+  if (jn < 1) return MJK_OK;
   GradRows g{};
   g.cb = static_cast<const double *>(c->w->cbank.ptr);
+  g.cb32 = cb32;
   g.ldc = ldc;
   g.z = z;
   g.sz = sz;
@@ -806,7 +841,7 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
     }
     // dL[J] = s * (alpha V_J P + beta C Z[J] C^T), lower triangle, packed
     if (grad_rows(c, nocc, ldc, Z, int64_t(ldc) * ldc, ldc, v + j0, P, alpha, beta, dl + j0 * ldd,
-                  ldd, jn) != MJK_OK) {  // nocc > 128: W = C Z first
+                  ldd, jn, nullptr) != MJK_OK) {  // nocc > 128: W = C Z first
       g = GemmBatch{};
       g.A = cb; g.sA = 0; g.lda = ldc;
       g.B = Z; g.sB = int64_t(ldc) * ldc; g.ldb = ldc;
@@ -820,14 +855,22 @@ extern "C" int mjk_grad_vectors(mjk_context *c, int32_t nocc, const double *corb
   return hip_check(c, hipGetLastError(), "grad_vectors launch");
 }
 
-extern "C" int mjk_grad_vectors_z(mjk_context *c, int32_t nocc, const double *corb,
-                                  const double *p_lt, const double *v, const double *z,
-                                  double alpha, double beta, double *dl, int64_t ldd) {
+namespace {
+__global__ void k_bank_f32(const double *__restrict__ src, float *__restrict__ dst, int64_t cnt) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < cnt; i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = float(src[i]);
+}
+
+// dE/dL from a given Z; vectors [j_fp32, M) take FP32 products (nocc <= 128).
+int grad_vectors_z_impl(Context *c, int32_t nocc, const double *corb, const double *p_lt,
+                        const double *v, const double *z, double alpha, double beta, double *dl,
+                        int64_t ldd, int64_t j_fp32, const char *who) {
   if (!c) return MJK_ERR_ARG;
   if (nocc < 1 || nocc > 256 || !corb || !v || !z || !dl || ldd < c->np)
-    return set_error(c, MJK_ERR_ARG, "grad_vectors_z: bad arguments");
+    return set_error(c, MJK_ERR_ARG, std::string(who) + ": bad arguments");
   const int n_pad = c->n_pad;
   const int64_t M = c->nchol;
+  if (j_fp32 < 0 || j_fp32 > M) return set_error(c, MJK_ERR_ARG, std::string(who) + ": split outside [0, nchol]");
   int rc;
   std::vector<int> off, opad;
   int ot = 0, tot = 0;
@@ -835,12 +878,23 @@ extern "C" int mjk_grad_vectors_z(mjk_context *c, int32_t nocc, const double *co
   if ((rc = pack_bank(c, 1, &nocc, sets, off, opad, &ot, &tot))) return rc;
   const int ldc = 16 * ot;
   const double *cb = static_cast<const double *>(c->w->cbank.ptr);
-  if (nocc <= 128) {  // one fused launch: W = C Z stays on chip
+  if (nocc <= 128) {  // one fused launch per precision: W = C Z stays on chip
     const double *P = nullptr;
     if ((rc = density_or(c, nocc, corb, p_lt, &P))) return rc;
+    const float *cb32 = nullptr;
+    if (j_fp32 < M) {
+      const int64_t cnt = int64_t(n_pad) * ldc;
+      if ((rc = ensure(c, c->w->gb32, sizeof(float) * size_t(cnt)))) return rc;
+      hipLaunchKernelGGL(k_bank_f32, dim3(unsigned(std::min<int64_t>(ceil_div(cnt, 256), 4096))), dim3(256), 0,
+                         c->stream, cb, static_cast<float *>(c->w->gb32.ptr), cnt);
+      cb32 = static_cast<const float *>(c->w->gb32.ptr);
+    }
+    const int64_t sz = int64_t(nocc) * nocc;
     phase_begin(c);
-    if ((rc = grad_rows(c, nocc, ldc, z, int64_t(nocc) * nocc, nocc, v, P, alpha, beta, dl, ldd, M)))
-      return set_error(c, rc, "grad_vectors_z: fused gradient launch");
+    if ((rc = grad_rows(c, nocc, ldc, z, sz, nocc, v, P, alpha, beta, dl, ldd, j_fp32, nullptr)) ||
+        (rc = grad_rows(c, nocc, ldc, z + j_fp32 * sz, sz, nocc, v + j_fp32, P, alpha, beta,
+                        dl + j_fp32 * ldd, ldd, M - j_fp32, cb32)))
+      return set_error(c, rc, std::string(who) + ": fused gradient launch");
     phase_end(c, MJK_PHASE_BACKWARD);
     return hip_check(c, hipGetLastError(), "grad_vectors_z launch");
   }
@@ -867,4 +921,20 @@ extern "C" int mjk_grad_vectors_z(mjk_context *c, int32_t nocc, const double *co
     phase_end(c, MJK_PHASE_BACKWARD);
   }
   return hip_check(c, hipGetLastError(), "grad_vectors_z launch");
+}
+}  // namespace
+
+extern "C" int mjk_grad_vectors_z(mjk_context *c, int32_t nocc, const double *corb,
+                                  const double *p_lt, const double *v, const double *z,
+                                  double alpha, double beta, double *dl, int64_t ldd) {
+  if (!c) return MJK_ERR_ARG;
+  return grad_vectors_z_impl(c, nocc, corb, p_lt, v, z, alpha, beta, dl, ldd, c->nchol, "grad_vectors_z");
+}
+
+extern "C" int mjk_grad_vectors_z_mixed(mjk_context *c, int32_t nocc, const double *corb,
+                                        const double *p_lt, const double *v, const double *z,
+                                        double alpha, double beta, double *dl, int64_t ldd,
+                                        int64_t j_fp32) {
+  if (!c) return MJK_ERR_ARG;
+  return grad_vectors_z_impl(c, nocc, corb, p_lt, v, z, alpha, beta, dl, ldd, j_fp32, "grad_vectors_z_mixed");
 }

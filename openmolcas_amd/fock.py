@@ -644,19 +644,27 @@ class JKContext:
         return dc
 
     def grad_vectors(self, corb: torch.Tensor, v: torch.Tensor, alpha: float, beta: float,
-                     p_lt=None, out=None, z=None) -> torch.Tensor:
+                     p_lt=None, out=None, z=None, fp32_from=None) -> torch.Tensor:
         """(nchol, npair) packed gradient s_ab (alpha V_J P_ab + beta (C Z^J C^T)_ab).
         z: the Z^J = C^T L^J C an exchange(z=...) or coulomb_exchange(z=...)
-        of the same orbitals wrote (no half-transform then); None recomputes."""
+        of the same orbitals wrote (no half-transform then); None recomputes.
+        fp32_from (with z): vectors J >= fp32_from form C Z^J C^T in FP32."""
         c = _dev_f64(corb, self.device)
         vv = v.to(torch.float64).contiguous()
         dl = out if out is not None else torch.empty((self.nchol, self.npair), dtype=torch.float64,
                                                      device=self.device)
         if z is not None:
             o = self._z_out([c], z)
+            if fp32_from is not None and int(fp32_from) < self.nchol:
+                self._check(self.lib.mjk_grad_vectors_z_mixed(
+                    self._h, o, _ptr(c), _ptr(p_lt), _ptr(vv), _ptr(z), float(alpha), float(beta),
+                    _ptr(dl), _ld(dl), int(fp32_from)))
+                return dl
             self._check(self.lib.mjk_grad_vectors_z(self._h, o, _ptr(c), _ptr(p_lt), _ptr(vv), _ptr(z),
                                                     float(alpha), float(beta), _ptr(dl), _ld(dl)))
             return dl
+        if fp32_from is not None:
+            raise ValueError("grad_vectors: fp32_from needs the saved z")
         self._check(self.lib.mjk_grad_vectors(self._h, c.shape[1], _ptr(c), _ptr(p_lt), _ptr(vv),
                                               float(alpha), float(beta), _ptr(dl), _ld(dl)))
         return dl

@@ -169,6 +169,38 @@ def test_vector_gradient_from_z_matches_dense(gpu, n, M, o):
     jk.close()
 
 
+@pytest.mark.parametrize("n,M,o,split", [(70, 9, 5, 9), (130, 7, 33, 3), (200, 6, 64, 0), (129, 4, 128, 2),
+                                         (257, 3, 96, 1)])
+def test_vector_gradient_fp32_tail(gpu, n, M, o, split):
+    """mjk_grad_vectors_z_mixed: vectors below the split are bit-identical to
+    mjk_grad_vectors_z; from the split on, C Z^J C^T is formed from FP32
+    operands with FP32 sums of o terms twice, so each element stays within the
+    a-priori bound (2 o + 4) 2^-24 |beta| s (|C| |Z^J| |C|^T) of the dense FP64
+    value (the V P term is FP64)."""
+    rng = np.random.default_rng(n + 3 * o + split)
+    Cn = rng.standard_normal((n, o))
+    Zn = rng.standard_normal((M, o, o))
+    Pn = pack(rng.standard_normal((n, n)))
+    vn = rng.standard_normal(M)
+    alpha, beta = 0.75, -2.0
+    jk = JKContext(n, M, device=gpu)
+    jk.upload_packed(np.zeros((M, npair(n))))
+    args = (torch.tensor(Cn, device=gpu), torch.tensor(vn, device=gpu), alpha, beta)
+    kw = dict(p_lt=torch.tensor(Pn, device=gpu), z=torch.tensor(Zn, device=gpu))
+    full = jk.grad_vectors(*args, **kw).cpu().numpy()
+    got = jk.grad_vectors(*args, fp32_from=split, **kw).cpu().numpy()
+    assert np.array_equal(got[:split], full[:split])
+    i, j = np.tril_indices(n)
+    s = np.where(i == j, 1.0, 2.0)
+    for J in range(split, M):
+        want = s * (alpha * vn[J] * Pn + beta * (Cn @ Zn[J] @ Cn.T)[i, j])
+        bound = (2 * o + 4) * 2.0 ** -24 * abs(beta) * s * (np.abs(Cn) @ np.abs(Zn[J]) @ np.abs(Cn).T)[i, j]
+        assert np.all(np.abs(got[J] - want) <= bound + 1e-300), J
+        if o >= 16:  # and it is an FP32 result, not the FP64 one
+            assert np.max(np.abs(got[J] - full[J])) > 0.0
+    jk.close()
+
+
 def test_full_size_backward_n1200_m6000_o150(gpu):
     """Mixed precision plus backward workload: nbas=1200, nchol=6000, nocc=150.
     autograd dE/dC and dE/dL against central differences (step 1e-5) on 32

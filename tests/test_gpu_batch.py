@@ -11,7 +11,8 @@
   in one HIP graph; every replay must reproduce the eager step bit for bit.
   At full size (64 systems) every system's FP64 energy and dE/dC are checked
   against an independent torch FP64 build of J and K, and the tuned ladder
-  keeps every system within 1e-6 Eh (dE/dC within 1e-5) of FP64.
+  keeps every system within 1e-6 Eh (dE/dC within 1e-5, dE/dL of four systems
+  within 1e-5 relative, its tail vectors from the FP32 backward) of FP64.
 """
 import numpy as np
 import pytest
@@ -168,6 +169,10 @@ def test_uneven_molecule_batch_64_graph(gpu):
         worst_c = max(worst_c, torch.max(torch.abs(dc - dC64[i])).item() / torch.max(torch.abs(dc)).item())
     print(f"64-system batch against torch FP64: max rel |dE| {worst_e:.2e}, max rel dE/dC {worst_c:.2e}")
     assert worst_e <= 1e-11 and worst_c <= 1e-10
+    # dE/dL of four systems (smallest, largest, two between) kept for the ladder
+    order = sorted(range(len(shapes)), key=lambda i: shapes[i][0])
+    keep = [order[0], order[len(order) // 3], order[2 * len(order) // 3], order[-1]]
+    dL64 = {i: batch.dL[i].clone() for i in keep}
     E32, dC32 = _run_batch_checks(batch, "fp32")
     dE = torch.max(torch.abs(E32 - E64)).item()
     dCerr = max(torch.max(torch.abs(a - b)).item() for a, b in zip(dC32, dC64))
@@ -179,6 +184,10 @@ def test_uneven_molecule_batch_64_graph(gpu):
     El, dCl = _run_batch_checks(batch, "ladder")
     dE = torch.max(torch.abs(El - E64)).item()
     dCerr = max(torch.max(torch.abs(a - b)).item() for a, b in zip(dCl, dC64))
-    print(f"64-system batch: max |dE| ladder vs fp64 = {dE:.3e}, max |d(dE/dC)| = {dCerr:.3e}")
-    assert dE <= 1e-6 and dCerr <= 1e-5
+    # dE/dL past each ladder's FP64 head comes from the FP32 backward products
+    dLerr = max((torch.max(torch.abs(batch.dL[i] - dL64[i])) / torch.max(torch.abs(dL64[i]))).item()
+                for i in keep)
+    print(f"64-system batch: max |dE| ladder vs fp64 = {dE:.3e}, max |d(dE/dC)| = {dCerr:.3e}, "
+          f"max rel d(dE/dL) = {dLerr:.3e}")
+    assert dE <= 1e-6 and dCerr <= 1e-5 and dLerr <= 1e-5
     batch.close()
