@@ -64,10 +64,40 @@ def parse():
     return p.parse_args()
 
 
+class _BlasTiming:
+    """The oracle's DGEMV calls through the host BLAS (as the reference links
+    one), threads capped at `threads` for OpenMP and the BLAS; restores the
+    plain loops on exit."""
+
+    def __init__(self, threads):
+        self.threads = int(threads)
+
+    def __enter__(self):
+        import oracle
+        from threadpoolctl import threadpool_limits
+
+        oracle.set_threads(self.threads)
+        self.limits = threadpool_limits(self.threads, "blas")
+        self.blas = oracle.use_blas(True)
+        return self
+
+    def __exit__(self, *exc):
+        import oracle
+
+        oracle.use_blas(False)
+        self.limits.restore_original_limits()
+        return False
+
+
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
 def cpu_baseline(n, M, o, seed, sample, repeats=5):
-    """The CPU oracle (C restatement of CHO_FSCF, OpenMP over the host cores) on
-    the same workload; `sample` vectors (all by default), median of `repeats`
-    builds, scaled to the full nchol when sampled."""
+    """The CPU oracle (C restatement of CHO_FSCF, OpenMP over the host cores,
+    its DGEMVs through the host BLAS) on the same workload; `sample` vectors
+    (all by default), median of `repeats` builds, scaled to the full nchol when
+    sampled.  The parity reference is the plain-loop oracle."""
     import numpy as np
 
     import oracle
@@ -79,23 +109,26 @@ def cpu_baseline(n, M, o, seed, sample, repeats=5):
     dlt = fold(C @ C.T)
     st = oracle.CholStore.full(L)
     flt = np.zeros((1, npair(n)))
-    want = oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)  # warm; the parity reference
+    want = oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)  # the parity reference
     ref = (L, C, dlt, want)
+    threads = host_threads()
     times = []
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)
-        times.append(time.perf_counter() - t0)
+    with _BlasTiming(threads) as bt:
+        oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)  # warm
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            oracle.cho_fscf(st, flt, [o], [C], dlt, 0.5, batch=256)
+            times.append(time.perf_counter() - t0)
     dt = float(np.median(times))
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": round(1.0 / (dt * M / ms), 6),
         "unit": "builds/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle CHO_FSCF restatement (C, OpenMP, {threads} threads) on {ms} of {M} vectors, "
-                  f"nbas={n} nocc={o}, batch 256 vectors, median of {repeats}: {dt:.2f} s per build"
-                  + (f", scaled x{M / ms:.1f}" if ms < M else ""),
+        "blas": bt.blas,
+        "sample": f"oracle CHO_FSCF restatement (C, OpenMP, {threads} threads, DGEMVs through {bt.blas}) on "
+                  f"{ms} of {M} vectors, nbas={n} nocc={o}, batch 256 vectors, median of {repeats}: "
+                  f"{dt:.2f} s per build" + (f", scaled x{M / ms:.1f}" if ms < M else ""),
     }, ref
 
 
@@ -363,8 +396,9 @@ def cpu_model():
 
 
 def batch_cpu_baseline(shapes, seed, frac=32):
-    """The CPU oracle (C restatement of CHO_FSCF) on the batch workload: every
-    system's forward J/K build with 1/frac of its vectors (at least 8), on
+    """The CPU oracle (C restatement of CHO_FSCF, its DGEMVs through the host
+    BLAS) on the batch workload: every system's forward J/K build with 1/frac
+    of its vectors (at least 8), on
     all host cores, then the first 16 systems on one core; both scaled to the
     full vector counts (the build is linear in the vector count).  The
     reference has no backward, so the CPU rate counts forward builds."""
@@ -387,21 +421,23 @@ def batch_cpu_baseline(shapes, seed, frac=32):
             tot += (time.perf_counter() - t0) * M / ms
         return tot
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    oracle.set_threads(threads)
-    t_all = run(list(enumerate(shapes)))
-    oracle.set_threads(1)
+    threads = host_threads()
+    with _BlasTiming(threads) as bt:
+        t_all = run(list(enumerate(shapes)))
     sub = list(enumerate(shapes))[:16]
-    t_one = run(sub) * sum(M * o * n * n for n, M, o in shapes) / sum(M * o * n * n for _, (n, M, o) in sub)
+    with _BlasTiming(1):
+        t_one = run(sub) * sum(M * o * n * n for n, M, o in shapes) / sum(M * o * n * n for _, (n, M, o) in sub)
     oracle.set_threads(threads)
     return {
         "value": round(len(shapes) / t_all, 4),
         "unit": "builds/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle CHO_FSCF restatement (C, OpenMP) on every one of the {len(shapes)} systems with "
-                  f"1/{frac} of its vectors (>= 8), forward J/K only (the reference has no backward), "
-                  f"scaled to the full vector counts: {t_all:.1f} s per batch on {threads} threads",
+        "blas": bt.blas,
+        "sample": f"oracle CHO_FSCF restatement (C, OpenMP, DGEMVs through {bt.blas}) on every one of the "
+                  f"{len(shapes)} systems with 1/{frac} of its vectors (>= 8), forward J/K only (the reference "
+                  f"has no backward), scaled to the full vector counts: {t_all:.1f} s per batch on {threads} "
+                  f"threads",
         "single_core": {"value": round(len(shapes) / t_one, 5), "unit": "builds/s", "cores": 1,
                         "sample": "the first 16 systems at 1/{frac} of their vectors, scaled by the work "
                                   "M o n^2 of all 64".replace("{frac}", str(frac))},

@@ -88,6 +88,7 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         L.ora_set_threads.argtypes = [ctypes.c_int32]
+        L.ora_set_dgemv.argtypes = [ctypes.c_void_p]
         L.ora_get_threads.restype = ctypes.c_int32
         L.ora_itri.restype = ctypes.c_int64
         L.ora_itri.argtypes = [ctypes.c_int64, ctypes.c_int64]
@@ -150,6 +151,36 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib().ora_get_threads())
+
+
+def use_blas(on: bool = True) -> str:
+    """Timing only: route the DGEMV calls of the restatement (the J pair and
+    DGEMM_Tri's per-column DGEMVs) through scipy's OpenBLAS dgemv, as the
+    reference calls its linked BLAS; its threads follow threadpoolctl.  Off
+    (the default) keeps the plain loops every check uses.  Returns the BLAS
+    build in use, or "" when off."""
+    if not on:
+        lib().ora_set_dgemv(None)
+        return ""
+    from scipy.linalg import cython_blas
+
+    cap = cython_blas.__pyx_capi__["dgemv"]
+    get_name = ctypes.pythonapi.PyCapsule_GetName
+    get_name.restype = ctypes.c_char_p
+    get_name.argtypes = [ctypes.py_object]
+    get_ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = ctypes.c_void_p
+    get_ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    lib().ora_set_dgemv(get_ptr(cap, get_name(cap)))
+    try:
+        from threadpoolctl import threadpool_info
+
+        info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        if info:
+            return f"{info[0].get('internal_api', 'blas')} {info[0].get('version', '')} (scipy)".strip()
+    except ImportError:
+        pass
+    return "scipy BLAS"
 
 
 def _p(a: np.ndarray, ct=_dp):

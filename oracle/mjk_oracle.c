@@ -9,7 +9,9 @@
  * diagonal pairs, the lower-triangle output of DGEMM_Tri).  The BLAS calls of
  * the reference are restated as plain loops (ora_dgemv); OpenMP is used where
  * the reference would use a threaded BLAS, never changing the summation order
- * inside one dot product.
+ * inside one dot product.  For timing only, ora_set_dgemv routes the DGEMV
+ * calls of ora_dgemv and ora_dgemm_tri_tn through a caller-supplied Fortran
+ * BLAS dgemv (the reference links one); the checks use the plain loops.
  */
 #include "mjk_oracle.h"
 
@@ -35,6 +37,20 @@ static void lt_pair(int64_t idx, int64_t *a, int64_t *b) {
   while ((r + 1) * (r + 2) / 2 <= idx) ++r;
   *a = r;
   *b = idx - r * (r + 1) / 2;
+}
+
+/* The pair (a, b) of every reduced-set entry, decoded once per call: the
+ * reference looks it up in its precomputed iRS2F map
+ * (src/cholesky_util/cho_vtra.F90:141-142); pab[2r], pab[2r+1].  Caller frees. */
+static int32_t *pair_map(int64_t nrs, const int64_t *rs2lt) {
+  int32_t *pab = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(nrs > 0 ? nrs : 1));
+  for (int64_t r = 0; r < nrs; ++r) {
+    int64_t a, b;
+    lt_pair(rs2lt[r], &a, &b);
+    pab[2 * r] = (int32_t)a;
+    pab[2 * r + 1] = (int32_t)b;
+  }
+  return pab;
 }
 
 /* src/misc_util/fold.F90:14-72 (nSym = 1): square -> LT, off-diagonal x2.
@@ -79,11 +95,26 @@ void ora_swap_rs2full(int64_t nrs, const int64_t *rs2lt, int32_t nmat,
     for (int32_t d = 0; d < nmat; ++d) xlt[d * ldlt + rs2lt[r]] += xrs[r + d * nrs];
 }
 
+/* Fortran BLAS dgemv (32-bit integers), or NULL for the plain loops. */
+typedef void (*ora_dgemv_fn)(const char *trans, const int *m, const int *n, const double *alpha,
+                             const double *a, const int *lda, const double *x, const int *incx,
+                             const double *beta, double *y, const int *incy);
+static ora_dgemv_fn g_dgemv = NULL;
+
+void ora_set_dgemv(void *fn) { g_dgemv = (ora_dgemv_fn)fn; }
+
+static int fits_int(int64_t v) { return v >= 0 && v <= 2147483647; }
+
 /* BLAS DGEMV semantics (column-major A(m,n)), as called through
  * src/misc_util/blas_wrappers.F90:219-293.  beta == 0 overwrites y. */
 void ora_dgemv(char trans, int64_t m, int64_t n, double alpha,
                const double *a, int64_t lda, const double *x, double beta,
                double *y) {
+  if (g_dgemv && fits_int(m) && fits_int(n) && fits_int(lda) && m > 0 && n > 0) {
+    const int im = (int)m, in = (int)n, il = (int)lda, one = 1;
+    g_dgemv(&trans, &im, &in, &alpha, a, &il, x, &one, &beta, y, &one);
+    return;
+  }
   if (trans == 'T' || trans == 't') {
 #pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < n; ++j) {
@@ -117,6 +148,16 @@ void ora_dgemm_tri_tn(int64_t n, int64_t k, double alpha, const double *a,
   else if (beta != 1.0)
     for (int64_t i = 0; i < np; ++i) c_lt[i] *= beta;
   if (noab) return;
+  if (g_dgemv && fits_int(k) && fits_int(n) && fits_int(lda)) {
+    /* dgemm_tri.F90:133-137: column j of C_LT += alpha A(:,1:j)^T B(:,j) */
+    const int ik = (int)k, il = (int)lda, one = 1;
+    const double b1 = 1.0;
+    for (int64_t j = 0; j < n; ++j) {
+      const int ij = (int)(j + 1);
+      g_dgemv("T", &ik, &ij, &alpha, a, &il, b + j * ldb, &one, &b1, c_lt + j * (j + 1) / 2, &one);
+    }
+    return;
+  }
 #pragma omp parallel for schedule(dynamic, 4)
   for (int64_t j = 0; j < n; ++j) {
     double *cj = c_lt + j * (j + 1) / 2;
@@ -162,12 +203,12 @@ void ora_cho_vtra2(int32_t n, int64_t nrs, const int64_t *rs2lt,
                    int64_t jnum, const double *lrs, int32_t norb,
                    const double *mo, double *chot) {
   memset(chot, 0, sizeof(double) * (size_t)norb * jnum * n);
+  int32_t *pab = pair_map(nrs, rs2lt);
 #pragma omp parallel for schedule(static)
   for (int64_t jv = 0; jv < jnum; ++jv) {
     const double *l = lrs + jv * nrs;
     for (int64_t r = 0; r < nrs; ++r) {
-      int64_t a, b;
-      lt_pair(rs2lt[r], &a, &b);
+      const int64_t a = pab[2 * r], b = pab[2 * r + 1];
       double xfd = (a == b) ? 0.5 : 1.0;
       double s = xfd * l[r];
       double *xa = chot + (size_t)norb * (jv + jnum * a);
@@ -178,6 +219,7 @@ void ora_cho_vtra2(int32_t n, int64_t nrs, const int64_t *rs2lt,
       for (int32_t p = 0; p < norb; ++p) xb[p] += s * ca[p];
     }
   }
+  free(pab);
 }
 
 /* src/cholesky_util/cho_vtra.F90:130-163 (iSwap = 0, JSYM = 1):
@@ -186,13 +228,13 @@ void ora_cho_vtra0(int32_t n, int64_t nrs, const int64_t *rs2lt,
                    int64_t jnum, const double *lrs, int32_t norb,
                    const double *mo, double *chot) {
   memset(chot, 0, sizeof(double) * (size_t)norb * jnum * n);
+  int32_t *pab = pair_map(nrs, rs2lt);
 #pragma omp parallel for schedule(static)
   for (int64_t jv = 0; jv < jnum; ++jv) {
     const double *l = lrs + jv * nrs;
     double *xj = chot + (size_t)norb * n * jv;
     for (int64_t r = 0; r < nrs; ++r) {
-      int64_t a, b;
-      lt_pair(rs2lt[r], &a, &b);
+      const int64_t a = pab[2 * r], b = pab[2 * r + 1];
       double xfd = (a == b) ? 0.5 : 1.0;
       double s = xfd * l[r];
       double *xa = xj + (size_t)norb * a;
@@ -203,6 +245,7 @@ void ora_cho_vtra0(int32_t n, int64_t nrs, const int64_t *rs2lt,
       for (int32_t p = 0; p < norb; ++p) xb[p] += s * ca[p];
     }
   }
+  free(pab);
 }
 
 /* Copy vectors jv0..jv0+jnum-1 (all of one reduced set) into Lrs(nrs,jnum):

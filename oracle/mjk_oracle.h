@@ -41,6 +41,9 @@ typedef struct {
 } ora_chol;
 
 void ora_set_threads(int32_t n);
+/* Timing only: a Fortran BLAS dgemv (32-bit integers) for ora_dgemv and
+ * ora_dgemm_tri_tn, or NULL for the plain loops (the default). */
+void ora_set_dgemv(void *fn);
 int32_t ora_get_threads(void);
 
 int64_t ora_itri(int64_t i, int64_t j); /* 1-based, either order */

@@ -145,6 +145,26 @@ def test_cho_fscf_batching_invariance():
     assert rel(a, b) < 1e-13
 
 
+def test_cho_fscf_blas_route_matches_loops():
+    """The timing-only BLAS route (scipy's OpenBLAS dgemv for the J pair and
+    DGEMM_Tri's columns) gives the plain loops' FLT to rounding, and the
+    switch back restores the loops bit for bit."""
+    n, M = 37, 90
+    L = chol_vectors_np(n, M, seed=24)
+    C = [orbitals_np(n, 6, seed=25)]
+    dlt = fold(2.0 * C[0] @ C[0].T)
+    flt = np.random.default_rng(26).standard_normal((1, npair(n)))
+    st = oracle.CholStore.full(L)
+    loops = oracle.cho_fscf(st, flt, [6], C, dlt, 0.5, batch=16)
+    try:
+        assert oracle.use_blas(True)
+        blas = oracle.cho_fscf(st, flt, [6], C, dlt, 0.5, batch=16)
+    finally:
+        oracle.use_blas(False)
+    assert rel(blas, loops) < 1e-13
+    assert np.array_equal(oracle.cho_fscf(st, flt, [6], C, dlt, 0.5, batch=16), loops)
+
+
 def test_cho_fscf_multiple_reduced_sets_equal_zero_filled_vectors():
     rng = np.random.default_rng(30)
     n, M1, M2 = 12, 9, 6
