@@ -152,7 +152,7 @@ int mjk_upload_vectors_file(mjk_context *ctx, const char *path, int32_t jred, in
 int mjk_upload_vectors(mjk_context *ctx, int32_t jred, int64_t jvec, int64_t jnum,
                        const double *lrs, int32_t on_device);
 
-/* ---- the reference's Cholesky files (Cho_X_Init, nSym = 1) --------------
+/* ---- the reference's Cholesky files (Cho_X_Init, nSym = 1..8) -----------
  * Host-side reading of the files the decomposition writes, so a caller passes
  * file names instead of maps and addresses.  mjk_choinfo_read parses the
  * restart file CHORST (src/cholesky_util/cho_wrrstc.F90:30-92, read by
@@ -161,30 +161,43 @@ int mjk_upload_vectors(mjk_context *ctx, int32_t jred, int64_t jvec, int64_t jnu
  * :46-78: nnBstRSh and IndRed per reduced set, IndRSh and iSP2F of set 1) and
  * rebuilds the LT index of every reduced-set element the way CHO_RSTOF does
  * (cho_rstof.F90:25-89), from isoshl = the runfile array ISOSHL (1-based shell
- * of each AO, nBas values).  Files are word addressed with 8-byte integers.
+ * of each SO, nBasT values, SOs numbered irrep by irrep).  With nSym > 1 the
+ * index is into the C1 triangle of all nBasT SOs (what the *_sym drop-ins
+ * use), each element is checked to lie in the irrep block it is filed under,
+ * and every (irrep iSym, reduced set JRED) is a map of its own, numbered
+ * (iSym-1)*MaxRed + JRED-1.  Files are word addressed with 8-byte integers.
  * No device work; safe without a GPU. */
 typedef struct mjk_choinfo mjk_choinfo;
 
 /* Reads CHORST and CHRED.  *info is always set (free it with
  * mjk_choinfo_free); on a nonzero rc mjk_choinfo_error tells why (4: a file
- * cannot be read, 1: inconsistent contents or nSym != 1). */
+ * cannot be read, 1: inconsistent contents or nSym outside 1..8). */
 int mjk_choinfo_read(mjk_choinfo **info, const char *chorst, const char *chred,
                      const int32_t *isoshl);
 const char *mjk_choinfo_error(const mjk_choinfo *info);
 int mjk_choinfo_free(mjk_choinfo *info);
 
-/* nBas, NumCho(1), MaxRed, nShell, Cho_AdrVec (1 word / 2 block addressed
- * vector file) and ThrCom of the files (any pointer may be NULL). */
+/* nBasT, NumCho summed over the irreps, MaxRed, nShell, Cho_AdrVec (1 word /
+ * 2 block addressed vector file) and ThrCom of the files (any pointer may be
+ * NULL). */
 int mjk_choinfo_dims(const mjk_choinfo *info, int32_t *nbas, int64_t *numcho, int32_t *maxred,
                      int32_t *nshell, int32_t *adrvec, double *thrcom);
 
-/* Reduced set jred (0-based = JRED-1): *nrs elements and, if rs_to_lt is not
- * NULL, the 0-based LT index of each (the map mjk_set_reduced_set takes). */
+/* nSym and, per irrep (arrays of 8, entries past nSym zero), nBas(iSym) and
+ * NumCho(iSym) (any pointer may be NULL).  No reference counterpart: the
+ * reference keeps them in module variables. */
+int mjk_choinfo_symmetry(const mjk_choinfo *info, int32_t *nsym, int32_t *nbas_sym,
+                         int64_t *numcho_sym);
+
+/* Map jred = (iSym-1)*MaxRed + JRED-1 (jred = JRED-1 for nSym = 1): *nrs
+ * elements and, if rs_to_lt is not NULL, the 0-based LT index of each (the map
+ * mjk_set_reduced_set takes). */
 int mjk_choinfo_reduced_set(const mjk_choinfo *info, int32_t jred, int64_t *nrs,
                             int64_t *rs_to_lt);
 
-/* Column col (1..5) of InfVec(1:NumCho, col, 1): 2 = reduced set of each
- * vector (1-based), 3 = its disk address in CHVEC1. */
+/* Column col (1..5) of InfVec(1:NumCho(iSym), col, iSym), irrep 1's vectors
+ * first: 2 = the map of each vector, 1-based ((iSym-1)*MaxRed + JRED; JRED for
+ * nSym = 1), 3 = its disk address in the CHVEC file of its irrep. */
 int mjk_choinfo_infvec(const mjk_choinfo *info, int32_t col, int64_t *values);
 
 /* Loads a context (nbas = nBas, capacity >= NumCho) from the files: declares
@@ -196,6 +209,14 @@ int mjk_choinfo_infvec(const mjk_choinfo *info, int32_t col, int64_t *values);
  * cho_iodiag.F90:28-33) into mjk_set_lk_diag with MaxRed of the files. */
 int mjk_load_choinfo(mjk_context *ctx, const mjk_choinfo *info, const char *chvec,
                      const char *chodiag, int64_t split_bytes);
+
+/* mjk_load_choinfo for any nSym: chvec[iSym-1] is the vector file of irrep
+ * iSym (CHVEC<iSym>, cho_openvr.F90), nfiles = nSym; the context has nbas =
+ * nBasT and holds the vectors of every irrep, irrep 1's first, each scattered
+ * into the C1 triangle by its map (the layout the *_sym drop-ins take).  No
+ * reference counterpart. */
+int mjk_load_choinfo_sym(mjk_context *ctx, const mjk_choinfo *info, const char *const *chvec,
+                         int32_t nfiles, const char *chodiag, int64_t split_bytes);
 
 /* ---- reference-shaped drop-ins (host arrays, synchronous) -------------- */
 

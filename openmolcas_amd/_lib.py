@@ -87,6 +87,8 @@ SIGNATURES = {
     "mjk_choinfo_reduced_set": (_c.c_int, [_vp, _i32, _vp, _vp]),
     "mjk_choinfo_infvec": (_c.c_int, [_vp, _i32, _vp]),
     "mjk_load_choinfo": (_c.c_int, [_vp, _vp, _c.c_char_p, _c.c_char_p, _i64]),
+    "mjk_load_choinfo_sym": (_c.c_int, [_vp, _vp, _vp, _i32, _c.c_char_p, _i64]),
+    "mjk_choinfo_symmetry": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "mjk_cd_incore": (_c.c_int, [_vp, _i32, _vp, _vp, _i32, _vp, _c.c_double, _vp]),
     "mjk_fock_update": (_c.c_int, [_vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mjk_energy": (_c.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -98,7 +100,7 @@ SIGNATURES = {
                                       _i64]),
     "mjk_grad_vectors_z_mixed": (_c.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _c.c_double, _c.c_double,
                                             _vp, _i64, _i64]),
-    "mjk_sync":(_c.c_int, [_vp]),
+    "mjk_sync": (_c.c_int, [_vp]),
     "mjk_set_profiling": (_c.c_int, [_vp, _i32]),
     "mjk_phase_times": (_c.c_int, [_vp, _vp, _vp]),
     "mjk_measure_peak": (_c.c_int, [_vp, _i32, _vp]),

@@ -729,9 +729,11 @@ class JKContext:
 
 class CholeskyFiles:
     """The reference's Cholesky index files read on the host (Cho_X_Init,
-    nSym = 1): CHORST (restart info, InfVec) and CHRED (reduced sets), the
+    nSym = 1..8): CHORST (restart info, InfVec) and CHRED (reduced sets), the
     element maps rebuilt from isoshl (the runfile's ISOSHL, 1-based shell of
-    each AO).  No device is needed to read; ``load`` feeds a JKContext."""
+    each SO, SOs irrep by irrep) into the C1 triangle of all nBasT SOs.  No
+    device is needed to read; ``load`` feeds a JKContext (one CHVEC path, or
+    one per irrep)."""
 
     def __init__(self, chorst: str, chred: str, isoshl):
         self.lib = _lib.load()
@@ -750,14 +752,24 @@ class CholeskyFiles:
                                   ctypes.byref(av), ctypes.byref(thr))
         self.nbas, self.numcho, self.maxred = nb.value, nc.value, mr.value
         self.nshell, self.adrvec, self.thrcom = nsh.value, av.value, thr.value
+        ns = ctypes.c_int32()
+        nbs = np.zeros(8, dtype=np.int32)
+        ncs = np.zeros(8, dtype=np.int64)
+        self.lib.mjk_choinfo_symmetry(h, ctypes.byref(ns), _ptr(nbs), _ptr(ncs))
+        self.nsym = ns.value
+        self.nbas_sym = [int(v) for v in nbs[:self.nsym]]
+        self.numcho_sym = [int(v) for v in ncs[:self.nsym]]
 
-    def reduced_set(self, jred: int) -> np.ndarray:
-        """0-based LT index of each element of reduced set jred (0-based)."""
+    def reduced_set(self, jred: int, isym: int = 1) -> np.ndarray:
+        """0-based LT index of each element of reduced set jred (0-based) in
+        irrep isym (1-based)."""
         nrs = ctypes.c_int64()
-        if self.lib.mjk_choinfo_reduced_set(self._h, int(jred), ctypes.byref(nrs), None) != 0:
-            raise MjkError(1, f"no reduced set {jred}")
+        k = (int(isym) - 1) * self.maxred + int(jred)
+        if not 1 <= int(isym) <= self.nsym or \
+                self.lib.mjk_choinfo_reduced_set(self._h, k, ctypes.byref(nrs), None) != 0:
+            raise MjkError(1, f"no reduced set {jred} in irrep {isym}")
         m = np.zeros(nrs.value, dtype=np.int64)
-        self.lib.mjk_choinfo_reduced_set(self._h, int(jred), ctypes.byref(nrs), _ptr(m))
+        self.lib.mjk_choinfo_reduced_set(self._h, k, ctypes.byref(nrs), _ptr(m))
         return m
 
     def infvec(self, col: int) -> np.ndarray:
@@ -766,9 +778,15 @@ class CholeskyFiles:
             raise MjkError(1, f"bad InfVec column {col}")
         return v
 
-    def load(self, jk: "JKContext", chvec: str, chodiag: str = None, split_bytes: int = 0) -> None:
-        jk._check(self.lib.mjk_load_choinfo(jk._h, self._h, os.fsencode(chvec),
-                                            os.fsencode(chodiag) if chodiag else None, int(split_bytes)))
+    def load(self, jk: "JKContext", chvec, chodiag: str = None, split_bytes: int = 0) -> None:
+        """chvec: the CHVEC file (nSym = 1) or a list of one per irrep."""
+        diag = os.fsencode(chodiag) if chodiag else None
+        if isinstance(chvec, (list, tuple)):
+            names = [os.fsencode(p) for p in chvec]
+            arr = (ctypes.c_char_p * len(names))(*names)
+            jk._check(self.lib.mjk_load_choinfo_sym(jk._h, self._h, arr, len(names), diag, int(split_bytes)))
+            return
+        jk._check(self.lib.mjk_load_choinfo(jk._h, self._h, os.fsencode(chvec), diag, int(split_bytes)))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
