@@ -44,6 +44,7 @@ static void lt_pair(int64_t idx, int64_t *a, int64_t *b) {
  * (src/cholesky_util/cho_vtra.F90:141-142); pab[2r], pab[2r+1].  Caller frees. */
 static int32_t *pair_map(int64_t nrs, const int64_t *rs2lt) {
   int32_t *pab = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(nrs > 0 ? nrs : 1));
+  if (!pab) abort(); /* the checker cannot continue without its map */
   for (int64_t r = 0; r < nrs; ++r) {
     int64_t a, b;
     lt_pair(rs2lt[r], &a, &b);
