@@ -50,12 +50,14 @@ FORTRAN_BIN = os.path.join(FORTRAN, "bin", "fscf_driver")
 FORTRAN_SYM_BIN = os.path.join(FORTRAN, "bin", "fscf_sym_driver")
 FORTRAN_UHF_CAS_BIN = os.path.join(FORTRAN, "bin", "uhf_cas_driver")
 FORTRAN_CAS_SYM_BIN = os.path.join(FORTRAN, "bin", "fmcscf_sym_driver")
+FORTRAN_CHOINFO_BIN = os.path.join(FORTRAN, "bin", "choinfo_driver")
 
 
 def build_fortran(verbose: bool = False):
     """ISO_C_BINDING interface module + the Fortran drivers of the drop-ins:
     CHO_FSCF C1 and with symmetry, the two-density UHF / CASSCF calls on
-    separately allocated arrays, and CHO_FMCSCF / CHO_LK_CASSCF with symmetry
+    separately allocated arrays, CHO_FMCSCF / CHO_LK_CASSCF with symmetry, and
+    a caller that takes its vectors from the reference's Cholesky files
     (amdflang).  Returns the C1 driver path, or None without a compiler."""
     if not os.path.exists(FLANG):
         if verbose:
@@ -75,6 +77,8 @@ def build_fortran(verbose: bool = False):
          "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_UHF_CAS_BIN],
         [FLANG, "-O2", "-J", obj, os.path.join(FORTRAN, "fmcscf_sym_driver.F90"), iface_o, "-L" + LIBDIR,
          "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_CAS_SYM_BIN],
+        [FLANG, "-O2", "-J", obj, os.path.join(FORTRAN, "choinfo_driver.F90"), iface_o, "-L" + LIBDIR,
+         "-lmolcas_jk", "-Wl,-rpath,$ORIGIN/../../lib", "-o", FORTRAN_CHOINFO_BIN],
     ]
     for cmd in cmds:
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -25,7 +25,9 @@ module molcas_jk_iface
             mjk_cho_fscf_sym, cho_fscf_sym_gpu, mjk_cho_lk_scf_sym, cho_lk_scf_sym_gpu, &
             cho_fock_dft_red_gpu, cho_fmcscf_gpu, mjk_cho_fmcscf_sym, cho_fmcscf_sym_gpu, mjk_cho_lk_casscf, &
             cho_lk_casscf_gpu, mjk_cho_lk_casscf_sym, cho_lk_casscf_sym_gpu, jk_gpu_init, jk_gpu_final, &
-            mjk_set_ladder
+            mjk_set_ladder, mjk_last_error, mjk_set_lk_screening, mjk_set_shells, mjk_set_lk_diag, &
+            mjk_choinfo_read, mjk_choinfo_error, mjk_choinfo_free, mjk_choinfo_dims, mjk_choinfo_symmetry, &
+            mjk_load_choinfo, mjk_load_choinfo_sym, jk_error_text
 
   integer(c_int32_t), parameter, public :: MJK_PREC_FP64 = 0, MJK_PREC_FP32 = 1, MJK_PREC_BF16X3 = 2, &
                                            MJK_PREC_LADDER = 3
@@ -69,6 +71,78 @@ module molcas_jk_iface
       integer(c_int64_t), value :: jvec, jnum, split_bytes
       integer(c_int64_t), intent(in) :: iadr(*)
     end function mjk_upload_vectors_file
+    ! text of the last error (a C string: jk_error_text turns it into a Fortran one)
+    type(c_ptr) function mjk_last_error(ctx) bind(C, name='mjk_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function mjk_last_error
+    ! LK state: screening on/off with ThrCom, the shells nBasSh(1:nShell) and the
+    ! integral diagonal CHO_IODIAG reads (LT order)
+    integer(c_int) function mjk_set_lk_screening(ctx, enable, thrcom) bind(C, name='mjk_set_lk_screening')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+      real(c_double), value :: thrcom
+    end function mjk_set_lk_screening
+    integer(c_int) function mjk_set_shells(ctx, nshell, nbassh) bind(C, name='mjk_set_shells')
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nshell
+      integer(c_int32_t), intent(in) :: nbassh(*)
+    end function mjk_set_shells
+    integer(c_int) function mjk_set_lk_diag(ctx, diag, maxred, estimate, update) bind(C, name='mjk_set_lk_diag')
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: diag(*)
+      integer(c_int32_t), value :: maxred, estimate, update
+    end function mjk_set_lk_diag
+    ! the reference's Cholesky files (CHORST, CHRED, CHVEC<iSym>, CHODIAG); names
+    ! end in c_null_char, isoshl = the runfile array ISOSHL (nBasT values)
+    integer(c_int) function mjk_choinfo_read(info, chorst, chred, isoshl) bind(C, name='mjk_choinfo_read')
+      import :: c_ptr, c_int, c_int32_t, c_char
+      type(c_ptr), intent(out) :: info
+      character(kind=c_char), intent(in) :: chorst(*), chred(*)
+      integer(c_int32_t), intent(in) :: isoshl(*)
+    end function mjk_choinfo_read
+    type(c_ptr) function mjk_choinfo_error(info) bind(C, name='mjk_choinfo_error')
+      import :: c_ptr
+      type(c_ptr), value :: info
+    end function mjk_choinfo_error
+    integer(c_int) function mjk_choinfo_free(info) bind(C, name='mjk_choinfo_free')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: info
+    end function mjk_choinfo_free
+    integer(c_int) function mjk_choinfo_dims(info, nbas, numcho, maxred, nshell, adrvec, thrcom) &
+        bind(C, name='mjk_choinfo_dims')
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: info
+      integer(c_int32_t), intent(out) :: nbas, maxred, nshell, adrvec
+      integer(c_int64_t), intent(out) :: numcho
+      real(c_double), intent(out) :: thrcom
+    end function mjk_choinfo_dims
+    integer(c_int) function mjk_choinfo_symmetry(info, nsym, nbas_sym, numcho_sym) &
+        bind(C, name='mjk_choinfo_symmetry')
+      import :: c_ptr, c_int, c_int32_t, c_int64_t
+      type(c_ptr), value :: info
+      integer(c_int32_t), intent(out) :: nsym, nbas_sym(8)
+      integer(c_int64_t), intent(out) :: numcho_sym(8)
+    end function mjk_choinfo_symmetry
+    ! chodiag: c_loc of a c_null_char-terminated name, or c_null_ptr
+    integer(c_int) function mjk_load_choinfo(ctx, info, chvec, chodiag, split_bytes) bind(C, name='mjk_load_choinfo')
+      import :: c_ptr, c_int, c_int64_t, c_char
+      type(c_ptr), value :: ctx, info, chodiag
+      character(kind=c_char), intent(in) :: chvec(*)
+      integer(c_int64_t), value :: split_bytes
+    end function mjk_load_choinfo
+    ! chvec(iSym): c_loc of the name of CHVEC<iSym>, nfiles = nSym
+    integer(c_int) function mjk_load_choinfo_sym(ctx, info, chvec, nfiles, chodiag, split_bytes) &
+        bind(C, name='mjk_load_choinfo_sym')
+      import :: c_ptr, c_int, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx, info, chodiag
+      type(c_ptr), intent(in) :: chvec(*)
+      integer(c_int32_t), value :: nfiles
+      integer(c_int64_t), value :: split_bytes
+    end function mjk_load_choinfo_sym
     integer(c_int) function mjk_set_ladder(ctx, j_fp32, j_bf16) bind(C, name='mjk_set_ladder')
       import :: c_ptr, c_int, c_int64_t
       type(c_ptr), value :: ctx
@@ -183,6 +257,27 @@ contains
     rc = int(mjk_destroy(jk_ctx))
     jk_ctx = c_null_ptr
   end subroutine jk_gpu_final
+
+  ! A C string (mjk_last_error, mjk_choinfo_error) as a Fortran string.
+  function jk_error_text(p) result(msg)
+    use, intrinsic :: iso_c_binding, only: c_f_pointer, c_associated, c_null_char
+    type(c_ptr), intent(in) :: p
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: ch(:)
+    integer :: k
+    msg = ''
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, ch, [4096])
+    k = 0
+    do while (k < 4096)
+      if (ch(k + 1) == c_null_char) exit
+      k = k + 1
+    end do
+    allocate(character(len=k) :: msg)
+    do k = 1, len(msg)
+      msg(k:k) = ch(k)
+    end do
+  end function jk_error_text
 
   integer(c_int32_t) function prec_of(prec)
     integer, intent(in), optional :: prec

@@ -1,8 +1,10 @@
 """The drop-ins called from Fortran through the ISO_C_BINDING module
 (openmolcas_amd/fortran): CHO_FSCF C1 and with two irreps, and the
 two-density calls (UHF, CASSCF FI/FA) on separately allocated arrays and
-CHO_FMCSCF with two irreps,
-checked against the oracle (FP64, relative 1e-11)."""
+CHO_FMCSCF with two irreps, checked against the oracle (FP64, relative
+1e-11); and a Fortran caller that reads the reference's Cholesky files
+(CHORST, CHRED, CHVEC1..4 of four irreps) and runs CHO_FSCF, bit-identical
+to the Python caller on the same files."""
 import os
 import subprocess
 
@@ -10,7 +12,8 @@ import numpy as np
 import pytest
 
 import oracle
-from openmolcas_amd.build import FORTRAN_BIN, FORTRAN_CAS_SYM_BIN, FORTRAN_SYM_BIN, FORTRAN_UHF_CAS_BIN
+from openmolcas_amd.build import (FORTRAN_BIN, FORTRAN_CAS_SYM_BIN, FORTRAN_CHOINFO_BIN, FORTRAN_SYM_BIN,
+                                  FORTRAN_UHF_CAS_BIN)
 from openmolcas_amd.synthetic import chol_vectors_np, fold, npair, orbitals_np
 
 pytestmark = pytest.mark.gpu
@@ -163,3 +166,38 @@ def test_fortran_cho_fmcscf_symmetry_dropin(gpu, tmp_path, do_active):
         else:
             assert np.array_equal(fa, q["fsym0"][1])
         assert np.max(np.abs(w - q["want_w"])) <= 1e-11 * np.max(np.abs(q["want_w"]))
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_CHOINFO_BIN), reason="Fortran driver not built (no amdflang)")
+def test_fortran_caller_reads_cholesky_files(gpu, tmp_path):
+    import test_chofiles as tc
+    from _chofiles import write_cholesky_files_sym
+    from openmolcas_amd.fock import CholeskyFiles, JKContext
+
+    sets, vec_sym, vec_set, vec_lt, diag = tc.sym_problem(seed=41, nvec=(8, 5, 4, 3))
+    write_cholesky_files_sym(tmp_path, tc.NBAS_SYM, tc.ISOSHL_SYM, sets, vec_sym, vec_set, vec_lt, diag)
+    nbas, nocc = tc.NBAS_SYM, [2, 1, 2, 1]
+    rng = np.random.default_rng(42)
+    porb = [rng.standard_normal((b, o)) for b, o in zip(nbas, nocc)]   # A2(nOcc(s), nBas(s)) memory
+    ntri = sum(b * (b + 1) // 2 for b in nbas)
+    dlt = rng.standard_normal(ntri)
+    flt0 = rng.standard_normal(ntri)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([len(nbas)] + nbas + list(tc.ISOSHL_SYM) + nocc + [0.5], dtype=np.float64).tofile(f)
+        for a in [np.concatenate([b.reshape(-1) for b in porb]), dlt, flt0]:
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+    r = subprocess.run([FORTRAN_CHOINFO_BIN, str(tmp_path), str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    got = np.fromfile(fout, dtype=np.float64)
+    cf = CholeskyFiles(str(tmp_path / "CHORST"), str(tmp_path / "CHRED"), tc.ISOSHL_SYM)
+    jk = JKContext(cf.nbas, cf.numcho, device=gpu)
+    cf.load(jk, [str(tmp_path / f"CHVEC{s + 1}") for s in range(cf.nsym)])
+    want = flt0.copy()[None]
+    jk.cho_fscf_sym(nbas, 1, want, [[0] * 4], [nocc], [porb], dlt, 0.5)
+    assert np.array_equal(got, want[0])
+    assert np.max(np.abs(got - flt0)) > 0
+    cf.close()
+    jk.close()
