@@ -61,6 +61,9 @@ def parse():
                    help="64-system batch only: set up, autotune, one graph replay, then (after a 0.5 s "
                         "pause that marks it in a kernel trace) one FP64 forward + backward step with the "
                         "systems one after another; prints that step's per-phase kernel times")
+    p.add_argument("--profile-precision", choices=["fp64", "ladder"], default="fp64",
+                   help="with --kernel-profile: the precision of the profiled step (ladder: split points "
+                        "tuned per system for |dE| <= 1e-6 Eh first)")
     return p.parse_args()
 
 
@@ -587,30 +590,38 @@ def batch_workload(args, dev):
     return out, shapes
 
 
-def kernel_profile(dev):
+def kernel_profile(dev, precision="fp64"):
     """The batch workload's kernels for a profiler: one serial eager step,
-    preceded by a 0.5 s pause so that a kernel trace can cut it out."""
+    preceded by a 0.5 s pause so that a kernel trace can cut it out.  With
+    precision "ladder" the split points are tuned per system first."""
     import torch
 
     from openmolcas_amd.batch import MoleculeBatch, uneven_shapes
 
     batch = MoleculeBatch(uneven_shapes(64, seed=5), device=dev, seed=3)
     batch.autotune("fp64")
-    batch.capture("fp64", True)
+    extra = {}
+    if precision == "ladder":
+        plans = batch.tune_ladder(tol=1e-6)
+        extra["fp64_vectors"] = int(sum(p.j_fp32 for p in plans))
+        extra["fp32_vectors"] = int(sum(p.j_bf16 - p.j_fp32 for p in plans))
+        extra["bf16x3_vectors"] = int(sum(M - p.j_bf16 for p, (n, M, o) in zip(plans, batch.shapes)))
+    batch.capture(precision, True)
     batch.replay()
     torch.cuda.synchronize()
     time.sleep(0.5)
     batch.set_profiling(True)
     t0 = time.perf_counter()
-    batch.step("fp64", serial=True)
+    batch.step(precision, serial=True)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     phases = batch.phase_times()
     batch.close()
-    return {"workload": "64-system batch, one FP64 forward + backward eager step, systems one after another",
+    label = "FP64" if precision == "fp64" else "precision-ladder"
+    return {"workload": f"64-system batch, one {label} forward + backward eager step, systems one after another",
             "step_ms": round(1e3 * t, 3),
             "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
-            "phase_launches": {k: v[1] for k, v in phases.items()}}
+            "phase_launches": {k: v[1] for k, v in phases.items()}, **extra}
 
 
 def single_build(args, dev, steps):
@@ -779,7 +790,7 @@ def main():
         "data": "synthetic (seeded N(0,1) symmetric vectors, norms 1..1e-6; QR orbitals; h = sym N(0,1)/nbas)",
     }
     if args.kernel_profile:
-        print(json.dumps(kernel_profile(dev)), flush=True)
+        print(json.dumps(kernel_profile(dev, args.profile_precision)), flush=True)
         return
     if args.workload == "batch64":
         head, shapes = batch_workload(args, dev)

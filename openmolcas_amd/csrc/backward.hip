@@ -305,9 +305,11 @@ struct GradRows {
   double gv, beta;
   double *g;
   int64_t ldg;
+  double *sink;      // kSinkSlots * kThreads doubles no one reads (masked epilogue stores)
   int n, T;          // basis functions, row tiles of 64
   int64_t items;     // T * vectors
 };
+constexpr int kSinkSlots = 256;  // sink rows, spread over workgroups (blockIdx mod 256)
 
 typedef double f4d __attribute__((ext_vector_type(4)));
 typedef float f4f __attribute__((ext_vector_type(4)));
@@ -432,15 +434,28 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
     for (int r = 0; r < 4; ++r) sW[16 * w + R::row(lane, r)][16 * t + (lane & 15)] = acc1[t][r];
   __syncthreads();
 
-  // ---- phase 2: the blocks W_I C_j^T, j = 0..I, as one stream of chunks
+  // ---- phase 2: the blocks W_I C_j^T, j = 0..I, as one stream of chunks.
+  // Every column tile is straight-line code (its OC16 chunks unrolled, the
+  // fragment stage of each chunk a compile-time constant) issuing the same
+  // global operations wherever it sits: the fragments of the chunk after
+  // next are loaded even past the last chunk (clamped to it, never used),
+  // and the epilogue stores every element, those above the diagonal or past
+  // n into a sink no one reads.  The vector memory counter of this
+  // architecture counts loads and stores in one queue, so with a fixed count
+  // per tile the compiler's waits are exact: the epilogue waits for the
+  // tile's density values only, a chunk for its own fragments, and neither
+  // for the stores of the tile before (conditional stores made it drain the
+  // whole queue once per tile).
   const int wm = w >> 1, wn = w & 1;
   const int Q = (I + 1) * OC16;
   const double vj = g.gv * g.v[J];
   const double *__restrict__ P = g.p;
   double *__restrict__ G = g.g + J * g.ldg;
+  double *__restrict__ sink = g.sink + (blockIdx.x % kSinkSlots) * kThreads + tid;
   typedef typename R::T4 T4;
   T4 bq[2][2];  // [stage][jj]: C_j fragments of the chunks in flight
-  auto loadB = [&](int q, T4 (&dst)[2]) {
+  auto loadB = [&](int q, T4 (&dst)[2]) __attribute__((always_inline)) {
+    q = q < Q ? q : Q - 1;
     const int j = q / OC16, kc = q - j * OC16;
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj) {
@@ -453,11 +468,11 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = Acc{0, 0, 0, 0};
-  // P values of the tile in progress, loaded at its first chunk so the
-  // epilogue only stores
+  // P values of the tile in progress, loaded at its start so the epilogue
+  // only stores
   double pv[2][2][4];
   uint32_t pi[2][2][4];
-  auto load_p = [&](int j) {
+  auto load_p = [&](int j) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -471,7 +486,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
           pv[i][jj][r] = P[ok ? pi[i][jj][r] : 0u];
         }
   };
-  auto epilogue = [&](int j) {
+  auto epilogue = [&](int j) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -479,45 +494,58 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const uint32_t p = pi[i][jj][r];
-          if (p == ~0u) continue;
           const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + R::row(lane, r));
           const uint32_t gj = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
-          G[p] = (gi == gj ? 1.0 : 2.0) * (vj * pv[i][jj][r] + g.beta * double(acc2[i][jj][r]));
+          double *__restrict__ dst = p != ~0u ? G + p : sink;
+          *dst = (gi == gj ? 1.0 : 2.0) * (vj * pv[i][jj][r] + g.beta * double(acc2[i][jj][r]));
         }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int jj = 0; jj < 2; ++jj) acc2[i][jj] = Acc{0, 0, 0, 0};
   };
-  // One chunk of the stream; the fragment stage S is a compile-time constant
-  // (the loop below is unrolled by two), so the stages never move.
-  auto step = [&](int q, auto stage) {
-    constexpr int S = decltype(stage)::value;
-    const T4 b0 = bq[S][0], b1 = bq[S][1];
-    if (q + 2 < Q) loadB(q + 2, bq[S]);
-    const int j = q / OC16, kc = q - j * OC16;
-    if (kc == 0) load_p(j);
+  // Column tile j; PAR = parity of its first chunk in the stream (chunk q
+  // uses fragment stage q & 1 and refills it with chunk q + 2).
+  // The scheduler keeps each chunk to itself (sched_barrier): hoisting the
+  // loads and LDS reads of later chunks raised the register count to the
+  // point of one wave per SIMD.
+  auto tile = [&](int j, auto parity) __attribute__((always_inline)) {
+    constexpr int PAR = decltype(parity)::value;
+    load_p(j);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      T af[2];
+    for (int kc = 0; kc < OC16; ++kc) {
+      const int S = (PAR + kc) & 1;
+      const T4 b0 = bq[S][0], b1 = bq[S][1];
+      loadB(j * OC16 + kc + 2, bq[S]);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = sW[wm * 32 + i * 16 + (lane & 15)][kc * kBK + 4 * gq + ks];
+      for (int ks = 0; ks < 4; ++ks) {
+        T af[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        acc2[i][0] = R::mma(af[i], b0[ks], acc2[i][0]);
-        acc2[i][1] = R::mma(af[i], b1[ks], acc2[i][1]);
+        for (int i = 0; i < 2; ++i) af[i] = sW[wm * 32 + i * 16 + (lane & 15)][kc * kBK + 4 * gq + ks];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          acc2[i][0] = R::mma(af[i], b0[ks], acc2[i][0]);
+          acc2[i][1] = R::mma(af[i], b1[ks], acc2[i][1]);
+        }
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (kc == OC16 - 1) epilogue(j);
+    epilogue(j);
+    __builtin_amdgcn_sched_barrier(0);
   };
+  // tile j starts at chunk j OC16: parity 0 for even j, OC16 & 1 for odd j.
+  // Tile 0 runs before the loop so that both ways into the loop end with an
+  // epilogue and the wait counts at the loop head stay exact.
   loadB(0, bq[0]);
-  if (Q > 1) loadB(1, bq[1]);
-  int q = 0;
-  for (; q + 1 < Q; q += 2) {
-    step(q, std::integral_constant<int, 0>{});
-    step(q + 1, std::integral_constant<int, 1>{});
+  loadB(1, bq[1]);
+  tile(0, std::integral_constant<int, 0>{});
+  int j = 1;
+  for (; j + 1 <= I; j += 2) {
+    tile(j, std::integral_constant<int, OC16 & 1>{});
+    tile(j + 1, std::integral_constant<int, 0>{});
   }
-  if (q < Q) step(q, std::integral_constant<int, 0>{});
+  if (j <= I) tile(j, std::integral_constant<int, OC16 & 1>{});
 }
 
 template <int OC16>
@@ -539,7 +567,10 @@ int grad_rows(Context *c, int nocc, int ldc, const double *z, int64_t sz, int ld
   const int oc16 = (nocc + 15) / 16;
   if (oc16 > 8 || int64_t(c->n_pad) * ldc >= (int64_t(1) << 28)) return MJK_ERR_ARG;
   if (jn < 1) return MJK_OK;
+  int rc;
+  if ((rc = ensure(c, c->w->sink, sizeof(double) * size_t(kSinkSlots) * kThreads))) return rc;
   GradRows g{};
+  g.sink = static_cast<double *>(c->w->sink.ptr);
   g.cb = static_cast<const double *>(c->w->cbank.ptr);
   g.cb32 = cb32;
   g.ldc = ldc;

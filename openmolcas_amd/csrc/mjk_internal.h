@@ -33,10 +33,10 @@ struct DevBuf {
 // Device workspaces.  Contexts created with mjk_create_shared use the same
 // Workspace and must queue their work on the same stream.
 struct Workspace {
-  DevBuf d, vpart, v, jpart, cbank, cb32, cb3, x, kpart, stage, j, k, red, misc, sets, xn2, cnorm, gb32;
+  DevBuf d, vpart, v, jpart, cbank, cb32, cb3, x, kpart, stage, j, k, red, misc, sets, xn2, cnorm, gb32, sink;
   ~Workspace() {
     for (DevBuf *b : {&d, &vpart, &v, &jpart, &cbank, &cb32, &cb3, &x, &kpart, &stage, &j, &k, &red,
-                      &misc, &sets, &xn2, &cnorm, &gb32})
+                      &misc, &sets, &xn2, &cnorm, &gb32, &sink})
       if (b->ptr) (void)hipFree(b->ptr);
   }
 };
