@@ -218,6 +218,12 @@ __global__ __launch_bounds__(kThreads) void k_ht(
     // d x + d (d + 1) / 2.  Diagonal and edge chunks take the general path.
     constexpr int DA = kThreads / BK, DB = kThreads / TA;
     static_assert(kThreads % BK == 0 && kThreads % TA == 0, "element walk");
+    // The chunk kinds differ only in the packed index of each element: the
+    // branches form the indices, and one load sequence after them fills rl
+    // (and rd).  Loads issued inside the branches left their registers to be
+    // merged by copies at the join, and a copy of a register whose load is in
+    // flight waits for the whole memory queue: the chunk after next was then
+    // waited for right after its loads were issued.
     auto fetch = [&](int b0) {
       const bool colmajor = b0 + BK <= a0;
       const bool inside = b0 + BK <= n && a0 + TA <= n;
@@ -225,51 +231,49 @@ __global__ __launch_bounds__(kThreads) void k_ht(
 #pragma unroll
       for (int i = 0; i < CV; ++i)
         rc[i] = *reinterpret_cast<const VT *>(cs + (tid + kThreads * i) * VW);
+      uint32_t idx[LN];
       if (inside && colmajor) {  // packed row a = a0 + tid/BK + DA i holds b0 + tid%BK
         uint32_t a = uint32_t(a0 + tid / BK);
-        uint32_t idx = tri0u(a) + uint32_t(b0 + tid % BK);
+        uint32_t x = tri0u(a) + uint32_t(b0 + tid % BK);
 #pragma unroll
         for (int i = 0; i < LN; ++i) {
-#pragma unroll
-          for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx];
-          idx += uint32_t(DA) * a + uint32_t(DA * (DA + 1) / 2);
+          idx[i] = x;
+          x += uint32_t(DA) * a + uint32_t(DA * (DA + 1) / 2);
           a += uint32_t(DA);
         }
         lok = LN == 32 ? ~0u : (1u << LN) - 1u;
-        return;
-      }
-      if (inside && b0 >= a0 + TA) {  // packed row b = b0 + tid/TA + DB i at column a0 + tid%TA
+        if constexpr (FV) vok = 0u;
+      } else if (inside && b0 >= a0 + TA) {  // packed row b = b0 + tid/TA + DB i at column a0 + tid%TA
         uint32_t b = uint32_t(b0 + tid / TA);
-        uint32_t idx = tri0u(b) + uint32_t(a0 + tid % TA);
+        uint32_t x = tri0u(b) + uint32_t(a0 + tid % TA);
 #pragma unroll
         for (int i = 0; i < LN; ++i) {
-#pragma unroll
-          for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx];
-          if constexpr (FV) rd[i] = fv.p[idx];
-          idx += uint32_t(DB) * b + uint32_t(DB * (DB + 1) / 2);
+          idx[i] = x;
+          x += uint32_t(DB) * b + uint32_t(DB * (DB + 1) / 2);
           b += uint32_t(DB);
         }
         lok = LN == 32 ? ~0u : (1u << LN) - 1u;
         if constexpr (FV) vok = lok;
-        return;
+      } else {
+#pragma unroll
+        for (int i = 0; i < LN; ++i) {
+          const int e = tid + kThreads * i;
+          const int bb = colmajor ? e % BK : e / TA, aa = colmajor ? e / BK : e % TA;
+          const int b = b0 + bb, a = a0 + aa;
+          const bool ok = (b < n) && (a < n);
+          const int hi = b >= a ? b : a, lo = b >= a ? a : b;
+          idx[i] = ok ? tri0u(uint32_t(hi)) + uint32_t(lo) : 0u;  // branch-free masked load
+          lok = ok ? (lok | (1u << i)) : (lok & ~(1u << i));
+          if constexpr (FV) vok = (!colmajor && ok && b >= a) ? (vok | (1u << i)) : (vok & ~(1u << i));
+        }
       }
 #pragma unroll
-      for (int i = 0; i < LN; ++i) {
-        const int e = tid + kThreads * i;
-        const int bb = colmajor ? e % BK : e / TA, aa = colmajor ? e / BK : e % TA;
-        const int b = b0 + bb, a = a0 + aa;
-        const bool ok = (b < n) && (a < n);
-        const int hi = b >= a ? b : a, lo = b >= a ? a : b;
-        const uint32_t idx = ok ? tri0u(uint32_t(hi)) + uint32_t(lo) : 0u;  // branch-free masked load
+      for (int i = 0; i < LN; ++i)
 #pragma unroll
-        for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx];
-        lok = ok ? (lok | (1u << i)) : (lok & ~(1u << i));
-        if constexpr (FV) {
-          if (!colmajor) {  // uniform per chunk: no select on the loaded value
-            rd[i] = fv.p[idx];
-            vok = (ok && b >= a) ? (vok | (1u << i)) : (vok & ~(1u << i));
-          }
-        }
+        for (int q = 0; q < NJ; ++q) rl[q][i] = Lq[q][idx[i]];
+      if constexpr (FV) {  // column chunks hold no b >= a element: one cached address, masked by vok
+#pragma unroll
+        for (int i = 0; i < LN; ++i) rd[i] = fv.p[colmajor ? 0u : idx[i]];
       }
     };
     auto stash = [&](int b0, int buf) {
