@@ -679,38 +679,46 @@ __global__ __launch_bounds__(kThreads) void k_zsym(const TX *__restrict__ X, int
 #pragma unroll
       for (int j = 0; j < OC16; ++j) acc[r][j] = v4d{0.0, 0.0, 0.0, 0.0};
     typedef TX TX4 __attribute__((ext_vector_type(4)));
-    TX4 xa[NR > 0 ? NR : 1], xn[NR > 0 ? NR : 1];
-    auto loadx = [&](int a0, TX4 (&dst)[NR > 0 ? NR : 1]) {
+    TX4 xs[2][NR > 0 ? NR : 1];  // [stage]: X fragments of chunk c in stage c & 1
+    auto loadx = [&](int a0, TX4 (&dst)[NR > 0 ? NR : 1]) __attribute__((always_inline)) {
 #pragma unroll
       for (int r = 0; r < NR; ++r)
         dst[r] = *reinterpret_cast<const TX4 *>(xj + int64_t(16 * P.rows[W][r] + (lane & 15)) * ldx + a0 +
                                                 4 * gq);
     };
-    fetch(0);
-    stash(0);
-    loadx(0, xa);
-    if (nch > 1) fetch(16);
-    __syncthreads();
-    for (int c = 0; c < nch; ++c) {
-      const int cur = c & 1;
-      if (c + 1 < nch) loadx(16 * (c + 1), xn);
+    // Every chunk issues the same loads (the ones past the last chunk clamped
+    // to it, never used) and the X stages are compile-time (the loop runs two
+    // chunks per trip), so no register copy of a load in flight and no
+    // conditional load leaves the compiler a wait for the whole memory queue.
+    auto chunk = [&](int c, auto stage) __attribute__((always_inline)) {
+      constexpr int S = decltype(stage)::value;  // = c & 1: X stage and LDS buffer
+      loadx(16 * (c + 1 < nch ? c + 1 : nch - 1), xs[S ^ 1]);
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
-        const double *__restrict__ brow = sC[cur] + (4 * gq + ks) * OC + (lane & 15);
+        const double *__restrict__ brow = sC[S] + (4 * gq + ks) * OC + (lane & 15);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-          const double a = double(xa[r][ks]);
+          const double a = double(xs[S][r][ks]);
 #pragma unroll
           for (int j = 0; j < OC16; ++j)
             if (j <= P.rows[W][r]) acc[r][j] = mfma64(a, brow[16 * j], acc[r][j]);
         }
       }
-      if (c + 1 < nch) stash(cur ^ 1);
-      if (c + 2 < nch) fetch(16 * (c + 2));
-#pragma unroll
-      for (int r = 0; r < NR; ++r) xa[r] = xn[r];
+      stash(S ^ 1);
+      fetch(16 * (c + 2 < nch ? c + 2 : nch - 1));
       __syncthreads();
+    };
+    fetch(0);
+    stash(0);
+    loadx(0, xs[0]);
+    fetch(16 * (nch > 1 ? 1 : 0));
+    __syncthreads();
+    int c = 0;
+    for (; c + 1 < nch; c += 2) {
+      chunk(c, std::integral_constant<int, 0>{});
+      chunk(c + 1, std::integral_constant<int, 1>{});
     }
+    if (c < nch) chunk(c, std::integral_constant<int, 0>{});
     double *__restrict__ zj = z + J * sz;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
