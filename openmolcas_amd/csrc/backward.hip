@@ -355,15 +355,21 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
   typedef typename R::T T;
   typedef typename R::Acc Acc;
   constexpr int OC = 16 * OC16;
-  constexpr int WS = OC + 1;                                   // row stride of W_I in LDS
+  constexpr bool F32 = sizeof(T) == 4;
+  constexpr int WS = OC + 1;                                   // row stride of the phase-1 Z chunks
+  // Row stride of W_I.  FP32: a lane's four A values of a chunk are one
+  // 16-byte read; OC + 4 keeps rows 16-byte aligned and puts the 8 lanes of
+  // a read pass (rows r..r+7, 32 banks of 4 bytes) on 8 distinct 16-byte
+  // bank groups.  FP64 reads them one 8-byte value at a time (odd stride).
+  constexpr int WW = F32 ? OC + 4 : OC + 1;
   constexpr int S1 = 2 * (kTM * (kBK + 1) + kBK * WS);         // phase-1 staging
-  constexpr int SW = kTM * WS;
-  __shared__ T smem[S1 > SW ? S1 : SW];
+  constexpr int SW = kTM * WW;
+  __shared__ __attribute__((aligned(16))) T smem[S1 > SW ? S1 : SW];
   typedef T RowA[kTM][kBK + 1];
   typedef T RowB[kBK][WS];
   RowA *sA = reinterpret_cast<RowA *>(smem);                   // [2]
   RowB *sB = reinterpret_cast<RowB *>(smem + 2 * kTM * (kBK + 1));  // [2]
-  T(*sW)[WS] = reinterpret_cast<T(*)[WS]>(smem);               // aliases the staging
+  T(*sW)[WW] = reinterpret_cast<T(*)[WW]>(smem);               // aliases the staging
 
   const int64_t nblk = gridDim.x;  // multiple of 8
   const int64_t l = (int64_t(blockIdx.x) & 7) * (nblk >> 3) + (int64_t(blockIdx.x) >> 3);
@@ -518,11 +524,18 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
       const int S = (PAR + kc) & 1;
       const T4 b0 = bq[S][0], b1 = bq[S][1];
       loadB(j * OC16 + kc + 2, bq[S]);
+      T4 a4[2];
+      if constexpr (F32) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          a4[i] = *reinterpret_cast<const T4 *>(&sW[wm * 32 + i * 16 + (lane & 15)][kc * kBK + 4 * gq]);
+      }
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         T af[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) af[i] = sW[wm * 32 + i * 16 + (lane & 15)][kc * kBK + 4 * gq + ks];
+        for (int i = 0; i < 2; ++i)
+          af[i] = F32 ? a4[i][ks] : sW[wm * 32 + i * 16 + (lane & 15)][kc * kBK + 4 * gq + ks];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
           acc2[i][0] = R::mma(af[i], b0[ks], acc2[i][0]);
