@@ -305,11 +305,16 @@ struct GradRows {
   double gv, beta;
   double *g;
   int64_t ldg;
-  double *sink;      // kSinkSlots * kThreads doubles no one reads (masked epilogue stores)
+  int32_t rowbytes;  // bytes of one packed triangle n (n + 1) / 2 (< 2^31)
   int n, T;          // basis functions, row tiles of 64
   int64_t items;     // T * vectors
 };
-constexpr int kSinkSlots = 256;  // sink rows, spread over workgroups (blockIdx mod 256)
+// Element index of a masked element in the packed triangle: byte offset
+// 2^31, past every row, so the buffer load returns 0 and the store is dropped
+// by the range check of the buffer resource (stride 0, rowbytes records).
+constexpr uint32_t kMasked = 0x10000000u;
+constexpr int kRsrcWord3 = 0x00020000;  // buffer resource word 3 of this architecture family
+typedef int v2u __attribute__((ext_vector_type(2)));  // operand type of the 64-bit buffer builtins
 
 typedef double f4d __attribute__((ext_vector_type(4)));
 typedef float f4f __attribute__((ext_vector_type(4)));
@@ -446,7 +451,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
   // global operations wherever it sits: the fragments of the chunk after
   // next are loaded even past the last chunk (clamped to it, never used),
   // and the epilogue stores every element, those above the diagonal or past
-  // n into a sink no one reads.  The vector memory counter of this
+  // n with an offset past the buffer's range, which the hardware drops.  The vector memory counter of this
   // architecture counts loads and stores in one queue, so with a fixed count
   // per tile the compiler's waits are exact: the epilogue waits for the
   // tile's density values only, a chunk for its own fragments, and neither
@@ -457,7 +462,9 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
   const double vj = g.gv * g.v[J];
   const double *__restrict__ P = g.p;
   double *__restrict__ G = g.g + J * g.ldg;
-  double *__restrict__ sink = g.sink + (blockIdx.x % kSinkSlots) * kThreads + tid;
+  const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(G, 0, g.rowbytes, kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t rp =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(P), 0, g.rowbytes, kRsrcWord3);
   typedef typename R::T4 T4;
   T4 bq[2][2];  // [stage][jj]: C_j fragments of the chunks in flight
   auto loadB = [&](int q, T4 (&dst)[2]) __attribute__((always_inline)) {
@@ -488,8 +495,8 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
           const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + R::row(lane, r));
           const uint32_t gj = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
           const bool ok = gi < uint32_t(g.n) && gi >= gj;
-          pi[i][jj][r] = ok ? gi * (gi + 1u) / 2u + gj : ~0u;
-          pv[i][jj][r] = P[ok ? pi[i][jj][r] : 0u];
+          pi[i][jj][r] = (ok ? gi * (gi + 1u) / 2u + gj : kMasked) * 8u;  // byte offset
+          pv[i][jj][r] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rp, int(pi[i][jj][r]), 0, 0));
         }
   };
   auto epilogue = [&](int j) __attribute__((always_inline)) {
@@ -499,11 +506,10 @@ __global__ __launch_bounds__(kThreads) void k_grad_rows(GradRows g) {
       for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const uint32_t p = pi[i][jj][r];
           const uint32_t gi = uint32_t(i0 + wm * 32 + i * 16 + R::row(lane, r));
           const uint32_t gj = uint32_t(j * kTM + wn * 32 + jj * 16 + (lane & 15));
-          double *__restrict__ dst = p != ~0u ? G + p : sink;
-          *dst = (gi == gj ? 1.0 : 2.0) * (vj * pv[i][jj][r] + g.beta * double(acc2[i][jj][r]));
+          const double val = (gi == gj ? 1.0 : 2.0) * (vj * pv[i][jj][r] + g.beta * double(acc2[i][jj][r]));
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, val), rg, int(pi[i][jj][r]), 0, 0);
         }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -578,12 +584,12 @@ int grad_rows(Context *c, int nocc, int ldc, const double *z, int64_t sz, int ld
               const double *P, double alpha, double beta, double *dl, int64_t ldd, int64_t jn,
               const float *cb32) {
   const int oc16 = (nocc + 15) / 16;
-  if (oc16 > 8 || int64_t(c->n_pad) * ldc >= (int64_t(1) << 28)) return MJK_ERR_ARG;
+  const int64_t rowbytes = int64_t(c->n) * (c->n + 1) / 2 * int64_t(sizeof(double));
+  if (oc16 > 8 || int64_t(c->n_pad) * ldc >= (int64_t(1) << 28) || rowbytes >= (int64_t(1) << 31))
+    return MJK_ERR_ARG;
   if (jn < 1) return MJK_OK;
-  int rc;
-  if ((rc = ensure(c, c->w->sink, sizeof(double) * size_t(kSinkSlots) * kThreads))) return rc;
   GradRows g{};
-  g.sink = static_cast<double *>(c->w->sink.ptr);
+  g.rowbytes = int32_t(rowbytes);
   g.cb = static_cast<const double *>(c->w->cbank.ptr);
   g.cb32 = cb32;
   g.ldc = ldc;

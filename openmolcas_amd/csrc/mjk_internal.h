@@ -33,10 +33,10 @@ struct DevBuf {
 // Device workspaces.  Contexts created with mjk_create_shared use the same
 // Workspace and must queue their work on the same stream.
 struct Workspace {
-  DevBuf d, vpart, v, jpart, cbank, cb32, cb3, x, kpart, stage, j, k, red, misc, sets, xn2, cnorm, gb32, sink;
+  DevBuf d, vpart, v, jpart, cbank, cb32, cb3, x, kpart, stage, j, k, red, misc, sets, xn2, cnorm, gb32;
   ~Workspace() {
     for (DevBuf *b : {&d, &vpart, &v, &jpart, &cbank, &cb32, &cb3, &x, &kpart, &stage, &j, &k, &red,
-                      &misc, &sets, &xn2, &cnorm, &gb32, &sink})
+                      &misc, &sets, &xn2, &cnorm, &gb32})
       if (b->ptr) (void)hipFree(b->ptr);
   }
 };
