@@ -572,8 +572,9 @@ int mjk_grad_vectors_z(mjk_context *ctx, int32_t nocc, const double *corb, const
 /* mjk_grad_vectors_z with the products C Z^J C^T of the vectors J >= j_fp32
  * formed in FP32 (FP64 V P term and output): the backward of the precision
  * ladder, whose vectors past its FP64 head carry FP32-sized errors already.
- * 0 <= j_fp32 <= nchol; j_fp32 = nchol is mjk_grad_vectors_z.  For nocc > 128
- * every vector is formed in FP64.  No reference counterpart. */
+ * 0 <= j_fp32 <= nchol; j_fp32 = nchol is mjk_grad_vectors_z.  For nocc > 128,
+ * or a packed dL row of 2^31 bytes or more (nbas >= 23170), every vector is
+ * formed in FP64.  No reference counterpart. */
 int mjk_grad_vectors_z_mixed(mjk_context *ctx, int32_t nocc, const double *corb, const double *p_lt,
                              const double *v, const double *z, double alpha, double beta, double *dl,
                              int64_t ldd, int64_t j_fp32);

@@ -577,19 +577,25 @@ void launch_grad_rows(Context *c, const GradRows &g) {
                        c->stream, g);
 }
 
-// dL of the jn vectors from Z in one fused launch (nocc <= 128); returns
-// MJK_ERR_ARG when nocc is larger (the caller then forms W = C Z first).
+// Whether the fused k_grad_rows takes this shape: nocc <= 128, 32-bit bank
+// offsets, and a packed dL row under 2^31 bytes (the range of its buffer
+// resources).  Other shapes form W = C Z first.
+bool grad_rows_fits(const Context *c, int nocc, int ldc) {
+  const int64_t rowbytes = int64_t(c->n) * (c->n + 1) / 2 * int64_t(sizeof(double));
+  return (nocc + 15) / 16 <= 8 && int64_t(c->n_pad) * ldc < (int64_t(1) << 28) && rowbytes < (int64_t(1) << 31);
+}
+
+// dL of the jn vectors from Z in one fused launch; returns MJK_ERR_ARG when
+// grad_rows_fits rejects the shape (the caller then forms W = C Z first).
 // cb32: the bank in FP32 for FP32 products, or null for FP64.
 int grad_rows(Context *c, int nocc, int ldc, const double *z, int64_t sz, int ldz, const double *v,
               const double *P, double alpha, double beta, double *dl, int64_t ldd, int64_t jn,
               const float *cb32) {
   const int oc16 = (nocc + 15) / 16;
-  const int64_t rowbytes = int64_t(c->n) * (c->n + 1) / 2 * int64_t(sizeof(double));
-  if (oc16 > 8 || int64_t(c->n_pad) * ldc >= (int64_t(1) << 28) || rowbytes >= (int64_t(1) << 31))
-    return MJK_ERR_ARG;
+  if (!grad_rows_fits(c, nocc, ldc)) return MJK_ERR_ARG;
   if (jn < 1) return MJK_OK;
   GradRows g{};
-  g.rowbytes = int32_t(rowbytes);
+  g.rowbytes = int32_t(int64_t(c->n) * (c->n + 1) / 2 * int64_t(sizeof(double)));
   g.cb = static_cast<const double *>(c->w->cbank.ptr);
   g.cb32 = cb32;
   g.ldc = ldc;
@@ -936,7 +942,7 @@ int grad_vectors_z_impl(Context *c, int32_t nocc, const double *corb, const doub
   if ((rc = pack_bank(c, 1, &nocc, sets, off, opad, &ot, &tot))) return rc;
   const int ldc = 16 * ot;
   const double *cb = static_cast<const double *>(c->w->cbank.ptr);
-  if (nocc <= 128) {  // one fused launch per precision: W = C Z stays on chip
+  if (grad_rows_fits(c, nocc, ldc)) {  // one fused launch per precision: W = C Z stays on chip
     const double *P = nullptr;
     if ((rc = density_or(c, nocc, corb, p_lt, &P))) return rc;
     const float *cb32 = nullptr;
